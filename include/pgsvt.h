@@ -1,0 +1,143 @@
+/*
+ * pgsvt.h -- C ABI of the MI355X-native PGURE-SVT patchwise singular-value
+ * thresholding path (library: pgure-svt_amd/pguresvt/libpgsvt.so).
+ *
+ * Plain pointers and sizes only.  All volumes are column-major "cubes" in the
+ * reference's Armadillo layout: element (row r, col c, frame f) lives at
+ * r + n_rows * (c + n_cols * f).
+ *
+ * Every entry point returns 0 on success or a negative PGSVT_E* code; the
+ * message of the last failure on the calling thread is in pgsvt_last_error().
+ */
+#ifndef PGSVT_H
+#define PGSVT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Input element types: the four instantiations PGURESVT<T1,double> of the
+ * reference (pguresvt/_pguresvt.pyx:143-148, 181, 240, 299, 358). */
+enum pgsvt_dtype { PGSVT_U8 = 0, PGSVT_U16 = 1, PGSVT_F32 = 2, PGSVT_F64 = 3 };
+
+enum pgsvt_status {
+  PGSVT_OK = 0,
+  PGSVT_E_ARG = -1,      /* invalid argument / shape the reference cannot run */
+  PGSVT_E_HIP = -2,      /* HIP runtime failure (allocation, launch, copy)   */
+  PGSVT_E_NUMERIC = -3,  /* non-finite input or optimiser failure            */
+  PGSVT_E_NOGPU = -4     /* no usable gfx950 device                          */
+};
+
+/* The 19 scalar parameters of the reference driver
+ * PGURESVT(Y, estimates, X, trajLength, blockSize, blockOverlap, motionWindow,
+ *          medianSize, noiseMethod, maxIter, nJobs, randomSeed, optimizePGURE,
+ *          expWeighting, motionEstimation, lambdaEst, alphaEst, muEst,
+ *          sigmaEst, tol)                       (src/pguresvt.hpp:17-38)
+ * with the Python keyword names of pguresvt_{u8,u16,f,d}
+ * (pguresvt/_pguresvt.pyx:151-168) and the same sentinels (-1 = estimate). */
+typedef struct pgsvt_params {
+  uint32_t trajectory_length;     /* trajLength, default 15                  */
+  uint32_t patch_size;            /* blockSize, default 4                    */
+  uint32_t patch_overlap;         /* blockOverlap (patch step), default 1    */
+  uint32_t motion_window;         /* ARPS search half-width, default 7       */
+  int64_t motion_filter;          /* medianSize; <=0 disables, default 5     */
+  uint32_t noise_method;          /* 1..4, default 4                         */
+  uint32_t max_iter;              /* optimiser evaluation cap, default 500   */
+  int64_t n_jobs;                 /* host threads, -1 = all                  */
+  int64_t random_seed;            /* pcg64 seed; <0 = nondeterministic       */
+  int32_t optimize_pgure;         /* bool, default 1                         */
+  int32_t exponential_weighting;  /* bool, default 1                         */
+  int32_t motion_estimation;      /* bool, default 1                         */
+  double lambda1;                 /* lambdaEst; start / fixed threshold      */
+  double noise_alpha;             /* alphaEst, <0 = estimate                 */
+  double noise_mu;                /* muEst,    <0 = estimate                 */
+  double noise_sigma;             /* sigmaEst, <0 = estimate                 */
+  double tol;                     /* relative f tolerance, default 1e-7      */
+} pgsvt_params;
+
+/* Fill *p with the defaults of pguresvt_{u8,u16,f,d}
+ * (pguresvt/_pguresvt.pyx:151-168). */
+void pgsvt_default_params(pgsvt_params* p);
+
+/* Drop-in for PGURESVT<T1,double> (src/pguresvt.hpp:17-172) as bound by the
+ * Cython entry points pguresvt_{u8,u16,f,d} (pguresvt/_pguresvt.pyx:151-384).
+ *   X          input cube, n_rows x n_cols x n_frames, element type `dtype`
+ *   Y          caller-allocated output cube of the same shape (float64)
+ *   estimates  caller-allocated n_frames x 4 column-major matrix (float64):
+ *              column 0 = lambda, 1 = alpha, 2 = mu, 3 = sigma per frame
+ *              (src/pguresvt.hpp:150-153)
+ * Runs on the current HIP device.  Deviation from the reference: inputs the
+ * reference aborts on (frames < window, non-square frames, NaN) return
+ * PGSVT_E_ARG / PGSVT_E_NUMERIC instead of terminating the process. */
+int pgsvt_denoise(const void* X, int dtype, uint32_t n_rows, uint32_t n_cols,
+                  uint32_t n_frames, const pgsvt_params* p, double* Y,
+                  double* estimates);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* pgsvt_last_error(void);
+
+/* Library version string. */
+const char* pgsvt_version(void);
+
+/* ------------------------------------------------------------------------
+ * Window-level entry points.  They expose the stages of one time window of
+ * the driver (src/pguresvt.hpp:90-166) so a caller can check each stage on
+ * its own; `t` is the output-frame index whose window is processed.
+ * Window size Nw = 2*floor(Nt/2)+1 with Nt the clamped trajectory length
+ * (src/pguresvt.hpp:57-58); P_all = (n_rows-bs+1)*(n_cols-bs+1).
+ * ------------------------------------------------------------------------ */
+
+/* ARPS trajectories of window t: MotionEstimator<T>::Estimate
+ * (src/arps.hpp:52-134).  patches: int32 [Nw][P_all][2] with [..][0] = row,
+ * [..][1] = col, i.e. the reference icube(2, P_all, Nw) in its own layout. */
+int pgsvt_window_trajectories(const void* X, int dtype, uint32_t n_rows,
+                              uint32_t n_cols, uint32_t n_frames,
+                              const pgsvt_params* p, uint32_t t,
+                              int32_t* patches);
+
+/* PGURE risk of window t at n_lambda thresholds: PGURE<T>::CalculatePGURE
+ * (src/pgure.hpp:120-137) after the constructor (src/pgure.hpp:26-93).
+ * noise[3] = (alpha, mu, sigma) exactly as the driver holds them before the
+ * constructor call (src/pguresvt.hpp:133 passes them as alpha, sigma, mu). */
+int pgsvt_window_risk(const void* X, int dtype, uint32_t n_rows,
+                      uint32_t n_cols, uint32_t n_frames,
+                      const pgsvt_params* p, uint32_t t, const double* noise,
+                      const double* lambdas, uint32_t n_lambda, double* risk);
+
+/* Thresholded reconstruction of the whole window t at lambda:
+ * SVT<T>::Reconstruct (src/svt.hpp:121-167) of the unperturbed data, in
+ * normalised units (before the `*= uMax` of src/pguresvt.hpp:147).
+ * vol: float64 n_rows x n_cols x Nw, column-major. */
+int pgsvt_window_reconstruct(const void* X, int dtype, uint32_t n_rows,
+                             uint32_t n_cols, uint32_t n_frames,
+                             const pgsvt_params* p, uint32_t t, double lambda,
+                             double* vol);
+
+/* Singular values of every selected patch of window t
+ * (arma::svd_econ at src/svt.hpp:111): float64 [P][Nw], each row sorted
+ * descending.  Returns the patch count P through *n_patches. */
+int pgsvt_window_singular_values(const void* X, int dtype, uint32_t n_rows,
+                                 uint32_t n_cols, uint32_t n_frames,
+                                 const pgsvt_params* p, uint32_t t,
+                                 double* s, uint32_t* n_patches);
+
+/* Poisson-Gaussian noise estimate of a normalised window u (float64 cube
+ * n_rows x n_cols x n_t): NoiseEstimator::Estimate (src/noise.hpp:35-153).
+ * On entry alpha, mu and sigma hold user values (<0 = estimate). */
+int pgsvt_noise_estimate(const double* u, uint32_t n_rows, uint32_t n_cols,
+                         uint32_t n_t, uint32_t method, double* alpha,
+                         double* mu, double* sigma);
+
+/* The delta2 perturbation signs the GPU generates: PGURE<T>::
+ * GenerateRandomPerturbations (src/pgure.hpp:167-186) with pcg64 seeded by
+ * `seed` (src/pgure.hpp:52-59).  out[v] = 1 where delta2[v] is negative
+ * (-sqrt(vQ/vP)), 0 where it is +sqrt(vP/vQ); n_vox values. */
+int pgsvt_delta2_signs(int64_t seed, uint64_t n_vox, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PGSVT_H */

@@ -1,0 +1,933 @@
+// host.cpp -- C ABI and host orchestration of the MI355X PGURE-SVT path.
+//
+// One call of pgsvt_denoise does what PGURESVT<T1,double> does
+// (src/pguresvt.hpp:17-172), window by window, but windows are processed in
+// batches that live in HBM together:
+//   1. upload X, per-frame maxima, optional median pre-filter (GPU);
+//   2. noise estimation per window on host threads (overlaps the GPU work);
+//   3. per batch: normalised volumes, ARPS trajectories, overlap counts and
+//      the batched Jacobi SVD of every patch of every window (GPU);
+//   4. the lambda searches of all windows of the batch advance in lockstep:
+//      each round evaluates one lambda per still-searching window in one
+//      reconstruct + risk launch, then every search takes its next step;
+//   5. the final reconstruction of each window's reference frame at its
+//      lambda goes straight into the output cube.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <future>
+#include <memory>
+#include <mutex>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/pgsvt.h"
+#include "common.hpp"
+#include "kernels.hpp"
+#include "noise.hpp"
+#include "sbplx.hpp"
+
+namespace pgsvt {
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+struct HipError {
+  hipError_t e;
+  const char* what;
+};
+
+inline void hipchk(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw HipError{e, what};
+}
+
+// ---- kernel timing (HIP events on the library stream), off by default ----
+struct ProfEntry {
+  std::string name;
+  uint64_t launches = 0;
+  double ms = 0, units = 0;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfEntry> g_prof;
+
+struct ProfPending {
+  int idx;
+  hipEvent_t a, b;
+  double units;
+};
+
+class Profiler {
+ public:
+  explicit Profiler(hipStream_t s) : s_(s) {}
+  ~Profiler() { flush(); }
+  // time `launch` (which enqueues kernels on the stream) under `name`
+  template <class F>
+  void run(const char* name, double units, F launch) {
+    if (!g_prof_on) {
+      launch();
+      return;
+    }
+    hipEvent_t a, b;
+    hipchk(hipEventCreate(&a), "event");
+    hipchk(hipEventCreate(&b), "event");
+    hipchk(hipEventRecord(a, s_), "event record");
+    launch();
+    hipchk(hipEventRecord(b, s_), "event record");
+    int idx;
+    {
+      std::lock_guard<std::mutex> g(g_prof_mu);
+      idx = -1;
+      for (size_t k = 0; k < g_prof.size(); ++k)
+        if (g_prof[k].name == name) idx = (int)k;
+      if (idx < 0) {
+        g_prof.push_back(ProfEntry{name});
+        idx = (int)g_prof.size() - 1;
+      }
+    }
+    pend_.push_back({idx, a, b, units});
+  }
+  // read the events of everything already synchronised
+  void flush() {
+    if (pend_.empty()) return;
+    (void)hipStreamSynchronize(s_);
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (auto& p : pend_) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess && p.idx < (int)g_prof.size()) {
+        g_prof[p.idx].launches++;
+        g_prof[p.idx].ms += ms;
+        g_prof[p.idx].units += p.units;
+      }
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    pend_.clear();
+  }
+
+ private:
+  hipStream_t s_;
+  std::vector<ProfPending> pend_;
+};
+
+// device buffer owning its allocation
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() {}
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    hipchk(hipMalloc((void**)&p, count * sizeof(T)), "hipMalloc");
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+struct Geometry {
+  uint32_t Nt, fw, Nw;
+};
+
+Geometry geometry(const pgsvt_params& p) {
+  Geometry g;
+  g.Nt = (p.patch_size * p.patch_size < p.trajectory_length) ? p.patch_size * p.patch_size - 1 : p.trajectory_length;
+  g.fw = (uint32_t)std::floor(g.Nt / 2);
+  g.Nw = 2 * g.fw + 1;
+  return g;
+}
+
+double elem(const void* X, int dtype, size_t k) {
+  switch (dtype) {
+    case PGSVT_U8: return (double)((const uint8_t*)X)[k];
+    case PGSVT_U16: return (double)((const uint16_t*)X)[k];
+    case PGSVT_F32: return (double)((const float*)X)[k];
+    default: return ((const double*)X)[k];
+  }
+}
+
+size_t dtype_size(int dtype) {
+  switch (dtype) {
+    case PGSVT_U8: return 1;
+    case PGSVT_U16: return 2;
+    case PGSVT_F32: return 4;
+    default: return 8;
+  }
+}
+
+// patch index set (src/svt.hpp:61-92), square frames
+std::vector<uint32_t> patch_indices(uint32_t N, uint32_t bs, uint32_t ov) {
+  const uint32_t n = N - bs, nd = n / ov;
+  std::vector<uint32_t> j;
+  j.reserve((size_t)(nd + 1) * (nd + 1) + 2 * (nd + 1));
+  for (uint32_t i = 0; i < 1 + n; i += ov)
+    for (uint32_t k = 0; k < 1 + n; k += ov) j.push_back(i * n + k);
+  for (uint32_t i = 0; i < 1 + n; i += ov) j.push_back((n + 1) * n + i);  // right edge
+  for (uint32_t i = 0; i < 1 + n; i += ov) j.push_back((n + 1) * i + n);  // bottom edge
+  std::sort(j.begin(), j.end());
+  j.erase(std::unique(j.begin(), j.end()), j.end());
+  return j;
+}
+
+// Window placement (src/pguresvt.hpp:100-114, 155-166).
+void window_of(uint32_t t, uint32_t nf, const Geometry& g, uint32_t& s0, uint32_t& ref) {
+  if (t < g.fw) {
+    s0 = 0;
+    ref = t;
+  } else if (t >= nf - g.fw) {
+    s0 = nf - 2 * g.fw - 1;
+    ref = t - (nf - g.Nt);
+  } else {
+    s0 = t - g.fw;
+    ref = g.fw;
+  }
+}
+
+// ARPS schedule of one window (src/arps.hpp:52-134) as jobs of at most two
+// passes; the second pass of a job reads the motion slot the first wrote.
+struct Pass {
+  int src, dst, slot;
+};
+
+bool arps_jobs(uint32_t t, uint32_t nf, const Geometry& g, std::vector<int16_t>& jobs) {
+  const uint32_t Nt = g.Nt, tw = g.fw;
+  std::vector<Pass> ps;
+  auto add = [&](int, int a, int b, int c) { ps.push_back({a, b, c}); };
+  if (t < tw) {
+    const uint32_t loopEnd = Nt - t - 1;
+    for (uint32_t i = 0; i < loopEnd; ++i) add(i, t + i, t + i + 1, t + i);
+    for (uint32_t i = 0; i < t; ++i) {
+      const int n = -1 * (int)(i + 1);
+      add(n, t + n + 1, t + n, t + n + 1);
+    }
+  } else if (t >= nf - tw) {
+    const uint32_t endFrame = t - (nf - Nt);
+    const uint32_t loopEnd = 2 * tw - endFrame;
+    for (uint32_t i = 0; i < loopEnd; ++i) add(i, endFrame + i, endFrame + i + 1, endFrame + i);
+    for (uint32_t i = 0; i < endFrame; ++i) {
+      const int n = -1 * (int)(i + 1);
+      if (2 * tw == endFrame)
+        add(n, endFrame + n + 1, endFrame + n, endFrame + n);
+      else
+        add(n, endFrame + n + 1, endFrame + n, endFrame + n + 1);
+    }
+  } else {
+    for (uint32_t i = 0; i < tw; ++i) add(i, tw + i, tw + i + 1, tw + i);
+    for (uint32_t i = 0; i < tw; ++i) {
+      const int n = -1 * (int)(i + 1);
+      add(n, tw + n + 1, tw + n, tw + n + 1);
+    }
+  }
+  // dependency: latest earlier pass writing the same slot
+  std::vector<int> dep(ps.size(), -1), dependents(ps.size(), 0);
+  for (size_t k = 0; k < ps.size(); ++k)
+    for (size_t m = 0; m < k; ++m)
+      if (ps[m].slot == ps[k].slot) dep[k] = (int)m;
+  for (size_t k = 0; k < ps.size(); ++k)
+    if (dep[k] >= 0) {
+      if (dep[dep[k]] >= 0) return false;  // chains longer than two
+      if (++dependents[dep[k]] > 1) return false;
+    }
+  jobs.clear();
+  for (size_t k = 0; k < ps.size(); ++k) {
+    if (dep[k] >= 0) continue;
+    int16_t sb = -1, db = -1;
+    for (size_t m = k + 1; m < ps.size(); ++m)
+      if (dep[m] == (int)k) {
+        sb = (int16_t)ps[m].src;
+        db = (int16_t)ps[m].dst;
+      }
+    jobs.push_back((int16_t)ps[k].src);
+    jobs.push_back((int16_t)ps[k].dst);
+    jobs.push_back(sb);
+    jobs.push_back(db);
+  }
+  return true;
+}
+
+int check_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(PGSVT_E_NOGPU, "no HIP device");
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return fail(PGSVT_E_NOGPU, "no HIP device properties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(PGSVT_E_NOGPU, std::string("this library is built for gfx950, device is ") + prop.gcnArchName);
+  return 0;
+}
+
+int validate(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, const pgsvt_params* p) {
+  if (!X || !p) return fail(PGSVT_E_ARG, "null pointer");
+  if (dtype < 0 || dtype > 3) return fail(PGSVT_E_ARG, "invalid dtype");
+  if (nr != nc) return fail(PGSVT_E_ARG, "frames must be square: the reference patch tables assume n_rows == n_cols");
+  if (nr > 65535) return fail(PGSVT_E_ARG, "frame side above 65535");
+  if (p->patch_size < 2 || p->patch_size > 8 || p->patch_size > nr)
+    return fail(PGSVT_E_ARG, "patch_size must be in [2, 8] and fit the frame");
+  if (p->patch_overlap < 1 || p->patch_overlap > p->patch_size) return fail(PGSVT_E_ARG, "invalid patch_overlap");
+  if (p->trajectory_length < 1 || p->trajectory_length % 2 == 0)
+    return fail(PGSVT_E_ARG, "trajectory_length must be a positive odd integer");
+  const Geometry g = geometry(*p);
+  if (g.Nt < 1 || nf < g.Nw) return fail(PGSVT_E_ARG, "fewer frames than the temporal window");
+  if (p->motion_estimation && (p->motion_window < 3 || p->motion_window % 2 == 0 || p->motion_window > 15))
+    return fail(PGSVT_E_ARG, "motion_window must be odd, >= 3 and <= 15");
+  if (!svd_shape_supported(p->patch_size, g.Nw)) return fail(PGSVT_E_ARG, "unsupported (patch_size, window) shape");
+  if (p->optimize_pgure && p->noise_method < 1) return fail(PGSVT_E_ARG, "invalid noise_method");
+  return 0;
+}
+
+uint64_t seed_of(const pgsvt_params& p) {
+  if (p.random_seed >= 0) return (uint64_t)p.random_seed;
+  std::random_device rd;
+  return ((uint64_t)rd() << 32) ^ rd();
+}
+
+// ------------------------------------------------------- host job pool --
+// Noise estimation and lambda start points of every window, computed on
+// host threads while the GPU works; results are awaited per window.
+struct WindowPrep {
+  double alpha = -1, mu = -1, sigma = -1, start = 0;
+};
+
+class PrepPool {
+ public:
+  template <class F>
+  PrepPool(uint32_t count, uint32_t threads, F fn) : res_(count), ready_(count, 0) {
+    fn_ = fn;
+    for (uint32_t k = 0; k < std::max(1u, threads); ++k)
+      th_.emplace_back([this, count] {
+        for (;;) {
+          const uint32_t t = next_.fetch_add(1);
+          if (t >= count) return;
+          WindowPrep r = fn_(t);
+          {
+            std::lock_guard<std::mutex> g(mu_);
+            res_[t] = r;
+            ready_[t] = 1;
+          }
+          cv_.notify_all();
+        }
+      });
+  }
+  ~PrepPool() {
+    for (auto& t : th_) t.join();
+  }
+  WindowPrep wait(uint32_t t) {
+    std::unique_lock<std::mutex> g(mu_);
+    cv_.wait(g, [&] { return ready_[t] != 0; });
+    return res_[t];
+  }
+
+ private:
+  std::function<WindowPrep(uint32_t)> fn_;
+  std::vector<WindowPrep> res_;
+  std::vector<char> ready_;
+  std::vector<std::thread> th_;
+  std::atomic<uint32_t> next_{0};
+  std::mutex mu_;
+  std::condition_variable cv_;
+};
+
+// ------------------------------------------------------------ pipeline --
+struct Pipeline {
+  const void* X = nullptr;
+  int dtype = 0;
+  uint32_t nr = 0, nc = 0, nf = 0;
+  pgsvt_params p{};
+  Geometry g{};
+  Shape sh{};
+  bool optimize = false, median = false;
+  uint32_t nvar = 1;
+  double eps1 = 1e-4, eps2 = 1e-2, d2neg = 0, d2pos = 0;
+  uint64_t seed = 0;
+  std::vector<uint32_t> sel;
+  std::vector<double> fmaxX, fmaxZ;
+  hipStream_t s = nullptr;
+
+  DBuf<char> raw;
+  DBuf<double> Xd, Zd, Yd;
+  double* Yout = nullptr;           // final frames (Yd.p or a caller device buffer)
+  std::vector<char> host_copy;      // host copy of a device-resident input
+  std::unique_ptr<Profiler> prof;
+  DBuf<uint8_t> d2;
+  DBuf<uint32_t> sel_d;
+  // batch buffers
+  uint32_t NB = 0;
+  DBuf<WinMeta> meta_d;
+  DBuf<uint32_t> wl_d;
+  DBuf<double> u64, w64, partial, sums;
+  DBuf<float> u32, factors;
+  DBuf<int32_t> pos, W;
+  DBuf<int16_t> jobs_d;
+  DBuf<unsigned long long> acc;
+  uint32_t max_jobs = 0, risk_blocks = 0;
+  DBuf<uint32_t> stats;
+
+  ~Pipeline() {
+    prof.reset();
+    if (s) (void)hipStreamDestroy(s);
+  }
+
+  // X_ is a host pointer, or a device pointer when on_device is set; the
+  // host-side noise estimator then works on a host copy
+  void setup(const void* X_, int dtype_, uint32_t nr_, uint32_t nc_, uint32_t nf_, const pgsvt_params& p_,
+             bool on_device = false) {
+    X = X_;
+    dtype = dtype_;
+    nr = nr_;
+    nc = nc_;
+    nf = nf_;
+    p = p_;
+    g = geometry(p);
+    optimize = p.optimize_pgure != 0;
+    median = p.motion_filter > 0;
+    nvar = optimize ? 3 : 1;
+    sh.nr = nr;
+    sh.nc = nc;
+    sh.fs = nr * nc;
+    sh.nw = g.Nw;
+    sh.vol = g.Nw * sh.fs;
+    sh.bs = p.patch_size;
+    sh.m = sh.bs * sh.bs;
+    sh.n1 = nr - sh.bs + 1;
+    sh.p_all = sh.n1 * sh.n1;
+    sel = patch_indices(nr, sh.bs, p.patch_overlap);
+    sh.p_sel = (uint32_t)sel.size();
+    sh.fstride = (g.Nw * g.Nw + g.Nw + 3) & ~3u;
+    const double vP = 0.5 + 0.5 * 1. / std::sqrt(1. * 1. + 4);
+    const double vQ = 1 - vP;
+    d2neg = -1 * std::sqrt(vQ / vP);
+    d2pos = std::sqrt(vP / vQ);
+    seed = seed_of(p);
+
+    hipchk(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    prof.reset(new Profiler(s));
+    const size_t N = (size_t)sh.fs * nf;
+    const void* src = X;
+    if (on_device) {
+      if (optimize) {  // the noise estimator runs on the host
+        host_copy.resize(N * dtype_size(dtype));
+        hipchk(hipMemcpyAsync(host_copy.data(), X, host_copy.size(), hipMemcpyDeviceToHost, s), "D2H X");
+        X = host_copy.data();
+      }
+    } else {
+      raw.alloc(N * dtype_size(dtype));
+      hipchk(hipMemcpyAsync(raw.p, X, raw.bytes(), hipMemcpyHostToDevice, s), "H2D X");
+      src = raw.p;
+    }
+    Xd.alloc(N);
+    hipchk(launch_to_double(src, dtype, Xd.p, N, s), "to_double");
+    DBuf<double> fm;
+    fm.alloc(nf);
+    fmaxX.resize(nf);
+    hipchk(launch_frame_max(Xd.p, sh.fs, nf, fm.p, s), "frame_max");
+    hipchk(hipMemcpyAsync(fmaxX.data(), fm.p, nf * sizeof(double), hipMemcpyDeviceToHost, s), "D2H max");
+    if (median) {
+      Zd.alloc(N);
+      prof->run("median_filter", (double)N, [&] {
+        hipchk(launch_median(Xd.p, Zd.p, nr, nc, nf, (int)p.motion_filter, s), "median");
+      });
+      fmaxZ.resize(nf);
+      hipchk(launch_frame_max(Zd.p, sh.fs, nf, fm.p, s), "frame_max Z");
+      hipchk(hipMemcpyAsync(fmaxZ.data(), fm.p, nf * sizeof(double), hipMemcpyDeviceToHost, s), "D2H max");
+    }
+    if (optimize) {
+      d2.alloc(sh.vol);
+      hipchk(launch_delta2((int64_t)seed, sh.vol, d2.p, s), "delta2");
+    }
+    sel_d.alloc(sel.size());
+    hipchk(hipMemcpyAsync(sel_d.p, sel.data(), sel.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D sel");
+    hipchk(hipStreamSynchronize(s), "setup sync");
+    raw.release();
+  }
+
+  WinMeta meta_of(uint32_t t) const {
+    WinMeta m{};
+    m.t = t;
+    window_of(t, nf, g, m.s0, m.ref);
+    double um = fmaxX[m.s0], wm = median ? fmaxZ[m.s0] : 0;
+    for (uint32_t k = 1; k < g.Nw; ++k) {
+      um = std::max(um, fmaxX[m.s0 + k]);
+      if (median) wm = std::max(wm, fmaxZ[m.s0 + k]);
+    }
+    m.uMax = um;
+    m.wMax = median ? wm : um;
+    m.lambda = 0;
+    return m;
+  }
+
+  size_t bytes_per_window() const {
+    size_t b = 0;
+    b += 8ull * sh.vol;                                  // u64
+    if (median) b += 8ull * sh.vol;                      // w64
+    b += 4ull * sh.vol * nvar;                           // u32
+    b += 4ull * sh.nw * sh.p_all;                        // pos
+    b += 4ull * sh.vol;                                  // W
+    b += 8ull * sh.vol * nvar;                           // acc
+    b += 4ull * nvar * (size_t)sh.p_sel * sh.fstride;    // factors
+    return b;
+  }
+
+  void alloc_batch(uint32_t want) {
+    size_t freeb = 0, total = 0;
+    hipchk(hipMemGetInfo(&freeb, &total), "hipMemGetInfo");
+    const size_t budget = std::min<size_t>((size_t)(freeb * 0.6), 64ull << 30);
+    const size_t per = bytes_per_window();
+    NB = (uint32_t)std::max<size_t>(1, std::min<size_t>(want, budget / per));
+    if (const char* e = std::getenv("PGSVT_BATCH")) NB = std::max(1u, std::min<uint32_t>(NB, (uint32_t)atoi(e)));
+    meta_d.alloc(NB);
+    wl_d.alloc(NB);
+    u64.alloc((size_t)NB * sh.vol);
+    if (median) w64.alloc((size_t)NB * sh.vol);
+    u32.alloc((size_t)NB * sh.vol * nvar);
+    pos.alloc((size_t)NB * sh.nw * sh.p_all);
+    W.alloc((size_t)NB * sh.vol);
+    acc.alloc((size_t)NB * sh.vol * nvar);
+    factors.alloc((size_t)NB * nvar * sh.p_sel * sh.fstride);
+    risk_blocks = std::min<uint32_t>(256, (sh.vol + 255) / 256);
+    partial.alloc((size_t)NB * risk_blocks * 4);
+    sums.alloc((size_t)NB * 4);
+    max_jobs = 2 * g.Nw;
+    jobs_d.alloc((size_t)NB * max_jobs * 4);
+    stats.alloc(1);
+    hipchk(hipMemsetAsync(acc.p, 0, acc.bytes(), s), "memset acc");
+    hipchk(hipMemsetAsync(stats.p, 0, stats.bytes(), s), "memset stats");
+  }
+
+  // volumes, trajectories, counts and factors of the windows in `ms`
+  void decompose(std::vector<WinMeta>& ms, uint32_t nvar_run) {
+    const uint32_t nb = (uint32_t)ms.size();
+    hipchk(hipMemcpyAsync(meta_d.p, ms.data(), nb * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
+    hipchk(launch_prepare(Xd.p, median ? Zd.p : nullptr, meta_d.p, nb, d2.p, sh, nvar, eps2, d2neg, d2pos, u64.p,
+                          median ? w64.p : nullptr, u32.p, s),
+           "prepare");
+    hipchk(hipMemsetAsync(pos.p, 0, (size_t)nb * sh.nw * sh.p_all * sizeof(int32_t), s), "memset pos");
+    hipchk(launch_fill_ref(meta_d.p, nb, sh, pos.p, s), "fill_ref");
+    if (p.motion_estimation) {
+      std::vector<int16_t> all((size_t)nb * max_jobs * 4, (int16_t)-1);
+      for (uint32_t b = 0; b < nb; ++b) {
+        std::vector<int16_t> j;
+        if (!arps_jobs(ms[b].t, nf, g, j)) throw HipError{hipErrorInvalidValue, "unsupported ARPS schedule"};
+        std::copy(j.begin(), j.end(), all.begin() + (size_t)b * max_jobs * 4);
+      }
+      hipchk(hipMemcpyAsync(jobs_d.p, all.data(), all.size() * sizeof(int16_t), hipMemcpyHostToDevice, s),
+             "H2D jobs");
+      hipchk(hipStreamSynchronize(s), "jobs sync");  // `all` is pageable and local
+      // searches: one per (window pass, block)
+      prof->run("arps", (double)nb * (g.Nw - 1) * sh.p_all, [&] {
+        hipchk(launch_arps(median ? w64.p : u64.p, jobs_d.p, max_jobs, nb, sh, (int)p.motion_window, pos.p, s),
+               "arps");
+      });
+    }
+    hipchk(hipMemsetAsync(W.p, 0, (size_t)nb * sh.vol * sizeof(int32_t), s), "memset W");
+    hipchk(launch_count(pos.p, sel_d.p, nb, sh, W.p, s), "count");
+    // units: matrices decomposed
+    prof->run("svd_jacobi", (double)nb * nvar_run * sh.p_sel, [&] {
+      hipchk(launch_svd(u32.p, pos.p, sel_d.p, nb, nvar_run, sh, factors.p, 30, 1e-6f, stats.p, s), "svd");
+    });
+  }
+
+  // PGURE risk (src/pgure.hpp:136) from the GPU sums; noise as the driver
+  // holds it (alpha, mu, sigma) before the swapped constructor call.
+  double risk(const double* S, const WindowPrep& w) const {
+    const double alpha = w.alpha, mup = w.sigma, sigp = w.mu;  // constructor receives (alpha, sigma, mu)
+    const double sigmasq = sigp * sigp;
+    const double OoN = 1.0 / (sh.nr * sh.nc * sh.nw);
+    const double t3 = 0.0;  // delta1 term: U1 == U (integer delta1 cube)
+    return OoN * (S[0] - (alpha + mup) * S[1] + (2 / eps1 * t3) - (2 * sigmasq * alpha / (eps2 * eps2) * S[2]) +
+                  (2 * mup * S[3]) + mup) -
+           sigmasq;
+  }
+
+  // evaluate the risk of the listed batch slots at their meta.lambda
+  void eval_risk(std::vector<WinMeta>& ms, const std::vector<uint32_t>& list, std::vector<double>& out) {
+    const uint32_t na = (uint32_t)list.size();
+    hipchk(hipMemcpyAsync(meta_d.p, ms.data(), ms.size() * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
+    hipchk(hipMemcpyAsync(wl_d.p, list.data(), na * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
+    // units: thresholded patch reconstructions (patch x variant)
+    prof->run("recon_overlap_add", (double)na * 3 * sh.p_sel, [&] {
+      hipchk(launch_recon(factors.p, u32.p, pos.p, sel_d.p, meta_d.p, wl_d.p, na, nvar, 3, sh,
+                          p.exponential_weighting, 0, acc.p, s),
+             "recon");
+    });
+    // units: voxels
+    prof->run("pgure_risk", (double)na * sh.vol, [&] {
+      hipchk(launch_risk(acc.p, W.p, u64.p, d2.p, wl_d.p, na, sh, d2neg, d2pos, partial.p, risk_blocks, sums.p, s),
+             "risk");
+    });
+    out.resize((size_t)na * 4);
+    hipchk(hipMemcpyAsync(out.data(), sums.p, (size_t)na * 4 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H sums");
+    hipchk(hipStreamSynchronize(s), "risk sync");
+    prof->flush();
+  }
+
+  // final frames of the batch into Yd
+  void finish(std::vector<WinMeta>& ms) {
+    const uint32_t nb = (uint32_t)ms.size();
+    std::vector<uint32_t> list(nb);
+    for (uint32_t b = 0; b < nb; ++b) list[b] = b;
+    hipchk(hipMemcpyAsync(meta_d.p, ms.data(), nb * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
+    hipchk(hipMemcpyAsync(wl_d.p, list.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
+    prof->run("recon_final_frame", (double)nb * sh.p_sel, [&] {
+      hipchk(launch_recon(factors.p, u32.p, pos.p, sel_d.p, meta_d.p, wl_d.p, nb, nvar, 1, sh,
+                          p.exponential_weighting, 1, acc.p, s),
+             "recon final");
+    });
+    hipchk(launch_output(acc.p, W.p, meta_d.p, nb, nvar, sh, Yout, s), "output");
+    hipchk(hipStreamSynchronize(s), "finish sync");
+    prof->flush();
+  }
+
+  // normalised window t on the host (for the noise estimator) and accu(u)
+  void host_window(uint32_t t, std::vector<double>& u, double& umax, double& accu) const {
+    const WinMeta m = meta_of(t);
+    u.resize(sh.vol);
+    const size_t base = (size_t)m.s0 * sh.fs;
+    for (size_t k = 0; k < sh.vol; ++k) u[k] = elem(X, dtype, base + k) / m.uMax;
+    umax = m.uMax;
+    double a1 = 0, a2 = 0;
+    size_t i = 0;
+    for (; i + 1 < u.size(); i += 2) {
+      a1 += u[i];
+      a2 += u[i + 1];
+    }
+    if (i < u.size()) a1 += u[i];
+    accu = a1 + a2;
+  }
+};
+
+uint32_t thread_count(int64_t n_jobs) {
+  if (n_jobs == 0 || n_jobs == 1) return 1;
+  if (n_jobs > 0) return (uint32_t)n_jobs;
+  return std::max(1u, std::thread::hardware_concurrency());
+}
+
+int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, const pgsvt_params* pp, double* Y,
+                double* est, bool on_device) {
+  if (int rc = validate(X, dtype, nr, nc, nf, pp)) return rc;
+  if (int rc = check_device()) return rc;
+  Pipeline P;
+  P.setup(X, dtype, nr, nc, nf, *pp, on_device);
+  const pgsvt_params& p = P.p;
+  const Geometry& g = P.g;
+  const double lambda0 = (p.lambda1 >= 0.0) ? p.lambda1 : -1.0;
+  const double alpha0 = (p.noise_alpha >= 0.0) ? p.noise_alpha : -1.0;
+  const double mu0 = (p.noise_mu >= 0.0) ? p.noise_mu : -1.0;
+  const double sigma0 = (p.noise_sigma >= 0.0) ? p.noise_sigma : -1.0;
+  if (on_device) {
+    P.Yout = Y;
+  } else {
+    P.Yd.alloc((size_t)P.sh.fs * nf);
+    P.Yout = P.Yd.p;
+  }
+  P.alloc_batch(nf);
+
+  LeafCache cache;
+  std::unique_ptr<PrepPool> pool;
+  if (P.optimize) {
+    pool.reset(new PrepPool(nf, thread_count(p.n_jobs), [&](uint32_t t) {
+      WindowPrep w;
+      w.alpha = alpha0;
+      w.mu = mu0;
+      w.sigma = sigma0;
+      std::vector<double> u;
+      double umax = 0, acc = 0;
+      P.host_window(t, u, umax, acc);
+      std::vector<uint32_t> ids(g.Nw);
+      const WinMeta m = P.meta_of(t);
+      for (uint32_t k = 0; k < g.Nw; ++k) ids[k] = m.s0 + k;
+      estimate_noise(u.data(), nr, g.Nw, p.noise_method, w.alpha, w.mu, w.sigma, &cache, ids.data(), umax);
+      const double OoNxNyNt = 1.0 / (nc * nr * g.Nt);
+      double start = (lambda0 >= 0.0) ? lambda0 : acc * OoNxNyNt;
+      w.start = std::max(0.0, start);
+      return w;
+    }));
+  }
+
+  int status = 0;
+  std::string warn;
+  for (uint32_t t0 = 0; t0 < nf; t0 += P.NB) {
+    const uint32_t nb = std::min(P.NB, nf - t0);
+    std::vector<WinMeta> ms(nb);
+    for (uint32_t b = 0; b < nb; ++b) ms[b] = P.meta_of(t0 + b);
+    P.decompose(ms, P.nvar);
+    std::vector<WindowPrep> prep(nb);
+    if (P.optimize) {
+      std::vector<std::unique_ptr<Sbplx1D>> opt(nb);
+      std::vector<int> code(nb, Sbplx1D::kNeed);
+      for (uint32_t b = 0; b < nb; ++b) {
+        prep[b] = pool->wait(t0 + b);
+        const double start = prep[b].start;
+        const double ub = std::max(100.0, start);
+        opt[b].reset(new Sbplx1D(start, 0.0, ub, std::sqrt(start), p.tol, 1e-12, (int)p.max_iter));
+      }
+      std::vector<double> sums;
+      for (;;) {
+        std::vector<uint32_t> list;
+        for (uint32_t b = 0; b < nb; ++b) {
+          if (code[b] != Sbplx1D::kNeed) continue;
+          const int rc = opt[b]->run();
+          if (rc == Sbplx1D::kNeed) {
+            ms[b].lambda = opt[b]->pending();
+            list.push_back(b);
+          } else {
+            code[b] = rc;
+          }
+        }
+        if (list.empty()) break;
+        P.eval_risk(ms, list, sums);
+        for (size_t k = 0; k < list.size(); ++k) opt[list[k]]->supply(P.risk(&sums[k * 4], prep[list[k]]));
+      }
+      for (uint32_t b = 0; b < nb; ++b) {
+        ms[b].lambda = opt[b]->last_x();
+        if (code[b] < 0) {
+          status = PGSVT_E_NUMERIC;
+          warn = "lambda optimisation failed (NLopt-equivalent error code " + std::to_string(code[b]) + ")";
+        } else if (code[b] == Sbplx1D::kMaxEval) {
+          std::fprintf(stderr,
+                       "WARNING: optimization terminated after max_eval (%u) was reached.\n"
+                       "Consider increasing the max_eval or increasing the convergence tolerance (tol=%g).\n",
+                       p.max_iter, p.tol);
+        }
+      }
+    } else {
+      for (uint32_t b = 0; b < nb; ++b) {
+        ms[b].lambda = lambda0;
+        prep[b].alpha = alpha0;
+        prep[b].mu = mu0;
+        prep[b].sigma = sigma0;
+      }
+    }
+    P.finish(ms);
+    for (uint32_t b = 0; b < nb; ++b) {
+      const uint32_t t = t0 + b;
+      est[t + (size_t)nf * 0] = ms[b].lambda;
+      est[t + (size_t)nf * 1] = prep[b].alpha;
+      est[t + (size_t)nf * 2] = prep[b].mu;
+      est[t + (size_t)nf * 3] = prep[b].sigma;
+    }
+  }
+  if (!on_device) hipchk(hipMemcpyAsync(Y, P.Yd.p, P.Yd.bytes(), hipMemcpyDeviceToHost, P.s), "D2H Y");
+  hipchk(hipStreamSynchronize(P.s), "final sync");
+  if (status) return fail(status, warn);
+  return 0;
+}
+
+template <class F>
+int guarded(F f) {
+  try {
+    return f();
+  } catch (const HipError& e) {
+    return fail(PGSVT_E_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
+  } catch (const std::bad_alloc&) {
+    return fail(PGSVT_E_HIP, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(PGSVT_E_HIP, e.what());
+  }
+}
+
+// one window through the GPU stages; used by the window-level entry points
+int one_window(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, const pgsvt_params* pp, uint32_t t,
+               bool want_risk, Pipeline& P, std::vector<WinMeta>& ms) {
+  if (int rc = validate(X, dtype, nr, nc, nf, pp)) return rc;
+  if (t >= nf) return fail(PGSVT_E_ARG, "window index out of range");
+  if (int rc = check_device()) return rc;
+  pgsvt_params q = *pp;
+  q.optimize_pgure = want_risk ? 1 : 0;
+  P.setup(X, dtype, nr, nc, nf, q);
+  P.alloc_batch(1);
+  ms.assign(1, P.meta_of(t));
+  P.decompose(ms, P.nvar);
+  return 0;
+}
+
+}  // namespace
+}  // namespace pgsvt
+
+using namespace pgsvt;
+
+extern "C" {
+
+void pgsvt_default_params(pgsvt_params* p) {
+  p->trajectory_length = 15;
+  p->patch_size = 4;
+  p->patch_overlap = 1;
+  p->motion_window = 7;
+  p->motion_filter = 5;
+  p->noise_method = 4;
+  p->max_iter = 500;
+  p->n_jobs = -1;
+  p->random_seed = -1;
+  p->optimize_pgure = 1;
+  p->exponential_weighting = 1;
+  p->motion_estimation = 1;
+  p->lambda1 = 0.0;
+  p->noise_alpha = -1.0;
+  p->noise_mu = -1.0;
+  p->noise_sigma = -1.0;
+  p->tol = 1e-7;
+}
+
+const char* pgsvt_last_error(void) { return g_err.c_str(); }
+
+const char* pgsvt_version(void) { return "pgsvt 0.1.0 (gfx950)"; }
+
+int pgsvt_denoise(const void* X, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
+                  const pgsvt_params* p, double* Y, double* estimates) {
+  if (!Y || !estimates) return fail(PGSVT_E_ARG, "null output pointer");
+  return guarded([&] { return run_denoise(X, dtype, n_rows, n_cols, n_frames, p, Y, estimates, false); });
+}
+
+int pgsvt_denoise_device(const void* X_dev, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
+                         const pgsvt_params* p, double* Y_dev, double* estimates) {
+  if (!Y_dev || !estimates) return fail(PGSVT_E_ARG, "null output pointer");
+  return guarded([&] { return run_denoise(X_dev, dtype, n_rows, n_cols, n_frames, p, Y_dev, estimates, true); });
+}
+
+void pgsvt_profile_enable(int on) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_prof_on = on != 0;
+  g_prof.clear();
+}
+
+int pgsvt_profile_read(char* buf, uint64_t len) {
+  std::string js = "{";
+  {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (size_t k = 0; k < g_prof.size(); ++k) {
+      char line[256];
+      std::snprintf(line, sizeof line, "%s\"%s\": {\"launches\": %llu, \"ms\": %.6f, \"units\": %.1f}",
+                    k ? ", " : "", g_prof[k].name.c_str(), (unsigned long long)g_prof[k].launches, g_prof[k].ms,
+                    g_prof[k].units);
+      js += line;
+    }
+  }
+  js += "}";
+  if (!buf || len < js.size() + 1) return (int)(js.size() + 1);
+  std::memcpy(buf, js.c_str(), js.size() + 1);
+  return 0;
+}
+
+int pgsvt_window_trajectories(const void* X, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
+                              const pgsvt_params* p, uint32_t t, int32_t* patches) {
+  return guarded([&] {
+    Pipeline P;
+    std::vector<WinMeta> ms;
+    if (int rc = one_window(X, dtype, n_rows, n_cols, n_frames, p, t, false, P, ms)) return rc;
+    std::vector<int32_t> packed((size_t)P.sh.nw * P.sh.p_all);
+    hipchk(hipMemcpyAsync(packed.data(), P.pos.p, packed.size() * 4, hipMemcpyDeviceToHost, P.s), "D2H pos");
+    hipchk(hipStreamSynchronize(P.s), "sync");
+    for (size_t k = 0; k < packed.size(); ++k) {
+      patches[2 * k] = packed[k] & 0xffff;
+      patches[2 * k + 1] = (packed[k] >> 16) & 0xffff;
+    }
+    return 0;
+  });
+}
+
+int pgsvt_window_risk(const void* X, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
+                      const pgsvt_params* p, uint32_t t, const double* noise, const double* lambdas,
+                      uint32_t n_lambda, double* risk) {
+  return guarded([&] {
+    Pipeline P;
+    std::vector<WinMeta> ms;
+    if (int rc = one_window(X, dtype, n_rows, n_cols, n_frames, p, t, true, P, ms)) return rc;
+    WindowPrep w;
+    w.alpha = noise[0];
+    w.mu = noise[1];
+    w.sigma = noise[2];
+    std::vector<double> sums;
+    for (uint32_t k = 0; k < n_lambda; ++k) {
+      ms[0].lambda = lambdas[k];
+      P.eval_risk(ms, {0u}, sums);
+      risk[k] = P.risk(sums.data(), w);
+    }
+    return 0;
+  });
+}
+
+int pgsvt_window_reconstruct(const void* X, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
+                             const pgsvt_params* p, uint32_t t, double lambda, double* vol) {
+  return guarded([&] {
+    Pipeline P;
+    std::vector<WinMeta> ms;
+    if (int rc = one_window(X, dtype, n_rows, n_cols, n_frames, p, t, false, P, ms)) return rc;
+    ms[0].lambda = lambda;
+    const uint32_t zero = 0;
+    hipchk(hipMemcpyAsync(P.meta_d.p, ms.data(), sizeof(WinMeta), hipMemcpyHostToDevice, P.s), "H2D meta");
+    hipchk(hipMemcpyAsync(P.wl_d.p, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, P.s), "H2D list");
+    hipchk(launch_recon(P.factors.p, P.u32.p, P.pos.p, P.sel_d.p, P.meta_d.p, P.wl_d.p, 1, P.nvar, 1, P.sh,
+                        P.p.exponential_weighting, 0, P.acc.p, P.s),
+           "recon");
+    DBuf<double> out;
+    out.alloc(P.sh.vol);
+    hipchk(launch_window_volume(P.acc.p, P.W.p, P.nvar, P.sh, out.p, P.s), "volume");
+    hipchk(hipMemcpyAsync(vol, out.p, out.bytes(), hipMemcpyDeviceToHost, P.s), "D2H vol");
+    hipchk(hipStreamSynchronize(P.s), "sync");
+    return 0;
+  });
+}
+
+int pgsvt_window_singular_values(const void* X, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
+                                 const pgsvt_params* p, uint32_t t, double* s, uint32_t* n_patches) {
+  return guarded([&] {
+    Pipeline P;
+    std::vector<WinMeta> ms;
+    if (int rc = one_window(X, dtype, n_rows, n_cols, n_frames, p, t, false, P, ms)) return rc;
+    std::vector<float> f((size_t)P.sh.p_sel * P.sh.fstride);
+    hipchk(hipMemcpyAsync(f.data(), P.factors.p, f.size() * 4, hipMemcpyDeviceToHost, P.s), "D2H factors");
+    hipchk(hipStreamSynchronize(P.s), "sync");
+    const uint32_t n = P.sh.nw;
+    for (uint32_t k = 0; k < P.sh.p_sel; ++k) {
+      std::vector<double> v(n);
+      for (uint32_t j = 0; j < n; ++j) v[j] = f[(size_t)k * P.sh.fstride + n * n + j];
+      std::sort(v.begin(), v.end(), [](double a, double b) { return a > b; });
+      std::memcpy(s + (size_t)k * n, v.data(), n * sizeof(double));
+    }
+    *n_patches = P.sh.p_sel;
+    return 0;
+  });
+}
+
+int pgsvt_noise_estimate(const double* u, uint32_t n_rows, uint32_t n_cols, uint32_t n_t, uint32_t method,
+                         double* alpha, double* mu, double* sigma) {
+  if (!u || !alpha || !mu || !sigma) return fail(PGSVT_E_ARG, "null pointer");
+  if (n_rows != n_cols) return fail(PGSVT_E_ARG, "quadtree noise estimation requires square frames");
+  return guarded([&] {
+    estimate_noise(u, n_rows, n_t, method, *alpha, *mu, *sigma);
+    return 0;
+  });
+}
+
+int pgsvt_delta2_signs(int64_t seed, uint64_t n_vox, uint8_t* out) {
+  return guarded([&] {
+    if (int rc = check_device()) return rc;
+    DBuf<uint8_t> d;
+    d.alloc(n_vox);
+    hipchk(launch_delta2(seed, n_vox, d.p, nullptr), "delta2");
+    hipchk(hipMemcpy(out, d.p, n_vox, hipMemcpyDeviceToHost), "D2H");
+    return 0;
+  });
+}
+
+}  // extern "C"

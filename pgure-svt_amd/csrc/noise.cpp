@@ -1,0 +1,312 @@
+// noise.cpp -- host Poisson-Gaussian noise estimator (src/noise.hpp).
+//
+// Same arithmetic, in the same order, as the reference: quadtree with the
+// 2.5% F-test, Huber IRLS robust mean, MAD robust variance of the
+// Laplacian-filtered leaf, robust WLS line fit of variance against mean.
+// Two changes make it fast without changing a single rounding:
+//   * the robust mean's interquartile distance is taken on r = A - m; since
+//     rounding is monotone, sorted(r)[k] = fl(sorted(A)[k] - m), so A is
+//     sorted once per leaf instead of once per IRLS iteration;
+//   * medians and order statistics use nth_element instead of full sorts.
+// Leaf statistics of a frame are cached per (frame, normalisation).
+#include "noise.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+
+namespace pgsvt {
+namespace {
+
+// Armadillo accu order: even indices, odd indices, then their sum.
+template <class F>
+inline double accu2f(size_t n, F f) {
+  double a1 = 0.0, a2 = 0.0;
+  size_t i = 0;
+  for (; i + 1 < n; i += 2) {
+    a1 += f(i);
+    a2 += f(i + 1);
+  }
+  if (i < n) a1 += f(i);
+  return a1 + a2;
+}
+
+// arma::median: nth_element, even counts average upper and lower middle as
+// upper + (lower - upper) / 2
+double median_inplace(std::vector<double>& v) {
+  const size_t n = v.size(), half = n / 2;
+  std::nth_element(v.begin(), v.begin() + half, v.end());
+  if (n % 2 == 0) {
+    const double v1 = v[half];
+    const double v2 = *std::max_element(v.begin(), v.begin() + half);
+    return v1 + (v2 - v1) / 2.0;
+  }
+  return v[half];
+}
+
+uint32_t iqd_m(uint32_t N) { return (uint32_t)std::floor((std::floor((N + 1) / 2) + 1) / 2); }
+
+// interquartile distance of an unsorted vector (two order statistics)
+double iqd_select(std::vector<double> v) {
+  const uint32_t N = (uint32_t)v.size(), m = iqd_m(N);
+  std::nth_element(v.begin(), v.begin() + (N - m - 1), v.end());
+  const double hi = v[N - m - 1];
+  std::nth_element(v.begin(), v.begin() + (m - 1), v.begin() + (N - m - 1));
+  return hi - v[m - 1];
+}
+
+inline void huber(const double* x, double* w, size_t n) {
+  const double p = 0.75;
+  for (size_t k = 0; k < n; ++k) {
+    const double a = std::fabs(x[k]);
+    w[k] = (a < p) ? 1. : p / a;
+  }
+}
+
+// RobustMeanEstimate (src/noise.hpp:238-271)
+double robust_mean(const std::vector<double>& A) {
+  const size_t N = A.size();
+  std::vector<double> sorted(A);
+  std::sort(sorted.begin(), sorted.end());
+  const uint32_t mq = iqd_m((uint32_t)N);
+  const double tol = 1e-6, eps = 1e-12;
+  double m = 0, m0 = 1e12;
+  std::vector<double> w(N, 1.0), r(N);
+  for (uint32_t it = 0; it < 10000; ++it) {
+    m = accu2f(N, [&](size_t k) { return w[k] * A[k]; });
+    const double aux = accu2f(N, [&](size_t k) { return w[k]; });
+    m = (std::fabs(aux) < eps) ? m0 : m / aux;
+    const double e = accu2f(N, [&](size_t k) { return std::fabs(A[k] - m); }) / (double)N;
+    if (std::fabs(m0 - m) < tol || e < tol) break;
+    m0 = m;
+    const double d = ((sorted[N - mq - 1] - m) - (sorted[mq - 1] - m)) + eps;
+    const double inv = 1. / d;
+    for (size_t k = 0; k < N; ++k) r[k] = (A[k] - m) * inv;
+    huber(r.data(), w.data(), N);
+  }
+  return m;
+}
+
+// RobustVarianceEstimate (src/noise.hpp:232-236)
+double robust_var(std::vector<double> a) {
+  std::vector<double> c(a);
+  const double med = median_inplace(c);
+  for (double& v : a) v = std::fabs(v - med);
+  const double sig = 1.4826 * median_inplace(a);
+  return sig * sig;
+}
+
+const uint32_t kDof[12] = {2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096};
+const double kF0025[12] = {15.4392, 2.86209, 1.64602, 1.27893, 1.13046, 1.06318,
+                           1.03110, 1.01543, 1.00769, 1.00384, 1.00192, 1.00096};
+
+// SplitBlockQ (src/noise.hpp:182-221) on the N x N block at (i0, j0)
+bool split_block(const double* F, uint32_t n, uint32_t i0, uint32_t j0, uint32_t N) {
+  if (N <= 8) return false;
+  auto A = [&](uint32_t y, uint32_t x) { return F[(i0 + y) + (size_t)n * (j0 + x)]; };
+  const double sq = std::sqrt((double)(5 * 5 + 5));
+  auto res = [&](size_t k) {
+    const uint32_t y = (uint32_t)(k % N), x = (uint32_t)(k / N);
+    const uint32_t xp = (x + 1 == N) ? 1 : x + 1, yp = (y + 1 == N) ? 1 : y + 1;
+    const uint32_t xm = (x == 0) ? N - 2 : x - 1, ym = (y == 0) ? N - 2 : y - 1;
+    return (5u * A(y, x) - (A(yp, x) + A(ym, x) + A(y, xm) + A(y, xp))) / sq;
+  };
+  const uint32_t R = N * N;
+  const double OoR = 1.0 / R, OoRm1 = 1.0 / (R - 1);
+  std::vector<double> a(R), rs(R);
+  for (size_t k = 0; k < R; ++k) {
+    a[k] = A((uint32_t)(k % N), (uint32_t)(k / N));
+    rs[k] = res(k);
+  }
+  const double accuZ = accu2f(R, [&](size_t k) { return a[k]; }) * OoR;
+  const double Sz = accu2f(R, [&](size_t k) { const double t = a[k] - accuZ; return t * t; }) * OoRm1;
+  const double accuR = accu2f(R, [&](size_t k) { return rs[k]; }) * OoR;
+  const double Se = accu2f(R, [&](size_t k) { const double t = rs[k] - accuR; return t * t; }) * OoRm1;
+  const double stat = (Sz > Se) ? Sz / Se : Se / Sz;
+  for (int k = 0; k < 12; ++k)
+    if (kDof[k] == N) return stat > kF0025[k];
+  return false;
+}
+
+// QuadTree (src/noise.hpp:419-458): after a split the recursion starts at the
+// previous last node, not at the first new child.
+void quadtree(const double* F, uint32_t n, std::vector<std::array<uint32_t, 3>>& tree, std::vector<uint32_t>& del,
+              uint32_t part) {
+  const auto node = tree[part];
+  const uint32_t i = node[0], j = node[1];
+  uint32_t s = node[2];
+  if (!split_block(F, n, i, j, s)) return;
+  s /= 2;
+  const uint32_t last = (uint32_t)tree.size() - 1;
+  tree.push_back({i, j, s});
+  tree.push_back({i + s, j, s});
+  tree.push_back({i, j + s, s});
+  tree.push_back({i + s, j + s, s});
+  del.push_back(part);
+  for (uint32_t it = last; it < last + 4; ++it) quadtree(F, n, tree, del, it);
+}
+
+// ConvolveFIR (src/noise.hpp:395-417) values, in any order (only their
+// median is used)
+std::vector<double> laplacian_values(const std::vector<double>& in, uint32_t N) {
+  std::vector<double> out((size_t)N * N);
+  auto I = [&](uint32_t r, uint32_t c) { return in[r + (size_t)N * c]; };
+  const double nl_edge = (1.0 * 0.125) * -1.0, nl_mid = -1.0 * -1.0;
+  const double nl[9] = {nl_edge, nl_edge, nl_edge, nl_edge, nl_mid, nl_edge, nl_edge, nl_edge, nl_edge};
+  for (uint32_t x = 0; x < N; ++x) {
+    for (uint32_t y = 0; y < N; ++y) {
+      const uint32_t xp = (x + 1 == N) ? 1 : x + 1, yp = (y + 1 == N) ? 1 : y + 1;
+      const uint32_t xm = (x == 0) ? N - 2 : x - 1, ym = (y == 0) ? N - 2 : y - 1;
+      const double nb[9] = {I(xm, ym), I(xm, y), I(xm, yp), I(x, ym), I(x, y), I(x, yp), I(xp, ym), I(xp, y), I(xp, yp)};
+      out[y + (size_t)N * x] = accu2f(9, [&](size_t k) { return nb[k] * nl[k]; });
+    }
+  }
+  return out;
+}
+
+// WLSFit (src/noise.hpp:328-383)
+void wls_fit(const std::vector<double>& x, const std::vector<double>& y, double& pa, double& pb) {
+  const size_t N = x.size();
+  const double tol = 1e-6, eps = 1e-12;
+  double a0 = 1e12, b0 = 1e12, p0 = 0, p1 = 0;
+  std::vector<double> w(N, 1.0), w2(N), r(N);
+  for (uint32_t it = 0; it < 10000; ++it) {
+    for (size_t k = 0; k < N; ++k) w2[k] = w[k] * w[k];
+    const double sw2 = accu2f(N, [&](size_t k) { return w2[k]; });
+    const double sw2x = accu2f(N, [&](size_t k) { return w2[k] * x[k]; });
+    const double sw2y = accu2f(N, [&](size_t k) { return w2[k] * y[k]; });
+    p0 = sw2 * accu2f(N, [&](size_t k) { return w2[k] * (x[k] * y[k]); }) - sw2x * sw2y;
+    const double aux = sw2 * accu2f(N, [&](size_t k) { return w2[k] * (x[k] * x[k]); }) - sw2x * sw2x;
+    p0 = (std::fabs(aux) < eps) ? a0 : p0 / aux;
+    p1 = sw2y - p0 * sw2x;
+    p1 = (std::fabs(aux) < eps) ? b0 : p1 / sw2;
+    for (size_t k = 0; k < N; ++k) r[k] = y[k] - (x[k] * p0 + p1);
+    const double e = accu2f(N, [&](size_t k) { return std::fabs(r[k]); }) / (double)N;
+    if ((std::fabs(a0 - p0) < tol && std::fabs(b0 - p1) < tol) || e < tol) break;
+    a0 = p0;
+    b0 = p1;
+    const double d = iqd_select(r) + eps;
+    for (size_t k = 0; k < N; ++k) r[k] /= d;
+    huber(r.data(), w.data(), N);
+  }
+  pa = p0;
+  pb = p1;
+}
+
+// ComputeMode (src/noise.hpp:273-301)
+double compute_mode(const std::vector<double>& A) {
+  const size_t N = A.size();
+  const double M = *std::max_element(A.begin(), A.end());
+  const double dyn = 1. * N;
+  std::vector<double> a(N);
+  for (size_t k = 0; k < N; ++k) a[k] = std::round(A[k] * dyn / M);
+  uint32_t maxCount = 0;
+  double maxValue = 0.;
+  for (size_t i = 0; i < N; ++i) {
+    uint32_t count = 0;
+    for (size_t j = 0; j < N; ++j) count += (a[j] == a[i]);
+    if (count > maxCount) {
+      maxCount = count;
+      maxValue = a[i];
+    }
+  }
+  return maxValue * (M / dyn);
+}
+
+}  // namespace
+
+FrameLeaves frame_leaves(const double* F, uint32_t n) {
+  std::vector<std::array<uint32_t, 3>> tree(1, {0u, 0u, n});
+  std::vector<uint32_t> del;
+  quadtree(F, n, tree, del, 0);
+  std::sort(del.begin(), del.end());
+  del.erase(std::unique(del.begin(), del.end()), del.end());
+  if (!del.empty())
+    for (size_t k = del.size() - 1; k > 0; --k) tree.erase(tree.begin() + del[k]);  // del[0] is never shed
+  FrameLeaves out;
+  out.mean.reserve(tree.size());
+  out.var.reserve(tree.size());
+  for (const auto& nd : tree) {
+    const uint32_t x = nd[0], y = nd[1], s = nd[2];
+    std::vector<double> col((size_t)s * s);
+    for (uint32_t c = 0; c < s; ++c)
+      std::memcpy(&col[(size_t)s * c], &F[x + (size_t)n * (y + c)], s * sizeof(double));
+    out.mean.push_back(robust_mean(col));
+    out.var.push_back(robust_var(laplacian_values(col, s)));
+  }
+  return out;
+}
+
+std::shared_ptr<const FrameLeaves> LeafCache::get(uint32_t frame, double scale, const double* f, uint32_t n) {
+  uint64_t bits;
+  std::memcpy(&bits, &scale, 8);
+  const uint64_t key = (uint64_t)frame * 0x9E3779B97F4A7C15ULL ^ bits;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = map_.find(key);
+    if (it != map_.end() && scale_of_[key] == scale) return it->second;
+  }
+  auto r = std::make_shared<const FrameLeaves>(frame_leaves(f, n));
+  std::lock_guard<std::mutex> g(mu_);
+  map_[key] = r;
+  scale_of_[key] = scale;
+  return r;
+}
+
+void estimate_noise(const double* u, uint32_t n, uint32_t nt, uint32_t method, double& alpha, double& mu,
+                    double& sigma, LeafCache* cache, const uint32_t* frame_ids, double scale) {
+  std::vector<double> means, vars;
+  const size_t fsz = (size_t)n * n;
+  for (uint32_t k = 0; k < nt; ++k) {
+    std::shared_ptr<const FrameLeaves> lv;
+    if (cache && frame_ids)
+      lv = cache->get(frame_ids[k], scale, u + k * fsz, n);
+    else
+      lv = std::make_shared<const FrameLeaves>(frame_leaves(u + k * fsz, n));
+    for (double v : lv->mean)
+      if (v >= 0.) means.push_back(v);
+    for (double v : lv->var)
+      if (v >= 0.) vars.push_back(v);
+  }
+  std::vector<size_t> idx(means.size());
+  for (size_t k = 0; k < idx.size(); ++k) idx[k] = k;
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return means[a] < means[b]; });
+  std::vector<double> rm(idx.size()), rv(idx.size());
+  for (size_t k = 0; k < idx.size(); ++k) {
+    rm[k] = means[idx[k]];
+    rv[k] = vars[idx[k]];
+  }
+  double a = 0, b = 0;
+  wls_fit(rm, rv, a, b);
+  alpha = (alpha >= 0.) ? alpha : a;
+  switch (method) {
+    case 1: {
+      const int L = (int)std::floor(1. * ((uint32_t)(n * n) / (uint32_t)means.size()));
+      const uint32_t ie = (uint32_t)std::round(0.05 * L);
+      std::vector<double> ra(rm.begin(), rm.begin() + ie + 1), rb(rv.begin(), rv.begin() + ie + 1);
+      mu = (mu >= 0.) ? mu : compute_mode(ra);
+      const double dSi = compute_mode(rb);
+      sigma = (sigma >= 0.) ? sigma : std::sqrt(dSi);
+      break;
+    }
+    case 2: {
+      mu = (mu >= 0.) ? mu : compute_mode(rm);
+      const double dSi = compute_mode(rv);
+      sigma = (sigma >= 0.) ? sigma : std::sqrt(std::max(dSi, std::max(b + a * dSi, 0.)));
+      break;
+    }
+    case 3: {
+      mu = (mu >= 0.) ? mu : compute_mode(rm);
+      sigma = (sigma >= 0.) ? sigma : std::sqrt(std::fabs(b + a * mu));
+      break;
+    }
+    default: {
+      mu = (mu >= 0.) ? mu : rm[0];
+      sigma = (sigma >= 0.) ? sigma : std::sqrt(std::fabs(b + a * mu));
+      break;
+    }
+  }
+}
+
+}  // namespace pgsvt

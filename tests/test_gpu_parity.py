@@ -1,0 +1,134 @@
+"""GPU path (libpgsvt.so, through the C ABI) against the CPU FP64 oracle.
+
+Bars: ARPS trajectories and delta2 signs bit-exact; reconstructed frames
+within 1e-5 relative L2; PGURE risk within 1e-4 relative; singular values
+within 1e-5 of the largest one of their patch.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import pgsvt_lib as G
+from pguresvt import phantom
+
+pytestmark = pytest.mark.gpu
+
+FRAME_TOL = 1e-5
+RISK_TOL = 1e-4
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def seq64():
+    clean, noisy = phantom.noisy_sequence(64, 24, drift=2, alpha=1.0, sigma=8.0, seed=2)
+    return noisy
+
+
+def test_delta2_signs(gpu):
+    log("delta2 signs seed 101")
+    n = 61440
+    s = G.delta2_signs(101, n)
+    d1, d2 = O.perturbations(101, n)
+    assert np.array_equal(s, (d2 < 0).astype(np.uint8))
+
+
+@pytest.mark.parametrize("t", [0, 3, 7, 12, 18, 21, 23])
+@pytest.mark.parametrize("mf", [-1, 2])
+def test_trajectories_bit_exact(gpu, seq64, t, mf):
+    log("trajectories t=%d motion_filter=%d" % (t, mf))
+    g = G.window_trajectories(seq64, t, motion_filter=mf)
+    o = O.window_trajectories(seq64, t, motion_filter=mf)
+    assert np.array_equal(g, o)
+
+
+def test_trajectories_motion_off(gpu, seq64):
+    g = G.window_trajectories(seq64, 9, motion_estimation=False)
+    o = O.window_trajectories(seq64, 9, motion_estimation=0)
+    assert np.array_equal(g, o)
+
+
+def test_singular_values(gpu, seq64):
+    log("singular values")
+    g = G.window_singular_values(seq64, 10)
+    o = O.window_singular_values(seq64, 10)
+    assert g.shape == o.shape
+    err = np.max(np.abs(g - o) / o[:, :1])
+    log("  max |ds|/smax =", err)
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("lam,expw", [(0.3, True), (5.0, True), (60.0, True), (0.02, False), (0.2, False)])
+def test_reconstruct(gpu, seq64, lam, expw):
+    log("reconstruct lambda=%g exp=%s" % (lam, expw))
+    g = G.window_reconstruct(seq64, 11, lam, exponential_weighting=expw)
+    o = O.window_reconstruct(seq64, 11, lam, exponential_weighting=int(expw))
+    e = G.rel_l2(g, o)
+    log("  rel L2 =", e)
+    assert e < FRAME_TOL
+
+
+def test_reconstruct_motion_off(gpu, seq64):
+    g = G.window_reconstruct(seq64, 2, 4.0, motion_estimation=False)
+    o = O.window_reconstruct(seq64, 2, 4.0, motion_estimation=0)
+    assert G.rel_l2(g, o) < FRAME_TOL
+
+
+@pytest.mark.parametrize("t", [0, 12, 23])
+def test_risk(gpu, seq64, t):
+    log("risk t=%d" % t)
+    noise = [0.012, 0.05, 0.06]
+    lams = [0.5, 2.0, 10.0, 40.0]
+    g = G.window_risk(seq64, t, noise, lams)
+    o = O.window_risk(seq64, t, noise, lams)
+    rel = np.abs(g - o) / np.abs(o)
+    log("  risk rel err =", rel)
+    assert np.all(rel < RISK_TOL)
+
+
+def test_denoise_fixed_lambda(gpu, seq64):
+    log("denoise fixed lambda")
+    kw = dict(optimize_pgure=False, lambda1=3.0)
+    Yg, eg = G.denoise(seq64, **kw)
+    Yo, eo = O.denoise(seq64, optimize_pgure=0, lambda1=3.0)
+    e = G.rel_l2(Yg, Yo)
+    log("  rel L2 =", e)
+    assert e < FRAME_TOL
+    assert np.array_equal(eg, eo)
+
+
+def test_denoise_pgure_motion_off(gpu):
+    # the CPU-reference configuration at reduced size: static blobs, motion off
+    log("denoise PGURE motion off")
+    clean, noisy = phantom.noisy_sequence(64, 20, drift=0, alpha=0.5, sigma=5.0, seed=1)
+    kw = dict(motion_estimation=False)
+    Yg, eg = G.denoise(noisy, **kw)
+    Yo, eo = O.denoise(noisy, motion_estimation=0)
+    # noise estimates come from the same arithmetic: identical
+    assert np.array_equal(eg[:, 1:], eo[:, 1:])
+    lam_rel = np.abs(eg[:, 0] - eo[:, 0]) / np.abs(eo[:, 0])
+    log("  lambda rel diff max =", lam_rel.max())
+    # frames at the GPU's lambda against the oracle at the same lambda
+    for t in (0, 9, 19):
+        g = G.window_reconstruct(noisy, t, eg[t, 0], motion_estimation=False)
+        o = O.window_reconstruct(noisy, t, eg[t, 0], motion_estimation=0)
+        assert G.rel_l2(g, o) < FRAME_TOL
+    assert lam_rel.max() < 1e-2
+
+
+def test_denoise_pgure_motion_on(gpu, seq64):
+    log("denoise PGURE motion on")
+    Yg, eg = G.denoise(seq64)
+    Yo, eo = O.denoise(seq64)
+    assert np.array_equal(eg[:, 1:], eo[:, 1:])
+    lam_rel = np.abs(eg[:, 0] - eo[:, 0]) / np.abs(eo[:, 0])
+    log("  lambda rel diff max =", lam_rel.max(), " frames rel L2 =", G.rel_l2(Yg, Yo))
+    same = lam_rel == 0
+    log("  windows with identical lambda: %d / %d" % (same.sum(), len(same)))
+    assert lam_rel.max() < 1e-2
