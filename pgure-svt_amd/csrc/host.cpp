@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
@@ -122,6 +123,28 @@ class Profiler {
  private:
   hipStream_t s_;
   std::vector<ProfPending> pend_;
+};
+
+// wall-clock time of a host phase, recorded next to the kernel timings
+void prof_host(const char* name, double ms) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  if (!g_prof_on) return;
+  for (auto& e : g_prof)
+    if (e.name == name) {
+      e.launches++;
+      e.ms += ms;
+      return;
+    }
+  g_prof.push_back(ProfEntry{name, 1, ms, 0});
+}
+
+struct HostTimer {
+  const char* name;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit HostTimer(const char* n) : name(n) {}
+  ~HostTimer() {
+    prof_host(name, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  }
 };
 
 // device buffer owning its allocation
@@ -337,6 +360,10 @@ class PrepPool {
     cv_.wait(g, [&] { return ready_[t] != 0; });
     return res_[t];
   }
+  bool ready(uint32_t t) {
+    std::lock_guard<std::mutex> g(mu_);
+    return ready_[t] != 0;
+  }
 
  private:
   std::function<WindowPrep(uint32_t)> fn_;
@@ -375,7 +402,7 @@ struct Pipeline {
   uint32_t NB = 0;
   DBuf<WinMeta> meta_d;
   DBuf<uint32_t> wl_d;
-  DBuf<double> u64, w64, partial, sums;
+  DBuf<double> u64, w64, invw, partial, partial_b, sums;
   DBuf<float> u32, factors;
   DBuf<int32_t> pos, W;
   DBuf<int16_t> jobs_d;
@@ -483,7 +510,8 @@ struct Pipeline {
     b += 4ull * sh.vol * nvar;                           // u32
     b += 4ull * sh.nw * sh.p_all;                        // pos
     b += 4ull * sh.vol;                                  // W
-    b += 8ull * sh.vol * nvar;                           // acc
+    b += 8ull * sh.vol;                                  // acc (variant 0)
+    b += 8ull * sh.vol;                                  // invw
     b += 4ull * nvar * (size_t)sh.p_sel * sh.fstride;    // factors
     return b;
   }
@@ -502,11 +530,13 @@ struct Pipeline {
     u32.alloc((size_t)NB * sh.vol * nvar);
     pos.alloc((size_t)NB * sh.nw * sh.p_all);
     W.alloc((size_t)NB * sh.vol);
-    acc.alloc((size_t)NB * sh.vol * nvar);
+    acc.alloc((size_t)NB * sh.vol);
+    invw.alloc((size_t)NB * sh.vol);
     factors.alloc((size_t)NB * nvar * sh.p_sel * sh.fstride);
     risk_blocks = std::min<uint32_t>(256, (sh.vol + 255) / 256);
-    partial.alloc((size_t)NB * risk_blocks * 4);
-    sums.alloc((size_t)NB * 4);
+    partial.alloc((size_t)NB * recon_risk_blocks(sh) * 3);
+    partial_b.alloc((size_t)NB * risk_blocks * 3);
+    sums.alloc((size_t)NB * 6);
     max_jobs = 2 * g.Nw;
     jobs_d.alloc((size_t)NB * max_jobs * 4);
     stats.alloc(1);
@@ -541,6 +571,7 @@ struct Pipeline {
     }
     hipchk(hipMemsetAsync(W.p, 0, (size_t)nb * sh.vol * sizeof(int32_t), s), "memset W");
     hipchk(launch_count(pos.p, sel_d.p, nb, sh, W.p, s), "count");
+    hipchk(launch_invw(W.p, (size_t)nb * sh.vol, invw.p, s), "invw");
     // units: matrices decomposed
     prof->run("svd_jacobi", (double)nb * nvar_run * sh.p_sel, [&] {
       hipchk(launch_svd(u32.p, pos.p, sel_d.p, nb, nvar_run, sh, factors.p, 30, 1e-6f, stats.p, s), "svd");
@@ -549,13 +580,16 @@ struct Pipeline {
 
   // PGURE risk (src/pgure.hpp:136) from the GPU sums; noise as the driver
   // holds it (alpha, mu, sigma) before the swapped constructor call.
+  // S = (sum Uhat^2, sum U, sum U^2, sum Uhat U, sum d2 (U2p - 2 Uhat + U2m),
+  // sum Uhat); sum (Uhat - U)^2 = sum Uhat^2 - 2 sum Uhat U + sum U^2.
   double risk(const double* S, const WindowPrep& w) const {
     const double alpha = w.alpha, mup = w.sigma, sigp = w.mu;  // constructor receives (alpha, sigma, mu)
     const double sigmasq = sigp * sigp;
     const double OoN = 1.0 / (sh.nr * sh.nc * sh.nw);
+    const double S1 = S[0] - 2.0 * S[3] + S[2];
     const double t3 = 0.0;  // delta1 term: U1 == U (integer delta1 cube)
-    return OoN * (S[0] - (alpha + mup) * S[1] + (2 / eps1 * t3) - (2 * sigmasq * alpha / (eps2 * eps2) * S[2]) +
-                  (2 * mup * S[3]) + mup) -
+    return OoN * (S1 - (alpha + mup) * S[1] + (2 / eps1 * t3) - (2 * sigmasq * alpha / (eps2 * eps2) * S[4]) +
+                  (2 * mup * S[5]) + mup) -
            sigmasq;
   }
 
@@ -565,18 +599,14 @@ struct Pipeline {
     hipchk(hipMemcpyAsync(meta_d.p, ms.data(), ms.size() * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
     hipchk(hipMemcpyAsync(wl_d.p, list.data(), na * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
     // units: thresholded patch reconstructions (patch x variant)
-    prof->run("recon_overlap_add", (double)na * 3 * sh.p_sel, [&] {
-      hipchk(launch_recon(factors.p, u32.p, pos.p, sel_d.p, meta_d.p, wl_d.p, na, nvar, 3, sh,
-                          p.exponential_weighting, 0, acc.p, s),
-             "recon");
+    prof->run("recon_risk", (double)na * 3 * sh.p_sel, [&] {
+      hipchk(launch_recon_risk(factors.p, u32.p, pos.p, sel_d.p, meta_d.p, wl_d.p, na, sh, p.exponential_weighting,
+                               invw.p, u64.p, d2.p, d2neg, d2pos, acc.p, partial.p, partial_b.p, risk_blocks,
+                               sums.p, s),
+             "recon_risk");
     });
-    // units: voxels
-    prof->run("pgure_risk", (double)na * sh.vol, [&] {
-      hipchk(launch_risk(acc.p, W.p, u64.p, d2.p, wl_d.p, na, sh, d2neg, d2pos, partial.p, risk_blocks, sums.p, s),
-             "risk");
-    });
-    out.resize((size_t)na * 4);
-    hipchk(hipMemcpyAsync(out.data(), sums.p, (size_t)na * 4 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H sums");
+    out.resize((size_t)na * 6);
+    hipchk(hipMemcpyAsync(out.data(), sums.p, (size_t)na * 6 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H sums");
     hipchk(hipStreamSynchronize(s), "risk sync");
     prof->flush();
   }
@@ -598,21 +628,25 @@ struct Pipeline {
     prof->flush();
   }
 
-  // normalised window t on the host (for the noise estimator) and accu(u)
-  void host_window(uint32_t t, std::vector<double>& u, double& umax, double& accu) const {
+  // accu(u) of the normalised window t in Armadillo's order (even / odd
+  // partial sums), the lambda start point of src/pguresvt.hpp:139
+  double host_window_accu(uint32_t t) const {
     const WinMeta m = meta_of(t);
-    u.resize(sh.vol);
     const size_t base = (size_t)m.s0 * sh.fs;
-    for (size_t k = 0; k < sh.vol; ++k) u[k] = elem(X, dtype, base + k) / m.uMax;
-    umax = m.uMax;
     double a1 = 0, a2 = 0;
     size_t i = 0;
-    for (; i + 1 < u.size(); i += 2) {
-      a1 += u[i];
-      a2 += u[i + 1];
+    for (; i + 1 < sh.vol; i += 2) {
+      a1 += elem(X, dtype, base + i) / m.uMax;
+      a2 += elem(X, dtype, base + i + 1) / m.uMax;
     }
-    if (i < u.size()) a1 += u[i];
-    accu = a1 + a2;
+    if (i < sh.vol) a1 += elem(X, dtype, base + i) / m.uMax;
+    return a1 + a2;
+  }
+
+  // frame f normalised by scale (true division, as u /= uMax)
+  void host_frame(uint32_t f, double scale, double* out) const {
+    const size_t base = (size_t)f * sh.fs;
+    for (size_t k = 0; k < sh.fs; ++k) out[k] = elem(X, dtype, base + k) / scale;
   }
 };
 
@@ -626,8 +660,12 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
                 double* est, bool on_device) {
   if (int rc = validate(X, dtype, nr, nc, nf, pp)) return rc;
   if (int rc = check_device()) return rc;
+  HostTimer total_timer("host_total_call");
   Pipeline P;
-  P.setup(X, dtype, nr, nc, nf, *pp, on_device);
+  {
+    HostTimer t("host_setup");
+    P.setup(X, dtype, nr, nc, nf, *pp, on_device);
+  }
   const pgsvt_params& p = P.p;
   const Geometry& g = P.g;
   const double lambda0 = (p.lambda1 >= 0.0) ? p.lambda1 : -1.0;
@@ -640,7 +678,10 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     P.Yd.alloc((size_t)P.sh.fs * nf);
     P.Yout = P.Yd.p;
   }
-  P.alloc_batch(nf);
+  {
+    HostTimer t("host_alloc_batch");
+    P.alloc_batch(nf);
+  }
 
   LeafCache cache;
   std::unique_ptr<PrepPool> pool;
@@ -650,15 +691,15 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
       w.alpha = alpha0;
       w.mu = mu0;
       w.sigma = sigma0;
-      std::vector<double> u;
-      double umax = 0, acc = 0;
-      P.host_window(t, u, umax, acc);
-      std::vector<uint32_t> ids(g.Nw);
       const WinMeta m = P.meta_of(t);
-      for (uint32_t k = 0; k < g.Nw; ++k) ids[k] = m.s0 + k;
-      estimate_noise(u.data(), nr, g.Nw, p.noise_method, w.alpha, w.mu, w.sigma, &cache, ids.data(), umax);
+      std::vector<std::shared_ptr<const FrameLeaves>> leaves;
+      for (uint32_t k = 0; k < g.Nw; ++k) {
+        const uint32_t f = m.s0 + k;
+        leaves.push_back(cache.get(f, m.uMax, nr, [&](double* buf) { P.host_frame(f, m.uMax, buf); }));
+      }
+      estimate_from_leaves(leaves, nr, p.noise_method, w.alpha, w.mu, w.sigma);
       const double OoNxNyNt = 1.0 / (nc * nr * g.Nt);
-      double start = (lambda0 >= 0.0) ? lambda0 : acc * OoNxNyNt;
+      const double start = (lambda0 >= 0.0) ? lambda0 : P.host_window_accu(t) * OoNxNyNt;
       w.start = std::max(0.0, start);
       return w;
     }));
@@ -675,17 +716,30 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     if (P.optimize) {
       std::vector<std::unique_ptr<Sbplx1D>> opt(nb);
       std::vector<int> code(nb, Sbplx1D::kNeed);
-      for (uint32_t b = 0; b < nb; ++b) {
+      {
+        HostTimer t("host_decompose_sync");
+        hipchk(hipStreamSynchronize(P.s), "decompose sync");
+      }
+      // A window joins the lockstep search as soon as its noise estimate is
+      // ready; the rounds run while the host threads finish the others.
+      std::vector<char> started(nb, 0);
+      uint32_t n_started = 0;
+      auto admit = [&](uint32_t b) {
         prep[b] = pool->wait(t0 + b);
         const double start = prep[b].start;
         const double ub = std::max(100.0, start);
         opt[b].reset(new Sbplx1D(start, 0.0, ub, std::sqrt(start), p.tol, 1e-12, (int)p.max_iter));
-      }
+        started[b] = 1;
+        n_started++;
+      };
+      HostTimer tr("host_search_rounds");
       std::vector<double> sums;
       for (;;) {
+        for (uint32_t b = 0; b < nb; ++b)
+          if (!started[b] && pool->ready(t0 + b)) admit(b);
         std::vector<uint32_t> list;
         for (uint32_t b = 0; b < nb; ++b) {
-          if (code[b] != Sbplx1D::kNeed) continue;
+          if (!started[b] || code[b] != Sbplx1D::kNeed) continue;
           const int rc = opt[b]->run();
           if (rc == Sbplx1D::kNeed) {
             ms[b].lambda = opt[b]->pending();
@@ -694,9 +748,18 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
             code[b] = rc;
           }
         }
-        if (list.empty()) break;
+        if (list.empty()) {
+          if (n_started == nb) break;
+          HostTimer tw("host_noise_wait");
+          for (uint32_t b = 0; b < nb; ++b)
+            if (!started[b]) {
+              admit(b);
+              break;
+            }
+          continue;
+        }
         P.eval_risk(ms, list, sums);
-        for (size_t k = 0; k < list.size(); ++k) opt[list[k]]->supply(P.risk(&sums[k * 4], prep[list[k]]));
+        for (size_t k = 0; k < list.size(); ++k) opt[list[k]]->supply(P.risk(&sums[k * 6], prep[list[k]]));
       }
       for (uint32_t b = 0; b < nb; ++b) {
         ms[b].lambda = opt[b]->last_x();

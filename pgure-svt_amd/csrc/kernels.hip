@@ -619,7 +619,7 @@ __global__ __launch_bounds__(256) void k_recon(const float* __restrict__ factors
     for (int j = 0; j < N; ++j) t = fmaf(x[j], sV[grp * VS + j * N + l], t);
     y[l] = t * d[l];
   }
-  unsigned long long* A = acc + ((size_t)b * nvar_stored + var) * vol;
+  unsigned long long* A = acc + (size_t)b * vol;  // variant-0 accumulator
   const int ref = (int)meta[b].ref;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
@@ -630,6 +630,196 @@ __global__ __launch_bounds__(256) void k_recon(const float* __restrict__ factors
     const long long q = __float2ll_rn(bq * kFixScale);
     atomicAdd(A + addr[k], (unsigned long long)q);
   }
+}
+
+// --------------------------------------- fused reconstruction + risk --
+// One lambda evaluation of PGURE<T>::CalculatePGURE (src/pgure.hpp:128-136)
+// for every patch of a window.  A group reconstructs its patch for the three
+// decomposed variants (U, U + eps2 delta2, U - eps2 delta2).  Only the
+// unperturbed reconstruction is needed as a volume (for sum Uhat^2); every
+// other risk term is linear in the overlap-added values, so with
+// Uhat_v = acc_v / W_v it is accumulated per contribution:
+//   sum Uhat                  = sum_c B0_c / W_v
+//   sum Uhat U                = sum_c B0_c U_v / W_v
+//   sum d2 (U2p - 2Uhat + U2m) = sum_c d2_v (B1_c - 2 B0_c + B2_c) / W_v
+// (c runs over patch contributions, v is the voxel c lands on).  Only B0 is
+// scattered into the fixed-point volume; the sums are FP64 per thread, then
+// reduced in a fixed order.
+template <int BS, int N, int L>
+__global__ __launch_bounds__(256) void k_recon_risk(
+    const float* __restrict__ factors, const float* __restrict__ u32, const int32_t* __restrict__ pos,
+    const uint32_t* __restrict__ sel, const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl,
+    uint32_t p_sel, uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim,
+    uint32_t fstride, int exp_w, const double* __restrict__ invw, const double* __restrict__ u64,
+    const uint8_t* __restrict__ d2, double d2neg, double d2pos, unsigned long long* __restrict__ acc,
+    double* __restrict__ partial) {
+  constexpr int M = BS * BS, G = 256 / L, VS = N * N;
+  __shared__ float sV[G * VS];
+  __shared__ float sS[G * N];
+  __shared__ double red[3][256];
+  const uint32_t slot = blockIdx.y, b = wl[slot];
+  const int grp = threadIdx.x / L, lane = threadIdx.x % L;
+  const uint32_t p = blockIdx.x * G + grp;
+  const bool valid = p < p_sel;
+  const bool row = valid && lane < M;
+  const double lam = meta[b].lambda;
+  const uint32_t it = valid ? sel[p] : 0;
+  const int rr = lane % BS, cc = lane / BS;
+  uint32_t addr[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    uint32_t a = 0;
+    if (row) {
+      const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
+      const uint32_t r0 = min((uint32_t)(pk & 0xffff), lim), c0 = min((uint32_t)(pk >> 16), lim);
+      a = (uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc);
+    }
+    addr[k] = a;
+  }
+  float B0[N];
+  double Cd[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    B0[k] = 0.f;
+    Cd[k] = 0.0;
+  }
+#pragma unroll 1
+  for (int var = 0; var < 3; ++var) {
+    __syncthreads();  // sV / sS reuse across variants
+    if (valid && lane < N) {
+      const float* F = factors + (((size_t)b * 3 + var) * p_sel + p) * fstride;
+#pragma unroll
+      for (int k = 0; k < N; ++k) sV[grp * VS + lane * N + k] = F[lane * N + k];
+      sS[grp * N + lane] = F[N * N + lane];
+    }
+    __syncthreads();
+    if (row) {
+      const float* S = sS + grp * N;
+      float smax = 0.f;
+#pragma unroll
+      for (int l = 0; l < N; ++l) smax = fmaxf(smax, S[l]);
+      const float kk = (float)(-0.5 * lam), lf = (float)lam;
+      const float* U = u32 + ((size_t)b * 3 + var) * vol;
+      float x[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) x[k] = U[addr[k]];
+      float y[N];
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        float t = 0.f;
+#pragma unroll
+        for (int j = 0; j < N; ++j) t = fmaf(x[j], sV[grp * VS + j * N + l], t);
+        const float sl = S[l];
+        const float w = exp_w ? fabsf(smax * expf(kk * (sl * sl))) : lf;
+        y[l] = sl > 0.f ? t * (fmaxf(sl - w, 0.f) / sl) : 0.f;
+      }
+      // Compiler-only barrier: re-read V from LDS for the second product
+      // instead of keeping all n*n values of the first one in registers.
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        float bq = 0.f;
+#pragma unroll
+        for (int l = 0; l < N; ++l) bq = fmaf(y[l], sV[grp * VS + k * N + l], bq);
+        if (var == 0)
+          B0[k] = bq;
+        else
+          Cd[k] += (double)bq;
+      }
+    }
+  }
+  double s5 = 0, sx = 0, s4 = 0;
+  if (row) {
+    unsigned long long* A = acc + (size_t)b * vol;
+    const double* IW = invw + (size_t)b * vol;
+    const double* UU = u64 + (size_t)b * vol;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const uint32_t v = addr[k];
+      atomicAdd(A + v, (unsigned long long)__float2ll_rn(B0[k] * kFixScale));
+      const double iw = IW[v];
+      const double b0 = (double)B0[k];
+      s5 = fma(b0, iw, s5);
+      sx = fma(b0 * UU[v], iw, sx);
+      const double c2 = Cd[k] - 2.0 * b0;
+      s4 = fma(c2 * (d2[v] ? d2neg : d2pos), iw, s4);
+    }
+  }
+  red[0][threadIdx.x] = s5;
+  red[1][threadIdx.x] = sx;
+  red[2][threadIdx.x] = s4;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) partial[((size_t)slot * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// sum Uhat^2, sum U, sum U^2 from the variant-0 volume (cleared as read)
+__global__ __launch_bounds__(256) void k_risk_volume(unsigned long long* __restrict__ acc,
+                                                     const double* __restrict__ invw,
+                                                     const double* __restrict__ u64, const uint32_t* __restrict__ wl,
+                                                     uint32_t vol, double* __restrict__ partial) {
+  __shared__ double sm[3][256];
+  const uint32_t slot = blockIdx.y, b = wl[slot];
+  double q = 0, s2 = 0, su2 = 0;
+  unsigned long long* A = acc + (size_t)b * vol;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < vol; v += gridDim.x * blockDim.x) {
+    const long long a = (long long)A[v];
+    A[v] = 0ull;
+    const double uh = ((double)a * kFixInv) * invw[(size_t)b * vol + v];
+    const double U = u64[(size_t)b * vol + v];
+    q = fma(uh, uh, q);
+    s2 += U;
+    su2 = fma(U, U, su2);
+  }
+  sm[0][threadIdx.x] = q;
+  sm[1][threadIdx.x] = s2;
+  sm[2][threadIdx.x] = su2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int c = 0; c < 3; ++c) sm[c][threadIdx.x] += sm[c][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) partial[((size_t)slot * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+// per slot: (sum Uhat^2, sum U, sum U^2, sum Uhat U, sum d2 (..), sum Uhat)
+__global__ __launch_bounds__(256) void k_risk_reduce2(const double* __restrict__ pa, uint32_t na,
+                                                      const double* __restrict__ pb, uint32_t nb_,
+                                                      double* __restrict__ sums) {
+  __shared__ double sm[6][256];
+  const uint32_t slot = blockIdx.x;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (uint32_t k = threadIdx.x; k < nb_; k += blockDim.x)
+    for (int c = 0; c < 3; ++c) s[c] += pb[((size_t)slot * nb_ + k) * 3 + c];
+  for (uint32_t k = threadIdx.x; k < na; k += blockDim.x) {
+    s[5] += pa[((size_t)slot * na + k) * 3 + 0];  // sum Uhat
+    s[3] += pa[((size_t)slot * na + k) * 3 + 1];  // sum Uhat U
+    s[4] += pa[((size_t)slot * na + k) * 3 + 2];  // sum d2 (U2p - 2 Uhat + U2m)
+  }
+  for (int c = 0; c < 6; ++c) sm[c][threadIdx.x] = s[c];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int c = 0; c < 6; ++c) sm[c][threadIdx.x] += sm[c][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) sums[(size_t)slot * 6 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+__global__ void k_invw(const int32_t* __restrict__ W, size_t n, double* __restrict__ invw) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    invw[i] = W[i] > 0 ? 1.0 / (double)W[i] : 0.0;
+}
+
+hipError_t launch_invw(const int32_t* W, size_t n, double* invw, hipStream_t s) {
+  const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_invw, dim3(grid), dim3(256), 0, s, W, n, invw);
+  return hipGetLastError();
 }
 
 // shape table of the instantiated SVD / reconstruction kernels
@@ -682,77 +872,39 @@ hipError_t launch_recon(const float* factors, const float* u32, const int32_t* p
   return hipErrorInvalidValue;
 }
 
+uint32_t recon_risk_blocks(const Shape& sh) {
+  const uint32_t G = (sh.m <= 16) ? 16 : (sh.m <= 32 ? 8 : 4);
+  return (sh.p_sel + G - 1) / G;
+}
+
+hipError_t launch_recon_risk(const float* factors, const float* u32, const int32_t* pos, const uint32_t* sel,
+                             const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int exp_w,
+                             const double* invw, const double* u64, const uint8_t* d2, double d2neg, double d2pos,
+                             unsigned long long* acc, double* partial_a, double* partial_b, uint32_t nblk_b,
+                             double* sums, hipStream_t s) {
+  const uint32_t lim = sh.nr - sh.bs;
+  const uint32_t nblk_a = recon_risk_blocks(sh);
+  bool done = false;
+#define PG_RR(B, N, L)                                                                                           \
+  if (!done && sh.bs == B && sh.nw == N) {                                                                     \
+    hipLaunchKernelGGL((k_recon_risk<B, N, L>), dim3(nblk_a, na), dim3(256), 0, s, factors, u32, pos, sel,     \
+                       meta, wl, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, exp_w, invw,  \
+                       u64, d2, d2neg, d2pos, acc, partial_a);                                                 \
+    done = true;                                                                                               \
+  }
+  PG_SHAPES(PG_RR)
+#undef PG_RR
+  if (!done) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_risk_volume, dim3(nblk_b, na), dim3(256), 0, s, acc, invw, u64, wl, sh.vol, partial_b);
+  hipLaunchKernelGGL(k_risk_reduce2, dim3(na), dim3(256), 0, s, partial_a, nblk_a, partial_b, nblk_b, sums);
+  return hipGetLastError();
+}
+
 // -------------------------------------------------------- PGURE risk --
-// src/pgure.hpp:130-136: per window the sums  S1 = sum (Uhat-U)^2,
-// S2 = sum U, S4 = sum delta2 (U2p - 2 Uhat + U2m), S5 = sum Uhat  in FP64
-// (the delta1 sum is identically zero).  Each voxel value is acc / W (0 where
-// no patch covers it, the NaN -> 0 of src/svt.hpp:163-164).  Accumulators are
-// cleared as they are read.  Sums are reduced in a fixed order.
 __device__ __forceinline__ double fix_mean(unsigned long long& a, int32_t w) {
   const long long v = (long long)a;
   a = 0ull;
   return w > 0 ? ((double)v * kFixInv) / (double)w : 0.0;
-}
-
-__global__ __launch_bounds__(256) void k_risk(unsigned long long* __restrict__ acc, const int32_t* __restrict__ W,
-                                              const double* __restrict__ u64, const uint8_t* __restrict__ d2,
-                                              uint32_t vol, double d2neg, double d2pos, const uint32_t* __restrict__ wl,
-                                              double* __restrict__ partial) {
-  __shared__ double sm[4][256];
-  const uint32_t b = wl[blockIdx.y];
-  double s1 = 0, s2 = 0, s4 = 0, s5 = 0;
-  unsigned long long* A0 = acc + (size_t)b * 3 * vol;
-  unsigned long long* A1 = A0 + vol;
-  unsigned long long* A2 = A1 + vol;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < vol; v += gridDim.x * blockDim.x) {
-    const int32_t w = W[(size_t)b * vol + v];
-    const double uh = fix_mean(A0[v], w);
-    const double up = fix_mean(A1[v], w);
-    const double um = fix_mean(A2[v], w);
-    const double U = u64[(size_t)b * vol + v];
-    const double dl = d2[v] ? d2neg : d2pos;
-    const double e = fabs(uh - U);
-    s1 += e * e;
-    s2 += U;
-    s4 += dl * (up - 2 * uh + um);
-    s5 += uh;
-  }
-  sm[0][threadIdx.x] = s1;
-  sm[1][threadIdx.x] = s2;
-  sm[2][threadIdx.x] = s4;
-  sm[3][threadIdx.x] = s5;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o)
-      for (int k = 0; k < 4; ++k) sm[k][threadIdx.x] += sm[k][threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x < 4) partial[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = sm[threadIdx.x][0];
-}
-
-__global__ __launch_bounds__(256) void k_risk_reduce(const double* __restrict__ partial, uint32_t nblk,
-                                                     double* __restrict__ sums) {
-  __shared__ double sm[4][256];
-  const uint32_t b = blockIdx.x;
-  double s[4] = {0, 0, 0, 0};
-  for (uint32_t k = threadIdx.x; k < nblk; k += blockDim.x)
-    for (int c = 0; c < 4; ++c) s[c] += partial[((size_t)b * nblk + k) * 4 + c];
-  for (int c = 0; c < 4; ++c) sm[c][threadIdx.x] = s[c];
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o)
-      for (int c = 0; c < 4; ++c) sm[c][threadIdx.x] += sm[c][threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x < 4) sums[(size_t)b * 4 + threadIdx.x] = sm[threadIdx.x][0];
-}
-
-hipError_t launch_risk(unsigned long long* acc, const int32_t* W, const double* u64, const uint8_t* d2,
-                       const uint32_t* wl, uint32_t nb, const Shape& sh, double d2neg, double d2pos, double* partial,
-                       uint32_t nblk, double* sums, hipStream_t s) {
-  hipLaunchKernelGGL(k_risk, dim3(nblk, nb), dim3(256), 0, s, acc, W, u64, d2, sh.vol, d2neg, d2pos, wl, partial);
-  hipLaunchKernelGGL(k_risk_reduce, dim3(nb), dim3(256), 0, s, partial, nblk, sums);
-  return hipGetLastError();
 }
 
 // final frame: Y(:,:,t) = uMax * Recon(lambda)(:,:,ref)  (src/pguresvt.hpp:146-166)
@@ -764,7 +916,8 @@ __global__ void k_output(unsigned long long* __restrict__ acc, const int32_t* __
   if (pix >= fs) return;
   const WinMeta m = meta[b];
   const size_t v = (size_t)m.ref * fs + pix;
-  const double val = fix_mean(acc[(size_t)b * nvar_acc * vol + v], W[(size_t)b * vol + v]);
+  (void)nvar_acc;
+  const double val = fix_mean(acc[(size_t)b * vol + v], W[(size_t)b * vol + v]);
   Y[(size_t)m.t * fs + pix] = val * m.uMax;
 }
 

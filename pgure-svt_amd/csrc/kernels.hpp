@@ -44,10 +44,17 @@ hipError_t launch_recon(const float* factors, const float* u32, const int32_t* p
                         const WinMeta* meta, const uint32_t* wl, uint32_t nb, uint32_t nvar_stored,
                         uint32_t nvar_run, const Shape& sh, int exp_w, int only_ref, unsigned long long* acc,
                         hipStream_t s);
-// sums: double [nb][4] = (S1, S2, S4, S5) per listed slot, in list order
-hipError_t launch_risk(unsigned long long* acc, const int32_t* W, const double* u64, const uint8_t* d2,
-                       const uint32_t* wl, uint32_t nb, const Shape& sh, double d2neg, double d2pos, double* partial,
-                       uint32_t nblk, double* sums, hipStream_t s);
+// fused lambda evaluation: threshold + reconstruct all three variants, the
+// variant-0 volume into acc ([nb][vol]) and the linear risk sums; then the
+// volume sums.  sums: double [na][6] = (sum Uhat^2, sum U, sum U^2,
+// sum Uhat U, sum d2 (U2p - 2 Uhat + U2m), sum Uhat) per listed slot.
+uint32_t recon_risk_blocks(const Shape& sh);
+hipError_t launch_recon_risk(const float* factors, const float* u32, const int32_t* pos, const uint32_t* sel,
+                             const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int exp_w,
+                             const double* invw, const double* u64, const uint8_t* d2, double d2neg, double d2pos,
+                             unsigned long long* acc, double* partial_a, double* partial_b, uint32_t nblk_b,
+                             double* sums, hipStream_t s);
+hipError_t launch_invw(const int32_t* W, size_t n, double* invw, hipStream_t s);
 hipError_t launch_output(unsigned long long* acc, const int32_t* W, const WinMeta* meta, uint32_t nb,
                          uint32_t nvar_acc, const Shape& sh, double* Y, hipStream_t s);
 hipError_t launch_window_volume(unsigned long long* acc, const int32_t* W, uint32_t nvar_acc, const Shape& sh,

@@ -3,11 +3,13 @@
 // Same arithmetic, in the same order, as the reference: quadtree with the
 // 2.5% F-test, Huber IRLS robust mean, MAD robust variance of the
 // Laplacian-filtered leaf, robust WLS line fit of variance against mean.
-// Two changes make it fast without changing a single rounding:
+// Three changes make it fast without changing a single rounding:
 //   * the robust mean's interquartile distance is taken on r = A - m; since
-//     rounding is monotone, sorted(r)[k] = fl(sorted(A)[k] - m), so A is
-//     sorted once per leaf instead of once per IRLS iteration;
-//   * medians and order statistics use nth_element instead of full sorts.
+//     rounding is monotone, sorted(r)[k] = fl(sorted(A)[k] - m), so the two
+//     order statistics of A it needs are selected once per leaf instead of
+//     sorting r in every IRLS iteration;
+//   * medians and order statistics use nth_element instead of full sorts;
+//   * repeated quadtree nodes are evaluated once.
 // Leaf statistics of a frame are cached per (frame, normalisation).
 #include "noise.hpp"
 
@@ -64,15 +66,23 @@ inline void huber(const double* x, double* w, size_t n) {
   }
 }
 
-// RobustMeanEstimate (src/noise.hpp:238-271)
+// RobustMeanEstimate (src/noise.hpp:238-271).  The interquartile distance of
+// r = A - m only reads two order statistics of r, and sorted(r)[k] =
+// fl(sorted(A)[k] - m), so the two order statistics of A are selected once.
 double robust_mean(const std::vector<double>& A) {
   const size_t N = A.size();
-  std::vector<double> sorted(A);
-  std::sort(sorted.begin(), sorted.end());
   const uint32_t mq = iqd_m((uint32_t)N);
+  double a_hi, a_lo;
+  {
+    std::vector<double> c(A);
+    std::nth_element(c.begin(), c.begin() + (N - mq - 1), c.end());
+    a_hi = c[N - mq - 1];
+    std::nth_element(c.begin(), c.begin() + (mq - 1), c.begin() + (N - mq - 1));
+    a_lo = c[mq - 1];
+  }
   const double tol = 1e-6, eps = 1e-12;
   double m = 0, m0 = 1e12;
-  std::vector<double> w(N, 1.0), r(N);
+  std::vector<double> w(N, 1.0);
   for (uint32_t it = 0; it < 10000; ++it) {
     m = accu2f(N, [&](size_t k) { return w[k] * A[k]; });
     const double aux = accu2f(N, [&](size_t k) { return w[k]; });
@@ -80,10 +90,12 @@ double robust_mean(const std::vector<double>& A) {
     const double e = accu2f(N, [&](size_t k) { return std::fabs(A[k] - m); }) / (double)N;
     if (std::fabs(m0 - m) < tol || e < tol) break;
     m0 = m;
-    const double d = ((sorted[N - mq - 1] - m) - (sorted[mq - 1] - m)) + eps;
+    const double d = ((a_hi - m) - (a_lo - m)) + eps;
     const double inv = 1. / d;
-    for (size_t k = 0; k < N; ++k) r[k] = (A[k] - m) * inv;
-    huber(r.data(), w.data(), N);
+    for (size_t k = 0; k < N; ++k) {
+      const double a = std::fabs((A[k] - m) * inv);
+      w[k] = (a < 0.75) ? 1. : 0.75 / a;
+    }
   }
   return m;
 }
@@ -227,18 +239,30 @@ FrameLeaves frame_leaves(const double* F, uint32_t n) {
   FrameLeaves out;
   out.mean.reserve(tree.size());
   out.var.reserve(tree.size());
+  // the reference's tree holds repeated nodes; identical blocks give
+  // identical statistics, so each distinct block is evaluated once
+  std::vector<std::array<uint32_t, 3>> seen;
+  std::vector<std::pair<double, double>> stat;
   for (const auto& nd : tree) {
-    const uint32_t x = nd[0], y = nd[1], s = nd[2];
-    std::vector<double> col((size_t)s * s);
-    for (uint32_t c = 0; c < s; ++c)
-      std::memcpy(&col[(size_t)s * c], &F[x + (size_t)n * (y + c)], s * sizeof(double));
-    out.mean.push_back(robust_mean(col));
-    out.var.push_back(robust_var(laplacian_values(col, s)));
+    size_t hit = seen.size();
+    for (size_t k = 0; k < seen.size(); ++k)
+      if (seen[k] == nd) hit = k;
+    if (hit == seen.size()) {
+      const uint32_t x = nd[0], y = nd[1], s = nd[2];
+      std::vector<double> col((size_t)s * s);
+      for (uint32_t c = 0; c < s; ++c)
+        std::memcpy(&col[(size_t)s * c], &F[x + (size_t)n * (y + c)], s * sizeof(double));
+      seen.push_back(nd);
+      stat.emplace_back(robust_mean(col), robust_var(laplacian_values(col, s)));
+    }
+    out.mean.push_back(stat[hit].first);
+    out.var.push_back(stat[hit].second);
   }
   return out;
 }
 
-std::shared_ptr<const FrameLeaves> LeafCache::get(uint32_t frame, double scale, const double* f, uint32_t n) {
+std::shared_ptr<const FrameLeaves> LeafCache::get(uint32_t frame, double scale, uint32_t n,
+                                                  const std::function<void(double*)>& fill) {
   uint64_t bits;
   std::memcpy(&bits, &scale, 8);
   const uint64_t key = (uint64_t)frame * 0x9E3779B97F4A7C15ULL ^ bits;
@@ -247,23 +271,19 @@ std::shared_ptr<const FrameLeaves> LeafCache::get(uint32_t frame, double scale, 
     auto it = map_.find(key);
     if (it != map_.end() && scale_of_[key] == scale) return it->second;
   }
-  auto r = std::make_shared<const FrameLeaves>(frame_leaves(f, n));
+  std::vector<double> buf((size_t)n * n);
+  fill(buf.data());
+  auto r = std::make_shared<const FrameLeaves>(frame_leaves(buf.data(), n));
   std::lock_guard<std::mutex> g(mu_);
   map_[key] = r;
   scale_of_[key] = scale;
   return r;
 }
 
-void estimate_noise(const double* u, uint32_t n, uint32_t nt, uint32_t method, double& alpha, double& mu,
-                    double& sigma, LeafCache* cache, const uint32_t* frame_ids, double scale) {
+void estimate_from_leaves(const std::vector<std::shared_ptr<const FrameLeaves>>& frames, uint32_t n,
+                          uint32_t method, double& alpha, double& mu, double& sigma) {
   std::vector<double> means, vars;
-  const size_t fsz = (size_t)n * n;
-  for (uint32_t k = 0; k < nt; ++k) {
-    std::shared_ptr<const FrameLeaves> lv;
-    if (cache && frame_ids)
-      lv = cache->get(frame_ids[k], scale, u + k * fsz, n);
-    else
-      lv = std::make_shared<const FrameLeaves>(frame_leaves(u + k * fsz, n));
+  for (const auto& lv : frames) {
     for (double v : lv->mean)
       if (v >= 0.) means.push_back(v);
     for (double v : lv->var)
@@ -307,6 +327,14 @@ void estimate_noise(const double* u, uint32_t n, uint32_t nt, uint32_t method, d
       break;
     }
   }
+}
+
+void estimate_noise(const double* u, uint32_t n, uint32_t nt, uint32_t method, double& alpha, double& mu,
+                    double& sigma) {
+  std::vector<std::shared_ptr<const FrameLeaves>> frames;
+  for (uint32_t k = 0; k < nt; ++k)
+    frames.push_back(std::make_shared<const FrameLeaves>(frame_leaves(u + (size_t)k * n * n, n)));
+  estimate_from_leaves(frames, n, method, alpha, mu, sigma);
 }
 
 }  // namespace pgsvt

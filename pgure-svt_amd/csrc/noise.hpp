@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <unordered_map>
@@ -19,7 +20,10 @@ struct FrameLeaves {
 // maximum it is normalised by, so windows that share both share the result.
 class LeafCache {
  public:
-  std::shared_ptr<const FrameLeaves> get(uint32_t frame, double scale, const double* frame_u, uint32_t n);
+  // leaves of `frame` normalised by `scale`; `fill(buf)` writes the
+  // normalised n x n frame into buf and is only called on a miss
+  std::shared_ptr<const FrameLeaves> get(uint32_t frame, double scale, uint32_t n,
+                                         const std::function<void(double*)>& fill);
 
  private:
   std::mutex mu_;
@@ -31,10 +35,14 @@ class LeafCache {
 // each n x n and already normalised (frame k at u + k*n*n).  alpha/mu/sigma
 // hold user values on entry (< 0 = estimate) and the estimates on return.
 void estimate_noise(const double* u, uint32_t n, uint32_t nt, uint32_t method, double& alpha, double& mu,
-                    double& sigma, LeafCache* cache = nullptr, const uint32_t* frame_ids = nullptr,
-                    double scale = 0.0);
+                    double& sigma);
 
 // Leaf statistics of one normalised n x n frame.
 FrameLeaves frame_leaves(const double* f, uint32_t n);
+
+// The window-level part of the estimate (robust line fit and the noise
+// method) from the leaf statistics of its frames, in frame order.
+void estimate_from_leaves(const std::vector<std::shared_ptr<const FrameLeaves>>& frames, uint32_t n,
+                          uint32_t method, double& alpha, double& mu, double& sigma);
 
 }  // namespace pgsvt
