@@ -402,12 +402,11 @@ struct Pipeline {
   uint32_t NB = 0;
   DBuf<WinMeta> meta_d;
   DBuf<uint32_t> wl_d;
-  DBuf<double> u64, w64, invw, partial, partial_b, sums;
-  DBuf<float> u32, factors;
-  DBuf<int32_t> pos, W;
+  DBuf<double> u64, w64, invw, dw, partial, partial_b, sums;
+  DBuf<float> u32, factors, B0;
+  DBuf<int32_t> pos, cnt, off, list, inv;
   DBuf<int16_t> jobs_d;
-  DBuf<unsigned long long> acc;
-  uint32_t max_jobs = 0, risk_blocks = 0;
+  uint32_t max_jobs = 0, gather_blocks = 0;
   DBuf<uint32_t> stats;
 
   ~Pipeline() {
@@ -440,7 +439,8 @@ struct Pipeline {
     sh.p_all = sh.n1 * sh.n1;
     sel = patch_indices(nr, sh.bs, p.patch_overlap);
     sh.p_sel = (uint32_t)sel.size();
-    sh.fstride = (g.Nw * g.Nw + g.Nw + 3) & ~3u;
+    const uint32_t sr = (g.Nw + 3) & ~3u;  // padded V^T row
+    sh.fstride = (g.Nw * sr + g.Nw * sh.m + g.Nw + 3) & ~3u;
     const double vP = 0.5 + 0.5 * 1. / std::sqrt(1. * 1. + 4);
     const double vQ = 1 - vP;
     d2neg = -1 * std::sqrt(vQ / vP);
@@ -509,9 +509,10 @@ struct Pipeline {
     if (median) b += 8ull * sh.vol;                      // w64
     b += 4ull * sh.vol * nvar;                           // u32
     b += 4ull * sh.nw * sh.p_all;                        // pos
-    b += 4ull * sh.vol;                                  // W
-    b += 8ull * sh.vol;                                  // acc (variant 0)
-    b += 8ull * sh.vol;                                  // invw
+    b += 8ull * sh.nw * ((size_t)sh.n1 * sh.n1 + 1);     // bucket counts + offsets
+    b += 8ull * sh.nw * sh.p_sel;                        // bucket lists + inverse
+    b += 8ull * sh.vol * (optimize ? 2 : 1);             // 1/W, delta2/W
+    b += 4ull * sh.nw * (size_t)sh.p_sel * sh.m;         // B0
     b += 4ull * nvar * (size_t)sh.p_sel * sh.fstride;    // factors
     return b;
   }
@@ -529,18 +530,22 @@ struct Pipeline {
     if (median) w64.alloc((size_t)NB * sh.vol);
     u32.alloc((size_t)NB * sh.vol * nvar);
     pos.alloc((size_t)NB * sh.nw * sh.p_all);
-    W.alloc((size_t)NB * sh.vol);
-    acc.alloc((size_t)NB * sh.vol);
+    const size_t npos = (size_t)sh.n1 * sh.n1;
+    cnt.alloc((size_t)NB * sh.nw * npos);
+    off.alloc((size_t)NB * sh.nw * (npos + 1));
+    list.alloc((size_t)NB * sh.nw * sh.p_sel);
+    inv.alloc((size_t)NB * sh.nw * sh.p_sel);
     invw.alloc((size_t)NB * sh.vol);
+    if (optimize) dw.alloc((size_t)NB * sh.vol);
+    B0.alloc((size_t)NB * sh.nw * sh.p_sel * sh.m);
     factors.alloc((size_t)NB * nvar * sh.p_sel * sh.fstride);
-    risk_blocks = std::min<uint32_t>(256, (sh.vol + 255) / 256);
-    partial.alloc((size_t)NB * recon_risk_blocks(sh) * 3);
-    partial_b.alloc((size_t)NB * risk_blocks * 3);
+    gather_blocks = pgsvt::gather_blocks(sh, 0);
+    partial.alloc((size_t)NB * eval_blocks(sh, 3));
+    partial_b.alloc((size_t)NB * gather_blocks * 5);
     sums.alloc((size_t)NB * 6);
     max_jobs = 2 * g.Nw;
     jobs_d.alloc((size_t)NB * max_jobs * 4);
     stats.alloc(1);
-    hipchk(hipMemsetAsync(acc.p, 0, acc.bytes(), s), "memset acc");
     hipchk(hipMemsetAsync(stats.p, 0, stats.bytes(), s), "memset stats");
   }
 
@@ -569,9 +574,10 @@ struct Pipeline {
                "arps");
       });
     }
-    hipchk(hipMemsetAsync(W.p, 0, (size_t)nb * sh.vol * sizeof(int32_t), s), "memset W");
-    hipchk(launch_count(pos.p, sel_d.p, nb, sh, W.p, s), "count");
-    hipchk(launch_invw(W.p, (size_t)nb * sh.vol, invw.p, s), "invw");
+    prof->run("buckets_weights", (double)nb * sh.nw * sh.p_sel, [&] {
+      hipchk(launch_buckets(pos.p, sel_d.p, nb, sh, cnt.p, off.p, list.p, inv.p, s), "buckets");
+      hipchk(launch_weights(off.p, d2.p, nb, sh, d2neg, d2pos, invw.p, optimize ? dw.p : nullptr, s), "weights");
+    });
     // units: matrices decomposed
     prof->run("svd_jacobi", (double)nb * nvar_run * sh.p_sel, [&] {
       hipchk(launch_svd(u32.p, pos.p, sel_d.p, nb, nvar_run, sh, factors.p, 30, 1e-6f, stats.p, s), "svd");
@@ -594,16 +600,21 @@ struct Pipeline {
   }
 
   // evaluate the risk of the listed batch slots at their meta.lambda
-  void eval_risk(std::vector<WinMeta>& ms, const std::vector<uint32_t>& list, std::vector<double>& out) {
-    const uint32_t na = (uint32_t)list.size();
+  void eval_risk(std::vector<WinMeta>& ms, const std::vector<uint32_t>& list_, std::vector<double>& out) {
+    const uint32_t na = (uint32_t)list_.size();
     hipchk(hipMemcpyAsync(meta_d.p, ms.data(), ms.size() * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
-    hipchk(hipMemcpyAsync(wl_d.p, list.data(), na * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
+    hipchk(hipMemcpyAsync(wl_d.p, list_.data(), na * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
     // units: thresholded patch reconstructions (patch x variant)
-    prof->run("recon_risk", (double)na * 3 * sh.p_sel, [&] {
-      hipchk(launch_recon_risk(factors.p, u32.p, pos.p, sel_d.p, meta_d.p, wl_d.p, na, sh, p.exponential_weighting,
-                               invw.p, u64.p, d2.p, d2neg, d2pos, acc.p, partial.p, partial_b.p, risk_blocks,
-                               sums.p, s),
-             "recon_risk");
+    prof->run("eval_reconstruct", (double)na * 3 * sh.p_sel, [&] {
+      hipchk(launch_eval(factors.p, pos.p, sel_d.p, meta_d.p, wl_d.p, na, nvar, 3, sh,
+                         p.exponential_weighting, dw.p, inv.p, B0.p, partial.p, s),
+             "eval");
+    });
+    // units: voxels
+    prof->run("gather_risk", (double)na * sh.vol, [&] {
+      hipchk(launch_gather(B0.p, off.p, invw.p, u64.p, meta_d.p, wl_d.p, na, sh, 0, nullptr, partial_b.p, s),
+             "gather");
+      hipchk(launch_risk_reduce(partial.p, eval_blocks(sh, 3), partial_b.p, gather_blocks, na, sums.p, s), "reduce");
     });
     out.resize((size_t)na * 6);
     hipchk(hipMemcpyAsync(out.data(), sums.p, (size_t)na * 6 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H sums");
@@ -611,19 +622,20 @@ struct Pipeline {
     prof->flush();
   }
 
-  // final frames of the batch into Yd
+  // final frames of the batch into Yout
   void finish(std::vector<WinMeta>& ms) {
     const uint32_t nb = (uint32_t)ms.size();
-    std::vector<uint32_t> list(nb);
-    for (uint32_t b = 0; b < nb; ++b) list[b] = b;
+    std::vector<uint32_t> lst(nb);
+    for (uint32_t b = 0; b < nb; ++b) lst[b] = b;
     hipchk(hipMemcpyAsync(meta_d.p, ms.data(), nb * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
-    hipchk(hipMemcpyAsync(wl_d.p, list.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
-    prof->run("recon_final_frame", (double)nb * sh.p_sel, [&] {
-      hipchk(launch_recon(factors.p, u32.p, pos.p, sel_d.p, meta_d.p, wl_d.p, nb, nvar, 1, sh,
-                          p.exponential_weighting, 1, acc.p, s),
-             "recon final");
+    hipchk(hipMemcpyAsync(wl_d.p, lst.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
+    prof->run("final_frame", (double)nb * sh.p_sel, [&] {
+      hipchk(launch_eval(factors.p, pos.p, sel_d.p, meta_d.p, wl_d.p, nb, nvar, 1, sh,
+                         p.exponential_weighting, nullptr, inv.p, B0.p, nullptr, s),
+             "eval final");
+      hipchk(launch_gather(B0.p, off.p, invw.p, u64.p, meta_d.p, wl_d.p, nb, sh, 1, Yout, nullptr, s),
+             "gather final");
     });
-    hipchk(launch_output(acc.p, W.p, meta_d.p, nb, nvar, sh, Yout, s), "output");
     hipchk(hipStreamSynchronize(s), "finish sync");
     prof->flush();
   }
@@ -939,12 +951,14 @@ int pgsvt_window_reconstruct(const void* X, int dtype, uint32_t n_rows, uint32_t
     const uint32_t zero = 0;
     hipchk(hipMemcpyAsync(P.meta_d.p, ms.data(), sizeof(WinMeta), hipMemcpyHostToDevice, P.s), "H2D meta");
     hipchk(hipMemcpyAsync(P.wl_d.p, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, P.s), "H2D list");
-    hipchk(launch_recon(P.factors.p, P.u32.p, P.pos.p, P.sel_d.p, P.meta_d.p, P.wl_d.p, 1, P.nvar, 1, P.sh,
-                        P.p.exponential_weighting, 0, P.acc.p, P.s),
-           "recon");
     DBuf<double> out;
     out.alloc(P.sh.vol);
-    hipchk(launch_window_volume(P.acc.p, P.W.p, P.nvar, P.sh, out.p, P.s), "volume");
+    hipchk(launch_eval(P.factors.p, P.pos.p, P.sel_d.p, P.meta_d.p, P.wl_d.p, 1, P.nvar, 1, P.sh,
+                       P.p.exponential_weighting, nullptr, P.inv.p, P.B0.p, nullptr, P.s),
+           "eval");
+    hipchk(launch_gather(P.B0.p, P.off.p, P.invw.p, P.u64.p, P.meta_d.p, P.wl_d.p, 1, P.sh, 2, out.p,
+                         nullptr, P.s),
+           "gather");
     hipchk(hipMemcpyAsync(vol, out.p, out.bytes(), hipMemcpyDeviceToHost, P.s), "D2H vol");
     hipchk(hipStreamSynchronize(P.s), "sync");
     return 0;
@@ -961,9 +975,10 @@ int pgsvt_window_singular_values(const void* X, int dtype, uint32_t n_rows, uint
     hipchk(hipMemcpyAsync(f.data(), P.factors.p, f.size() * 4, hipMemcpyDeviceToHost, P.s), "D2H factors");
     hipchk(hipStreamSynchronize(P.s), "sync");
     const uint32_t n = P.sh.nw;
+    const uint32_t s_at = n * ((n + 3) & ~3u) + n * P.sh.m;  // after V^T and W^T
     for (uint32_t k = 0; k < P.sh.p_sel; ++k) {
       std::vector<double> v(n);
-      for (uint32_t j = 0; j < n; ++j) v[j] = f[(size_t)k * P.sh.fstride + n * n + j];
+      for (uint32_t j = 0; j < n; ++j) v[j] = f[(size_t)k * P.sh.fstride + s_at + j];
       std::sort(v.begin(), v.end(), [](double a, double b) { return a > b; });
       std::memcpy(s + (size_t)k * n, v.data(), n * sizeof(double));
     }

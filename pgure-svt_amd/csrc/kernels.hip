@@ -399,42 +399,10 @@ hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs
   return hipGetLastError();
 }
 
-// ------------------------------------------------------- overlap counts --
-// weights of src/svt.hpp:157-159: one per patch covering a voxel
-template <int BS>
-__global__ void k_count(const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel, uint32_t p_sel,
-                        uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim,
-                        int32_t* __restrict__ W) {
-  const uint32_t b = blockIdx.z, k = blockIdx.y;
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= p_sel) return;
-  const int32_t pk = pos[((size_t)b * nw + k) * p_all + sel[p]];
-  const uint32_t row = min((uint32_t)(pk & 0xffff), lim), col = min((uint32_t)(pk >> 16), lim);
-  int32_t* Wk = W + (size_t)b * vol + (size_t)k * fs + row + (size_t)nr * col;
-#pragma unroll
-  for (int q = 0; q < BS * BS; ++q) atomicAdd(Wk + (q % BS) + (size_t)nr * (q / BS), 1);
-}
-
-hipError_t launch_count(const int32_t* pos, const uint32_t* sel, uint32_t nb, const Shape& sh, int32_t* W,
-                        hipStream_t s) {
-  const dim3 grid((sh.p_sel + 255) / 256, sh.nw, nb);
-  const uint32_t lim = sh.nr - sh.bs;
-#define PG_CNT(B)                                                                                                 \
-  case B:                                                                                                         \
-    hipLaunchKernelGGL(k_count<B>, grid, dim3(256), 0, s, pos, sel, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw,      \
-                       sh.vol, lim, W);                                                                           \
-    break;
-  switch (sh.bs) {
-    PG_CNT(2) PG_CNT(3) PG_CNT(4) PG_CNT(5) PG_CNT(6) PG_CNT(7) PG_CNT(8)
-    default: return hipErrorInvalidValue;
-  }
-#undef PG_CNT
-  return hipGetLastError();
-}
-
 // --------------------------------------------------- batched Jacobi SVD --
 // One-sided (Hestenes) Jacobi on the m x n Casorati matrix of each patch
-// (arma::svd_econ at src/svt.hpp:111).  A group of L lanes holds one matrix,
+// (arma::svd_econ at src/svt.hpp:111).  Output per patch, sorted by
+// descending singular value: V^T (n x n), W^T = (X V)^T (n x m) and s (n).  A group of L lanes holds one matrix,
 // lane q owning row q of X and row q of V; column dot products are
 // butterflies across the group (DPP inside a 16-lane row, then lane swaps
 // for 32/64-lane groups).  Every lane of a group ends the butterfly with the
@@ -529,278 +497,379 @@ __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ u32, cons
     if (!__any(tsum > 0.f)) break;
   }
   if (stats && lane == 0 && valid) atomicMax(stats, (uint32_t)sweep + 1);
+  // final column norms (identical in every lane of the group)
+#pragma unroll
+  for (int k = 0; k < N; ++k) a[k] = group_sum<L>(x[k] * x[k]);
   if (!valid) return;
+  // Store in descending singular-value order, transposed: row r of V^T is
+  // the right singular vector of the r-th largest singular value (row
+  // stride SR, padded with zeros), row r of W^T is the matching column of
+  // W = X V = U S (the rotated Casorati columns), then the sorted values.
+  // rank(l) counts the values that sort before column l (ties by index).
+  constexpr int SR = (N + 3) & ~3;
   float* F = factors + (((size_t)b * nvar + var) * p_sel + p) * fstride;
-  float sq = 0.f;
+  float* FW = F + N * SR;
+  float* FS = FW + N * M;
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    const float nk = group_sum<L>(x[k] * x[k]);
-    if (lane == k) sq = sqrtf(nk);
-  }
-  if (lane < N) {
+  for (int l = 0; l < N; ++l) {
+    int rank = 0;
 #pragma unroll
-    for (int k = 0; k < N; ++k) F[lane * N + k] = v[k];
-    F[N * N + lane] = sq;
+    for (int m = 0; m < N; ++m) rank += (a[m] > a[l] || (a[m] == a[l] && m < l)) ? 1 : 0;
+    if (lane < N) F[rank * SR + lane] = v[l];
+    if (lane >= N && lane < SR) F[rank * SR + lane] = 0.f;
+    if (lane < M) FW[rank * M + lane] = x[l];
+    if (lane == 0) FS[rank] = sqrtf(a[l]);
   }
 }
 
-// ------------------------------------------ threshold + reconstruction --
-// src/svt.hpp:121-167 with src/utils.hpp:96-106.  B = U diag(g(s)) V^T is
-// formed as (X V) diag(g(s)/s) V^T from the stored V and s and the original
-// Casorati matrix, so only n*n+n floats per patch are kept.  Each lane
-// reconstructs its row of B and adds it into the fixed-point accumulators.
-template <int BS, int N, int L>
-__global__ __launch_bounds__(256) void k_recon(const float* __restrict__ factors, const float* __restrict__ u32,
-                                               const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel,
-                                               const WinMeta* __restrict__ meta, uint32_t nvar_stored,
-                                               uint32_t nvar_run, uint32_t p_sel, uint32_t p_all, uint32_t nr,
-                                               uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim, uint32_t fstride,
-                                               int exp_w, int only_ref, const uint32_t* __restrict__ wl,
-                                               unsigned long long* __restrict__ acc) {
-  constexpr int M = BS * BS, G = 256 / L, VS = N * N;
-  __shared__ float sV[G * VS];
-  __shared__ float sS[G * N];
-  const uint32_t b = wl[blockIdx.y];
-  const int grp = threadIdx.x / L, lane = threadIdx.x % L;
-  const uint32_t g = blockIdx.x * G + grp;
-  const uint32_t var = g / p_sel, p = g % p_sel;
-  const bool valid = var < nvar_run;
-  if (valid && lane < N) {
-    const float* F = factors + (((size_t)b * nvar_stored + var) * p_sel + p) * fstride;
-#pragma unroll
-    for (int k = 0; k < N; ++k) sV[grp * VS + lane * N + k] = F[lane * N + k];
-    sS[grp * N + lane] = F[N * N + lane];
-  }
+// ---------------------------------------------------- position buckets --
+// The overlap-add of src/svt.hpp:148-160 is done as a gather.  For every
+// window and frame the selected patches are bucketed by their position in
+// that frame (a counting sort, built once per window since positions do
+// not depend on lambda).  A voxel then sums, in a fixed order, the
+// contributions of the patches in the bs x bs positions that cover it.
+__global__ void k_bucket_count(const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel, uint32_t p_sel,
+                               uint32_t p_all, uint32_t nw, uint32_t n1, int32_t* __restrict__ cnt) {
+  const uint32_t b = blockIdx.z, k = blockIdx.y;
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= p_sel) return;
+  const int32_t pk = pos[((size_t)b * nw + k) * p_all + sel[p]];
+  const uint32_t r = min((uint32_t)(pk & 0xffff), n1 - 1), c = min((uint32_t)(pk >> 16), n1 - 1);
+  atomicAdd(cnt + ((size_t)b * nw + k) * n1 * n1 + r + (size_t)n1 * c, 1);
+}
+
+// exclusive scan of one (window, frame) count table into offsets
+__global__ __launch_bounds__(1024) void k_bucket_scan(const int32_t* __restrict__ cnt, uint32_t npos,
+                                                      int32_t* __restrict__ off) {
+  __shared__ int32_t part[1024];
+  const size_t tab = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+  const int32_t* C = cnt + tab * npos;
+  int32_t* O = off + tab * (npos + 1);
+  const uint32_t chunk = (npos + 1023) / 1024;
+  const uint32_t lo = min(threadIdx.x * chunk, npos), hi = min(lo + chunk, npos);
+  int32_t sum = 0;
+  for (uint32_t i = lo; i < hi; ++i) sum += C[i];
+  part[threadIdx.x] = sum;
   __syncthreads();
-  if (!valid || lane >= M) return;
+  for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
+    const int32_t v = threadIdx.x >= (uint32_t)o ? part[threadIdx.x - o] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+  for (uint32_t i = lo; i < hi; ++i) {
+    O[i] = run;
+    run += C[i];
+  }
+  if (threadIdx.x == 1023) O[npos] = part[1023];
+}
 
-  const double lam = meta[b].lambda;
-  float s[N], d[N];
-  float smax = 0.f;
-#pragma unroll
-  for (int l = 0; l < N; ++l) {
-    s[l] = sS[grp * N + l];
-    smax = fmaxf(smax, s[l]);
-  }
-  if (exp_w) {
-    const float kk = (float)(-0.5 * lam);
-#pragma unroll
-    for (int l = 0; l < N; ++l) {
-      const float w = fabsf(smax * expf(kk * (s[l] * s[l])));
-      const float gl = fmaxf(s[l] - w, 0.f);
-      d[l] = s[l] > 0.f ? gl / s[l] : 0.f;
-    }
-  } else {
-    const float lf = (float)lam;
-#pragma unroll
-    for (int l = 0; l < N; ++l) {
-      const float gl = fmaxf(s[l] - lf, 0.f);
-      d[l] = s[l] > 0.f ? gl / s[l] : 0.f;
-    }
-  }
+__global__ void k_bucket_fill(const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel, uint32_t p_sel,
+                              uint32_t p_all, uint32_t nw, uint32_t n1, const int32_t* __restrict__ off,
+                              int32_t* __restrict__ fill, int32_t* __restrict__ list) {
+  const uint32_t b = blockIdx.z, k = blockIdx.y;
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= p_sel) return;
+  const size_t tab = (size_t)b * nw + k;
+  const int32_t pk = pos[tab * p_all + sel[p]];
+  const uint32_t r = min((uint32_t)(pk & 0xffff), n1 - 1), c = min((uint32_t)(pk >> 16), n1 - 1);
+  const size_t q = r + (size_t)n1 * c;
+  const int32_t slot = atomicAdd(fill + tab * n1 * n1 + q, 1);
+  list[tab * p_sel + off[tab * (n1 * n1 + 1) + q] + slot] = (int32_t)p;
+}
 
-  const uint32_t it = sel[p];
-  const int rr = lane % BS, cc = lane / BS;
-  const float* U = u32 + ((size_t)b * nvar_stored + var) * vol;
-  float x[N];
-  uint32_t addr[N];
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
-    const uint32_t row = min((uint32_t)(pk & 0xffff), lim), col = min((uint32_t)(pk >> 16), lim);
-    addr[k] = (uint32_t)k * fs + (row + rr) + nr * (col + cc);
-    x[k] = U[addr[k]];
-  }
-  float y[N];
-#pragma unroll
-  for (int l = 0; l < N; ++l) {
-    float t = 0.f;
-#pragma unroll
-    for (int j = 0; j < N; ++j) t = fmaf(x[j], sV[grp * VS + j * N + l], t);
-    y[l] = t * d[l];
-  }
-  unsigned long long* A = acc + (size_t)b * vol;  // variant-0 accumulator
-  const int ref = (int)meta[b].ref;
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    if (only_ref && k != ref) continue;
-    float bq = 0.f;
-#pragma unroll
-    for (int l = 0; l < N; ++l) bq = fmaf(y[l], sV[grp * VS + k * N + l], bq);
-    const long long q = __float2ll_rn(bq * kFixScale);
-    atomicAdd(A + addr[k], (unsigned long long)q);
+// ascending patch order inside every bucket (fixed summation order)
+__global__ void k_bucket_sort(const int32_t* __restrict__ off, uint32_t npos, uint32_t p_sel,
+                              int32_t* __restrict__ list) {
+  const size_t tab = blockIdx.y;
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npos) return;
+  const int32_t* O = off + tab * (npos + 1);
+  int32_t* Lst = list + tab * p_sel;
+  const int32_t lo = O[q], hi = O[q + 1];
+  for (int32_t i = lo + 1; i < hi; ++i) {
+    const int32_t v = Lst[i];
+    int32_t j = i - 1;
+    while (j >= lo && Lst[j] > v) {
+      Lst[j + 1] = Lst[j];
+      --j;
+    }
+    Lst[j + 1] = v;
   }
 }
 
-// --------------------------------------- fused reconstruction + risk --
-// One lambda evaluation of PGURE<T>::CalculatePGURE (src/pgure.hpp:128-136)
-// for every patch of a window.  A group reconstructs its patch for the three
-// decomposed variants (U, U + eps2 delta2, U - eps2 delta2).  Only the
-// unperturbed reconstruction is needed as a volume (for sum Uhat^2); every
-// other risk term is linear in the overlap-added values, so with
-// Uhat_v = acc_v / W_v it is accumulated per contribution:
-//   sum Uhat                  = sum_c B0_c / W_v
-//   sum Uhat U                = sum_c B0_c U_v / W_v
-//   sum d2 (U2p - 2Uhat + U2m) = sum_c d2_v (B1_c - 2 B0_c + B2_c) / W_v
-// (c runs over patch contributions, v is the voxel c lands on).  Only B0 is
-// scattered into the fixed-point volume; the sums are FP64 per thread, then
-// reduced in a fixed order.
+// inv[p] = place of patch p in its frame's bucket-sorted order
+__global__ void k_bucket_inverse(const int32_t* __restrict__ list, uint32_t p_sel, int32_t* __restrict__ inv) {
+  const size_t tab = blockIdx.y;
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= p_sel) return;
+  inv[tab * p_sel + list[tab * p_sel + j]] = (int32_t)j;
+}
+
+hipError_t launch_buckets(const int32_t* pos, const uint32_t* sel, uint32_t nb, const Shape& sh, int32_t* cnt,
+                          int32_t* off, int32_t* list, int32_t* inv, hipStream_t s) {
+  const uint32_t npos = sh.n1 * sh.n1;
+  const size_t tabs = (size_t)nb * sh.nw;
+  hipError_t e = hipMemsetAsync(cnt, 0, tabs * npos * sizeof(int32_t), s);
+  if (e != hipSuccess) return e;
+  const dim3 g1((sh.p_sel + 255) / 256, sh.nw, nb);
+  hipLaunchKernelGGL(k_bucket_count, g1, dim3(256), 0, s, pos, sel, sh.p_sel, sh.p_all, sh.nw, sh.n1, cnt);
+  hipLaunchKernelGGL(k_bucket_scan, dim3(sh.nw, nb), dim3(1024), 0, s, cnt, npos, off);
+  e = hipMemsetAsync(cnt, 0, tabs * npos * sizeof(int32_t), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bucket_fill, g1, dim3(256), 0, s, pos, sel, sh.p_sel, sh.p_all, sh.nw, sh.n1, off, cnt, list);
+  hipLaunchKernelGGL(k_bucket_sort, dim3((npos + 255) / 256, (unsigned)tabs), dim3(256), 0, s, off, npos, sh.p_sel,
+                     list);
+  hipLaunchKernelGGL(k_bucket_inverse, dim3((sh.p_sel + 255) / 256, (unsigned)tabs), dim3(256), 0, s, list, sh.p_sel,
+                     inv);
+  return hipGetLastError();
+}
+
+// weights of src/svt.hpp:157-163: W = number of patches covering a voxel
+// (from the bucket sizes), stored as 1/W (0 where nothing covers it) and
+// as delta2/W for the divergence sum.
+__global__ void k_weights(const int32_t* __restrict__ off, const uint8_t* __restrict__ d2, uint32_t nr, uint32_t fs,
+                          uint32_t vol, uint32_t n1, uint32_t bs, double d2neg, double d2pos,
+                          double* __restrict__ invw, double* __restrict__ dw) {
+  const uint32_t b = blockIdx.y;
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= vol) return;
+  const uint32_t k = v / fs, pix = v % fs;
+  const int r = (int)(pix % nr), c = (int)(pix / nr);
+  const int32_t* O = off + ((size_t)b * (vol / fs) + k) * ((size_t)n1 * n1 + 1);
+  int32_t w = 0;
+  for (int cb = 0; cb < (int)bs; ++cb) {
+    const int cc = c - cb;
+    if (cc < 0 || cc >= (int)n1) continue;
+    for (int ra = 0; ra < (int)bs; ++ra) {
+      const int rr = r - ra;
+      if (rr < 0 || rr >= (int)n1) continue;
+      const size_t q = (size_t)rr + (size_t)n1 * cc;
+      w += O[q + 1] - O[q];
+    }
+  }
+  const double iw = w > 0 ? 1.0 / (double)w : 0.0;
+  invw[(size_t)b * vol + v] = iw;
+  if (dw) dw[(size_t)b * vol + v] = (d2[v] ? d2neg : d2pos) * iw;
+}
+
+hipError_t launch_weights(const int32_t* off, const uint8_t* d2, uint32_t nb, const Shape& sh, double d2neg,
+                          double d2pos, double* invw, double* dw, hipStream_t s) {
+  hipLaunchKernelGGL(k_weights, dim3((sh.vol + 255) / 256, nb), dim3(256), 0, s, off, d2, sh.nr, sh.fs, sh.vol, sh.n1,
+                     sh.bs, d2neg, d2pos, invw, dw);
+  return hipGetLastError();
+}
+
+// --------------------------------------------- threshold + reconstruct --
+// One lambda evaluation of the thresholded reconstruction of every patch
+// (src/svt.hpp:121-146 with src/utils.hpp:96-106):
+//   B = U diag(g(s)) V^T = sum over kept ranks r of (g(s_r)/s_r) w_r v_r^T
+// with w_r = X v_r = s_r u_r, stored by the SVD.  Factors are sorted by
+// descending s and g(s)/s grows with s, so the kept ranks are a prefix and
+// only their rows are read.  A group of L lanes owns one (patch, variant),
+// lane q one row of B.
+// nvar_run == 3 (PGURE): the groups of a block take (patch, variant) pairs
+// for the variants U, U + eps2 delta2, U - eps2 delta2; the U group stores
+// its B in `B0` ([slot][frame][patch in bucket order][row]) for the gather
+// and adds sum_c (B1_c - 2 B0_c + B2_c) delta2_v / W_v, the divergence sum
+// of src/pgure.hpp:136 (linear in the overlap-add), reduced in FP64.
+// nvar_run == 1: every group reconstructs U of its own patch.
 template <int BS, int N, int L>
-__global__ __launch_bounds__(256) void k_recon_risk(
-    const float* __restrict__ factors, const float* __restrict__ u32, const int32_t* __restrict__ pos,
-    const uint32_t* __restrict__ sel, const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl,
+__global__ __launch_bounds__(256) void k_eval(
+    const float* __restrict__ factors, const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel,
+    const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl, uint32_t nvar_stored, uint32_t nvar_run,
     uint32_t p_sel, uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim,
-    uint32_t fstride, int exp_w, const double* __restrict__ invw, const double* __restrict__ u64,
-    const uint8_t* __restrict__ d2, double d2neg, double d2pos, unsigned long long* __restrict__ acc,
-    double* __restrict__ partial) {
-  constexpr int M = BS * BS, G = 256 / L, VS = N * N;
-  __shared__ float sV[G * VS];
-  __shared__ float sS[G * N];
-  __shared__ double red[3][256];
+    uint32_t fstride, int exp_w, const double* __restrict__ dw, const int32_t* __restrict__ inv,
+    float* __restrict__ B0, double* __restrict__ partial) {
+  constexpr int M = BS * BS, G = 256 / L, SR = (N + 3) & ~3, P3 = G / 3;
+  __shared__ float xch[P3][2][M][N];  // reconstructions of the two perturbed variants
+  __shared__ double red[4];
   const uint32_t slot = blockIdx.y, b = wl[slot];
   const int grp = threadIdx.x / L, lane = threadIdx.x % L;
-  const uint32_t p = blockIdx.x * G + grp;
-  const bool valid = p < p_sel;
+  int pl, var;
+  uint32_t p;
+  if (nvar_run == 3) {
+    pl = grp / 3;
+    var = (grp < 3 * P3) ? grp % 3 : -1;
+    p = blockIdx.x * P3 + pl;
+  } else {
+    pl = grp;
+    var = 0;
+    p = blockIdx.x * G + grp;
+  }
+  const bool valid = var >= 0 && p < p_sel;
   const bool row = valid && lane < M;
   const double lam = meta[b].lambda;
-  const uint32_t it = valid ? sel[p] : 0;
-  const int rr = lane % BS, cc = lane / BS;
-  uint32_t addr[N];
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    uint32_t a = 0;
-    if (row) {
-      const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
-      const uint32_t r0 = min((uint32_t)(pk & 0xffff), lim), c0 = min((uint32_t)(pk >> 16), lim);
-      a = (uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc);
-    }
-    addr[k] = a;
+  const float* F = factors + (((size_t)b * nvar_stored + (valid ? var : 0)) * p_sel + (valid ? p : 0)) * fstride;
+  const float* FW = F + N * SR;
+  const float* FS = FW + N * M;
+  // shrink factor g(s_r)/s_r in lane r < N; s_0 is the largest
+  float dl = 0.f;
+  if (valid && lane < N) {
+    const float sl = FS[lane], smax = FS[0];
+    const float w = exp_w ? fabsf(smax * expf((float)(-0.5 * lam) * (sl * sl))) : (float)lam;
+    dl = sl > 0.f ? fmaxf(sl - w, 0.f) / sl : 0.f;
   }
-  float B0[N];
-  double Cd[N];
+  // kept ranks of this group: 1 + highest lane with g > 0
+  const uint64_t bal = __ballot(dl > 0.f);
+  const uint32_t gbits = (uint32_t)((bal >> ((threadIdx.x % 64) / L * L)) & ((L >= 64) ? ~0ull : ((1ull << L) - 1)));
+  const int nkeep = gbits ? 32 - __clz(gbits) : 0;
+  float bq[N];
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    B0[k] = 0.f;
-    Cd[k] = 0.0;
-  }
-#pragma unroll 1
-  for (int var = 0; var < 3; ++var) {
-    __syncthreads();  // sV / sS reuse across variants
-    if (valid && lane < N) {
-      const float* F = factors + (((size_t)b * 3 + var) * p_sel + p) * fstride;
+  for (int k = 0; k < N; ++k) bq[k] = 0.f;
+  for (int r = 0; r < nkeep; ++r) {
+    const float d = __shfl(dl, r, L);
+    const float y = row ? d * FW[r * M + lane] : 0.f;
+    const float4* vr = reinterpret_cast<const float4*>(F + r * SR);
 #pragma unroll
-      for (int k = 0; k < N; ++k) sV[grp * VS + lane * N + k] = F[lane * N + k];
-      sS[grp * N + lane] = F[N * N + lane];
-    }
-    __syncthreads();
-    if (row) {
-      const float* S = sS + grp * N;
-      float smax = 0.f;
-#pragma unroll
-      for (int l = 0; l < N; ++l) smax = fmaxf(smax, S[l]);
-      const float kk = (float)(-0.5 * lam), lf = (float)lam;
-      const float* U = u32 + ((size_t)b * 3 + var) * vol;
-      float x[N];
-#pragma unroll
-      for (int k = 0; k < N; ++k) x[k] = U[addr[k]];
-      float y[N];
-#pragma unroll
-      for (int l = 0; l < N; ++l) {
-        float t = 0.f;
-#pragma unroll
-        for (int j = 0; j < N; ++j) t = fmaf(x[j], sV[grp * VS + j * N + l], t);
-        const float sl = S[l];
-        const float w = exp_w ? fabsf(smax * expf(kk * (sl * sl))) : lf;
-        y[l] = sl > 0.f ? t * (fmaxf(sl - w, 0.f) / sl) : 0.f;
-      }
-      // Compiler-only barrier: re-read V from LDS for the second product
-      // instead of keeping all n*n values of the first one in registers.
-      asm volatile("" ::: "memory");
-#pragma unroll
-      for (int k = 0; k < N; ++k) {
-        float bq = 0.f;
-#pragma unroll
-        for (int l = 0; l < N; ++l) bq = fmaf(y[l], sV[grp * VS + k * N + l], bq);
-        if (var == 0)
-          B0[k] = bq;
-        else
-          Cd[k] += (double)bq;
-      }
+    for (int j = 0; j < SR / 4; ++j) {
+      const float4 t = vr[j];
+      if (4 * j < N) bq[4 * j] = fmaf(y, t.x, bq[4 * j]);
+      if (4 * j + 1 < N) bq[4 * j + 1] = fmaf(y, t.y, bq[4 * j + 1]);
+      if (4 * j + 2 < N) bq[4 * j + 2] = fmaf(y, t.z, bq[4 * j + 2]);
+      if (4 * j + 3 < N) bq[4 * j + 3] = fmaf(y, t.w, bq[4 * j + 3]);
     }
   }
-  double s5 = 0, sx = 0, s4 = 0;
-  if (row) {
-    unsigned long long* A = acc + (size_t)b * vol;
-    const double* IW = invw + (size_t)b * vol;
-    const double* UU = u64 + (size_t)b * vol;
+  if (nvar_run == 3 && row && var > 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) xch[pl][var - 1][lane][k] = bq[k];
+  }
+  if (nvar_run == 3) __syncthreads();
+  double s4 = 0.0;
+  if (row && var == 0) {
+    const int32_t* I = inv + (size_t)b * nw * p_sel + p;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-      const uint32_t v = addr[k];
-      atomicAdd(A + v, (unsigned long long)__float2ll_rn(B0[k] * kFixScale));
-      const double iw = IW[v];
-      const double b0 = (double)B0[k];
-      s5 = fma(b0, iw, s5);
-      sx = fma(b0 * UU[v], iw, sx);
-      const double c2 = Cd[k] - 2.0 * b0;
-      s4 = fma(c2 * (d2[v] ? d2neg : d2pos), iw, s4);
+      // row of patch p for frame k at its bucket-sorted place
+      B0[(((size_t)slot * nw + k) * p_sel + I[(size_t)k * p_sel]) * M + lane] = bq[k];
+    }
+    if (nvar_run == 3) {
+      const uint32_t it = sel[p];
+      const int rr = lane % BS, cc = lane / BS;
+      const double* DW = dw + (size_t)b * vol;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
+        const uint32_t r0 = min((uint32_t)(pk & 0xffff), lim), c0 = min((uint32_t)(pk >> 16), lim);
+        const uint32_t v = (uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc);
+        const double c2 = ((double)xch[pl][0][lane][k] + (double)xch[pl][1][lane][k]) - 2.0 * (double)bq[k];
+        s4 = fma(c2, DW[v], s4);
+      }
     }
   }
-  red[0][threadIdx.x] = s5;
-  red[1][threadIdx.x] = sx;
-  red[2][threadIdx.x] = s4;
+  if (nvar_run != 3) return;
+  for (int o = 32; o > 0; o >>= 1) s4 += __shfl_xor(s4, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = s4;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o)
-      for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x < 3) partial[((size_t)slot * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x == 0) partial[(size_t)slot * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// sum Uhat^2, sum U, sum U^2 from the variant-0 volume (cleared as read)
-__global__ __launch_bounds__(256) void k_risk_volume(unsigned long long* __restrict__ acc,
-                                                     const double* __restrict__ invw,
-                                                     const double* __restrict__ u64, const uint32_t* __restrict__ wl,
-                                                     uint32_t vol, double* __restrict__ partial) {
-  __shared__ double sm[3][256];
+// ----------------------------------------------------------- gather --
+// Uhat_v = (sum of the contributions covering v) / W_v, contributions taken
+// position by position (patch column cb, then row ra) and bucket by bucket
+// in ascending patch order, summed in FP64.
+//   mode 0: risk sums per block: sum Uhat^2, sum Uhat, sum Uhat U, sum U, sum U^2
+//   mode 1: final frame: Y(:,:,t) = uMax * Uhat(:,:,ref)
+//   mode 2: whole window volume of slot 0 into out
+// One thread per voxel.  For each patch column cb the buckets of positions
+// (r-bs+1 .. r, c-cb) are one contiguous run of rows in bucket order, read
+// straight from memory (neighbouring voxels share them in cache).
+template <int BS>
+__global__ __launch_bounds__(256) void k_gather_direct(const float* __restrict__ B0, const int32_t* __restrict__ off,
+                                                       const double* __restrict__ invw,
+                                                       const double* __restrict__ u64,
+                                                       const WinMeta* __restrict__ meta,
+                                                       const uint32_t* __restrict__ wl, uint32_t nr, uint32_t fs,
+                                                       uint32_t nw, uint32_t vol, uint32_t n1, uint32_t p_sel,
+                                                       int mode, double* __restrict__ out,
+                                                       double* __restrict__ partial) {
+  constexpr int M = BS * BS;
+  __shared__ double sm[5][4];
   const uint32_t slot = blockIdx.y, b = wl[slot];
-  double q = 0, s2 = 0, su2 = 0;
-  unsigned long long* A = acc + (size_t)b * vol;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < vol; v += gridDim.x * blockDim.x) {
-    const long long a = (long long)A[v];
-    A[v] = 0ull;
-    const double uh = ((double)a * kFixInv) * invw[(size_t)b * vol + v];
-    const double U = u64[(size_t)b * vol + v];
-    q = fma(uh, uh, q);
-    s2 += U;
-    su2 = fma(U, U, su2);
+  const WinMeta m = meta[b];
+  const uint32_t span = (mode == 1) ? fs : vol;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double a_q = 0, a_h = 0, a_hu = 0, a_u = 0, a_u2 = 0;
+  if (i < span) {
+    const uint32_t v = (mode == 1) ? m.ref * fs + i : i;
+    const uint32_t k = v / fs, pix = v % fs;
+    const int r = (int)(pix % nr), c = (int)(pix / nr);
+    const int32_t* O = off + ((size_t)b * nw + k) * ((size_t)n1 * n1 + 1);
+    const float* Bk = B0 + ((size_t)slot * nw + k) * p_sel * M;
+    double acc = 0.0;
+#pragma unroll
+    for (int cb = 0; cb < BS; ++cb) {
+      const int pc = c - cb;
+      if (pc < 0 || pc >= (int)n1) continue;
+      const size_t colq = (size_t)n1 * pc;
+      // bucket bounds of rows r-ra, ra = 0..BS-1 (clamped to the column)
+      int32_t bnd[BS + 1];
+#pragma unroll
+      for (int e = 0; e <= BS; ++e) {
+        const int pr = r - (BS - 1) + e;  // e = 0 -> row r-BS+1, e = BS -> end of row r
+        const int prc = min(max(pr, 0), (int)n1);
+        bnd[e] = (prc >= (int)n1) ? O[colq + n1] : O[colq + prc];
+      }
+#pragma unroll
+      for (int ra = 0; ra < BS; ++ra) {
+        const int e = BS - 1 - ra;  // position r - ra
+        for (int32_t j = bnd[e]; j < bnd[e + 1]; ++j) acc += (double)Bk[(size_t)j * M + ra + BS * cb];
+      }
+    }
+    const double uh = acc * invw[(size_t)b * vol + v];
+    if (mode == 0) {
+      const double U = u64[(size_t)b * vol + v];
+      a_q = uh * uh;
+      a_h = uh;
+      a_hu = uh * U;
+      a_u = U;
+      a_u2 = U * U;
+    } else if (mode == 1) {
+      out[(size_t)m.t * fs + i] = uh * m.uMax;
+    } else {
+      out[v] = uh;
+    }
   }
-  sm[0][threadIdx.x] = q;
-  sm[1][threadIdx.x] = s2;
-  sm[2][threadIdx.x] = su2;
+  if (mode != 0) return;
+  // wave reduction (lanes exchange through DPP/permute), then 4 waves
+  for (int o = 32; o > 0; o >>= 1) {
+    a_q += __shfl_xor(a_q, o);
+    a_h += __shfl_xor(a_h, o);
+    a_hu += __shfl_xor(a_hu, o);
+    a_u += __shfl_xor(a_u, o);
+    a_u2 += __shfl_xor(a_u2, o);
+  }
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) {
+    sm[0][w] = a_q;
+    sm[1][w] = a_h;
+    sm[2][w] = a_hu;
+    sm[3][w] = a_u;
+    sm[4][w] = a_u2;
+  }
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o)
-      for (int c = 0; c < 3; ++c) sm[c][threadIdx.x] += sm[c][threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x < 3) partial[((size_t)slot * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = sm[threadIdx.x][0];
+  if (threadIdx.x < 5)
+    partial[((size_t)slot * gridDim.x + blockIdx.x) * 5 + threadIdx.x] =
+        ((sm[threadIdx.x][0] + sm[threadIdx.x][1]) + sm[threadIdx.x][2]) + sm[threadIdx.x][3];
 }
 
-// per slot: (sum Uhat^2, sum U, sum U^2, sum Uhat U, sum d2 (..), sum Uhat)
-__global__ __launch_bounds__(256) void k_risk_reduce2(const double* __restrict__ pa, uint32_t na,
-                                                      const double* __restrict__ pb, uint32_t nb_,
-                                                      double* __restrict__ sums) {
+// per slot: (sum Uhat^2, sum U, sum U^2, sum Uhat U, sum d2 (U2p - 2 Uhat + U2m), sum Uhat)
+__global__ __launch_bounds__(256) void k_risk_reduce(const double* __restrict__ pa, uint32_t na,
+                                                     const double* __restrict__ pb, uint32_t nb_,
+                                                     double* __restrict__ sums) {
   __shared__ double sm[6][256];
   const uint32_t slot = blockIdx.x;
   double s[6] = {0, 0, 0, 0, 0, 0};
-  for (uint32_t k = threadIdx.x; k < nb_; k += blockDim.x)
-    for (int c = 0; c < 3; ++c) s[c] += pb[((size_t)slot * nb_ + k) * 3 + c];
-  for (uint32_t k = threadIdx.x; k < na; k += blockDim.x) {
-    s[5] += pa[((size_t)slot * na + k) * 3 + 0];  // sum Uhat
-    s[3] += pa[((size_t)slot * na + k) * 3 + 1];  // sum Uhat U
-    s[4] += pa[((size_t)slot * na + k) * 3 + 2];  // sum d2 (U2p - 2 Uhat + U2m)
+  for (uint32_t k = threadIdx.x; k < nb_; k += blockDim.x) {
+    const double* P = pb + ((size_t)slot * nb_ + k) * 5;
+    s[0] += P[0];  // sum Uhat^2
+    s[5] += P[1];  // sum Uhat
+    s[3] += P[2];  // sum Uhat U
+    s[1] += P[3];  // sum U
+    s[2] += P[4];  // sum U^2
   }
+  for (uint32_t k = threadIdx.x; k < na; k += blockDim.x) s[4] += pa[(size_t)slot * na + k];
   for (int c = 0; c < 6; ++c) sm[c][threadIdx.x] = s[c];
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -811,18 +880,8 @@ __global__ __launch_bounds__(256) void k_risk_reduce2(const double* __restrict__
   if (threadIdx.x < 6) sums[(size_t)slot * 6 + threadIdx.x] = sm[threadIdx.x][0];
 }
 
-__global__ void k_invw(const int32_t* __restrict__ W, size_t n, double* __restrict__ invw) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    invw[i] = W[i] > 0 ? 1.0 / (double)W[i] : 0.0;
-}
-
-hipError_t launch_invw(const int32_t* W, size_t n, double* invw, hipStream_t s) {
-  const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_invw, dim3(grid), dim3(256), 0, s, W, n, invw);
-  return hipGetLastError();
-}
-
-// shape table of the instantiated SVD / reconstruction kernels
+// shape table of the instantiated SVD / evaluation kernels (patch size,
+// window frames, lanes per matrix)
 #define PG_SHAPES(X) \
   X(4, 3, 16) X(4, 5, 16) X(4, 7, 16) X(4, 9, 16) X(4, 11, 16) X(4, 13, 16) X(4, 15, 16) \
   X(2, 3, 16) X(3, 3, 16) X(3, 5, 16) X(3, 7, 16) X(3, 9, 16) \
@@ -853,93 +912,51 @@ hipError_t launch_svd(const float* u32, const int32_t* pos, const uint32_t* sel,
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_recon(const float* factors, const float* u32, const int32_t* pos, const uint32_t* sel,
-                        const WinMeta* meta, const uint32_t* wl, uint32_t nb, uint32_t nvar_stored,
-                        uint32_t nvar_run, const Shape& sh, int exp_w, int only_ref, unsigned long long* acc,
-                        hipStream_t s) {
+uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run) {
+  const uint32_t G = (sh.m <= 16) ? 16 : (sh.m <= 32 ? 8 : 4);
+  const uint32_t per = (nvar_run == 3) ? G / 3 : G;
+  return (sh.p_sel + per - 1) / per;
+}
+
+hipError_t launch_eval(const float* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
+                       const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh,
+                       int exp_w, const double* dw, const int32_t* inv, float* B0, double* partial, hipStream_t s) {
   const uint32_t lim = sh.nr - sh.bs;
-#define PG_REC(B, N, L)                                                                                          \
+  const uint32_t nblk = eval_blocks(sh, nvar_run);
+#define PG_EV(B, N, L)                                                                                           \
   if (sh.bs == B && sh.nw == N) {                                                                              \
-    const uint32_t groups = nvar_run * sh.p_sel;                                                               \
-    const dim3 grid((groups + (256 / L) - 1) / (256 / L), nb);                                                 \
-    hipLaunchKernelGGL((k_recon<B, N, L>), grid, dim3(256), 0, s, factors, u32, pos, sel, meta, nvar_stored,  \
-                       nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, exp_w,      \
-                       only_ref, wl, acc);                                                                     \
+    hipLaunchKernelGGL((k_eval<B, N, L>), dim3(nblk, na), dim3(256), 0, s, factors, pos, sel, meta, wl,       \
+                       nvar_stored, nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, \
+                       exp_w, dw, inv, B0, partial);                                                           \
     return hipGetLastError();                                                                                  \
   }
-  PG_SHAPES(PG_REC)
-#undef PG_REC
+  PG_SHAPES(PG_EV)
+#undef PG_EV
   return hipErrorInvalidValue;
 }
 
-uint32_t recon_risk_blocks(const Shape& sh) {
-  const uint32_t G = (sh.m <= 16) ? 16 : (sh.m <= 32 ? 8 : 4);
-  return (sh.p_sel + G - 1) / G;
-}
+uint32_t gather_blocks(const Shape& sh, int mode) { return ((mode == 1 ? sh.fs : sh.vol) + 255) / 256; }
 
-hipError_t launch_recon_risk(const float* factors, const float* u32, const int32_t* pos, const uint32_t* sel,
-                             const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int exp_w,
-                             const double* invw, const double* u64, const uint8_t* d2, double d2neg, double d2pos,
-                             unsigned long long* acc, double* partial_a, double* partial_b, uint32_t nblk_b,
-                             double* sums, hipStream_t s) {
-  const uint32_t lim = sh.nr - sh.bs;
-  const uint32_t nblk_a = recon_risk_blocks(sh);
-  bool done = false;
-#define PG_RR(B, N, L)                                                                                           \
-  if (!done && sh.bs == B && sh.nw == N) {                                                                     \
-    hipLaunchKernelGGL((k_recon_risk<B, N, L>), dim3(nblk_a, na), dim3(256), 0, s, factors, u32, pos, sel,     \
-                       meta, wl, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, exp_w, invw,  \
-                       u64, d2, d2neg, d2pos, acc, partial_a);                                                 \
-    done = true;                                                                                               \
+hipError_t launch_gather(const float* B0, const int32_t* off, const double* invw, const double* u64,
+                         const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int mode, double* out,
+                         double* partial, hipStream_t s) {
+  const uint32_t nblk = gather_blocks(sh, mode);
+#define PG_GA(B)                                                                                                 \
+  case B:                                                                                                      \
+    hipLaunchKernelGGL(k_gather_direct<B>, dim3(nblk, na), dim3(256), 0, s, B0, off, invw, u64, meta, wl,       \
+                       sh.nr, sh.fs, sh.nw, sh.vol, sh.n1, sh.p_sel, mode, out, partial);                      \
+    break;
+  switch (sh.bs) {
+    PG_GA(2) PG_GA(3) PG_GA(4) PG_GA(5) PG_GA(6) PG_GA(7) PG_GA(8)
+    default: return hipErrorInvalidValue;
   }
-  PG_SHAPES(PG_RR)
-#undef PG_RR
-  if (!done) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_risk_volume, dim3(nblk_b, na), dim3(256), 0, s, acc, invw, u64, wl, sh.vol, partial_b);
-  hipLaunchKernelGGL(k_risk_reduce2, dim3(na), dim3(256), 0, s, partial_a, nblk_a, partial_b, nblk_b, sums);
+#undef PG_GA
   return hipGetLastError();
 }
 
-// -------------------------------------------------------- PGURE risk --
-__device__ __forceinline__ double fix_mean(unsigned long long& a, int32_t w) {
-  const long long v = (long long)a;
-  a = 0ull;
-  return w > 0 ? ((double)v * kFixInv) / (double)w : 0.0;
-}
-
-// final frame: Y(:,:,t) = uMax * Recon(lambda)(:,:,ref)  (src/pguresvt.hpp:146-166)
-__global__ void k_output(unsigned long long* __restrict__ acc, const int32_t* __restrict__ W,
-                         const WinMeta* __restrict__ meta, uint32_t nvar_acc, uint32_t fs, uint32_t vol,
-                         double* __restrict__ Y) {
-  const uint32_t b = blockIdx.y;
-  const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pix >= fs) return;
-  const WinMeta m = meta[b];
-  const size_t v = (size_t)m.ref * fs + pix;
-  (void)nvar_acc;
-  const double val = fix_mean(acc[(size_t)b * vol + v], W[(size_t)b * vol + v]);
-  Y[(size_t)m.t * fs + pix] = val * m.uMax;
-}
-
-hipError_t launch_output(unsigned long long* acc, const int32_t* W, const WinMeta* meta, uint32_t nb,
-                         uint32_t nvar_acc, const Shape& sh, double* Y, hipStream_t s) {
-  hipLaunchKernelGGL(k_output, dim3((sh.fs + 255) / 256, nb), dim3(256), 0, s, acc, W, meta, nvar_acc, sh.fs,
-                     sh.vol, Y);
-  return hipGetLastError();
-}
-
-// whole reconstructed window (variant 0 of batch slot 0), normalised units
-__global__ void k_window_volume(unsigned long long* __restrict__ acc, const int32_t* __restrict__ W, uint32_t vol,
-                                double* __restrict__ out) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= vol) return;
-  out[v] = fix_mean(acc[v], W[v]);
-}
-
-hipError_t launch_window_volume(unsigned long long* acc, const int32_t* W, uint32_t nvar_acc, const Shape& sh,
-                                double* out, hipStream_t s) {
-  (void)nvar_acc;
-  hipLaunchKernelGGL(k_window_volume, dim3((sh.vol + 255) / 256), dim3(256), 0, s, acc, W, sh.vol, out);
+hipError_t launch_risk_reduce(const double* pa, uint32_t na_blk, const double* pb, uint32_t nb_blk, uint32_t na,
+                              double* sums, hipStream_t s) {
+  hipLaunchKernelGGL(k_risk_reduce, dim3(na), dim3(256), 0, s, pa, na_blk, pb, nb_blk, sums);
   return hipGetLastError();
 }
 

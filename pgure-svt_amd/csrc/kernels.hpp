@@ -20,7 +20,7 @@ struct Shape {
   uint32_t n1;          // nr - bs + 1 (grid positions per axis)
   uint32_t p_all;       // n1 * n1
   uint32_t p_sel;       // selected patches (src/svt.hpp:61-92)
-  uint32_t fstride;     // floats per stored factor block (V n x n, s n)
+  uint32_t fstride;     // floats per stored factor block (V^T n x sr, W^T n x m, s n)
 };
 
 bool svd_shape_supported(uint32_t bs, uint32_t nw);
@@ -35,29 +35,33 @@ hipError_t launch_prepare(const double* X, const double* Z, const WinMeta* meta,
 hipError_t launch_fill_ref(const WinMeta* meta, uint32_t nb, const Shape& sh, int32_t* pos, hipStream_t s);
 hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs, uint32_t nb, const Shape& sh,
                        int mw, int32_t* pos, hipStream_t s);
-hipError_t launch_count(const int32_t* pos, const uint32_t* sel, uint32_t nb, const Shape& sh, int32_t* W,
-                        hipStream_t s);
 hipError_t launch_svd(const float* u32, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
                       const Shape& sh, float* factors, int max_sweeps, float tol, uint32_t* stats, hipStream_t s);
-// wl: device list of batch slots to process (nb entries)
-hipError_t launch_recon(const float* factors, const float* u32, const int32_t* pos, const uint32_t* sel,
-                        const WinMeta* meta, const uint32_t* wl, uint32_t nb, uint32_t nvar_stored,
-                        uint32_t nvar_run, const Shape& sh, int exp_w, int only_ref, unsigned long long* acc,
-                        hipStream_t s);
-// fused lambda evaluation: threshold + reconstruct all three variants, the
-// variant-0 volume into acc ([nb][vol]) and the linear risk sums; then the
-// volume sums.  sums: double [na][6] = (sum Uhat^2, sum U, sum U^2,
-// sum Uhat U, sum d2 (U2p - 2 Uhat + U2m), sum Uhat) per listed slot.
-uint32_t recon_risk_blocks(const Shape& sh);
-hipError_t launch_recon_risk(const float* factors, const float* u32, const int32_t* pos, const uint32_t* sel,
-                             const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int exp_w,
-                             const double* invw, const double* u64, const uint8_t* d2, double d2neg, double d2pos,
-                             unsigned long long* acc, double* partial_a, double* partial_b, uint32_t nblk_b,
-                             double* sums, hipStream_t s);
-hipError_t launch_invw(const int32_t* W, size_t n, double* invw, hipStream_t s);
-hipError_t launch_output(unsigned long long* acc, const int32_t* W, const WinMeta* meta, uint32_t nb,
-                         uint32_t nvar_acc, const Shape& sh, double* Y, hipStream_t s);
-hipError_t launch_window_volume(unsigned long long* acc, const int32_t* W, uint32_t nvar_acc, const Shape& sh,
-                                double* out, hipStream_t s);
+// position buckets of every (window, frame): cnt and off are [nb][nw][n1*n1
+// (+1)], list [nb][nw][p_sel]
+// inv [nb][nw][p_sel]: bucket-sorted place of every patch
+hipError_t launch_buckets(const int32_t* pos, const uint32_t* sel, uint32_t nb, const Shape& sh, int32_t* cnt,
+                          int32_t* off, int32_t* list, int32_t* inv, hipStream_t s);
+// 1/W and delta2/W per voxel, [nb][vol] (dw may be null)
+hipError_t launch_weights(const int32_t* off, const uint8_t* d2, uint32_t nb, const Shape& sh, double d2neg,
+                          double d2pos, double* invw, double* dw, hipStream_t s);
+// one lambda evaluation of the listed slots; B0 [na][nw][p_sel][m] (rows in
+// bucket order); partial
+// [na][eval_blocks] (divergence sums, only with nvar_run == 3)
+uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run);
+hipError_t launch_eval(const float* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
+                       const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh,
+                       int exp_w, const double* dw, const int32_t* inv, float* B0, double* partial, hipStream_t s);
+// overlap-add gather: mode 0 risk sums into partial [na][gather_blocks][5]; mode 1
+// final reference frames into out (the output cube); mode 2 the whole
+// window of slot 0 into out
+uint32_t gather_blocks(const Shape& sh, int mode);
+hipError_t launch_gather(const float* B0, const int32_t* off, const double* invw, const double* u64,
+                         const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int mode, double* out,
+                         double* partial, hipStream_t s);
+// sums [na][6] = (sum Uhat^2, sum U, sum U^2, sum Uhat U,
+// sum d2 (U2p - 2 Uhat + U2m), sum Uhat)
+hipError_t launch_risk_reduce(const double* pa, uint32_t na_blk, const double* pb, uint32_t nb_blk, uint32_t na,
+                              double* sums, hipStream_t s);
 
 }  // namespace pgsvt
