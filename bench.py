@@ -40,16 +40,19 @@ N, FRAMES, PATCH, TRAJ = 256, 100, 4, 15
 M, NW = PATCH * PATCH, TRAJ
 NVOX = N * N * NW
 
-# algorithmic work per unit of each timed kernel (units as the library counts them)
-#   svd_jacobi         matrix      Golub-Van Loan thin SVD: 14 m n^2 + 8 n^3 flops
-#   recon_risk         patch x variant   threshold + U diag(s') V^T: 2 m n^2 + m n flops
-#   pgure_risk         voxel       3 reconstructions (fp32) + U (fp64) + weight (int32) + delta2 sign: 25 bytes
-#   arps               block search      ref + candidate block reads are cache-resident; not a roofline kernel
+P_SEL = (N - PATCH + 1) ** 2  # patches per window (step 1)
+
+# Algorithmic work per unit of each timed stage (units as the library counts
+# them in its profile):
+#   svd_jacobi        matrix            thin SVD of one m x n Casorati matrix with V and U S:
+#                                       14 m n^2 + 8 n^3 flops (Golub-Van Loan count)
+#   eval_reconstruct  patch x variant   U diag(g(s)) V^T of one patch: 2 m n^2 flops
+#   gather_risk       voxel             every patch row of the U reconstruction read once (fp32),
+#                                       U (fp64) and 1/W (fp64) of the voxel read once
 ALGO = {
     "svd_jacobi": ("mfma", 14 * M * NW**2 + 8 * NW**3),
-    "recon_risk": ("mfma", 2 * M * NW**2 + M * NW),
-    "recon_final_frame": ("mfma", 2 * M * NW + M),
-    "pgure_risk": ("hbm", 25),
+    "eval_reconstruct": ("mfma", 2 * M * NW**2),
+    "gather_risk": ("hbm", 4.0 * P_SEL * M * NW / NVOX + 16),
 }
 
 
@@ -139,6 +142,7 @@ def run_gpu(noisy, steps, warmup):
 
 
 def roofline(prof):
+    """Roofline of the stage with the largest GPU time in the timed steps."""
     known = {k: v for k, v in prof.items() if k in ALGO and v["launches"] > 0}
     if not known:
         return None, None
@@ -149,11 +153,11 @@ def roofline(prof):
     units_per_launch = e["units"] / e["launches"]
     work = per_unit * units_per_launch
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % name)
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            t = json.load(f)
-        if "hbm_bytes_per_unit" in t:
+            t = json.load(f).get("stages", {}).get(name)
+        if t and "hbm_bytes_per_unit" in t:
             traffic = t["hbm_bytes_per_unit"] * units_per_launch
     if bound == "hbm":
         achieved = work / avg_s / 1e9
@@ -165,12 +169,87 @@ def roofline(prof):
              "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic}
     r["kernel"] = name
     r["avg_launch_ms"] = e["ms"] / e["launches"]
+    r["units_per_launch"] = units_per_launch
     return r, name
+
+
+# kernels of each profiled stage, as rocprofv3 names them
+STAGE_KERNELS = {
+    "svd_jacobi": "pgsvt::k_svd<",
+    "eval_reconstruct": "pgsvt::k_eval<",
+    "gather_risk": "pgsvt::k_gather_direct<",
+}
+
+
+def copy_probe(mib):
+    """One elementwise copy (read + write of `mib` MiB) for calibrating the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": 4 * n}))
+
+
+def _pmc_rows(d, counter):
+    import csv
+    import glob
+
+    rows = []
+    for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    rows.append((r["Kernel_Name"], int(r["Grid_Size"]), float(r["Counter_Value"])))
+    if not rows:
+        raise SystemExit("no %s rows under %s" % (counter, d))
+    return rows
+
+
+def summarize_pmc(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_bytes, units_json, out):
+    """HBM traffic per unit of each stage from FETCH_SIZE / WRITE_SIZE runs
+    (KiB per dispatch), each scaled by what the counter reads for a copy of
+    `copy_bytes` bytes read and written.  units_json: the bench line of the
+    profiled run (for the units each stage processed)."""
+    def copy_kib(d, c):
+        rows = [r for r in _pmc_rows(d, c) if "elementwise" in r[0]]
+        return max(r[2] for r in rows)
+    f_fetch = copy_bytes / (1024.0 * copy_kib(copy_fetch_dir, "FETCH_SIZE"))
+    f_write = copy_bytes / (1024.0 * copy_kib(copy_write_dir, "WRITE_SIZE"))
+    with open(units_json) as f:
+        line = json.loads(f.read().strip().splitlines()[-1])
+    units = line["stage_units_per_step"]
+    steps = line["steps"] + line["warmup"]
+    fetch = _pmc_rows(fetch_dir, "FETCH_SIZE")
+    write = _pmc_rows(write_dir, "WRITE_SIZE")
+    res = {"copy_probe_bytes": copy_bytes, "fetch_scale": f_fetch, "write_scale": f_write, "stages": {}}
+    for st, kname in STAGE_KERNELS.items():
+        fk = sum(r[2] for r in fetch if kname in r[0])
+        wk = sum(r[2] for r in write if kname in r[0])
+        nd = sum(1 for r in fetch if kname in r[0])
+        if nd == 0 or st not in units:
+            continue
+        u = units[st] * steps
+        hbm = 1024.0 * (fk * f_fetch + wk * f_write)
+        res["stages"][st] = {"dispatches": nd, "fetch_kib": fk, "write_kib": wk, "units": u,
+                             "hbm_bytes_per_unit": hbm / u,
+                             "algorithmic_bytes_per_unit": ALGO[st][1] if ALGO[st][0] == "hbm" else None}
+    with open(out, "w") as f:
+        json.dump(res, f, indent=1)
+        f.write("\n")
+    print(json.dumps(res))
 
 
 def cpu_baseline(noisy, windows):
     """The oracle on `windows` output frames in the middle of the sequence,
-    one window per thread, SVDs through LAPACK dgesdd."""
+    one window per thread, SVDs through LAPACK dgesdd (single-threaded, so
+    `cores` counts every thread used)."""
+    for v in ("MKL_NUM_THREADS", "OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS"):
+        os.environ[v] = "1"
     import oracle_lib as O
 
     try:
@@ -199,9 +278,21 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-windows", type=int, default=None,
-                    help="output frames in the CPU-baseline sample (default: host cores, at most 16)")
+                    help="output frames in the CPU-baseline sample, one thread each (default 2: about 30 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, metavar="MIB", default=None,
+                    help="only run one MIB-MiB elementwise copy (counter calibration) and exit")
+    ap.add_argument("--summarize-pmc", nargs=6, default=None,
+                    metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR", "BENCH_JSON", "OUT"),
+                    help="write per-stage HBM traffic from rocprofv3 --pmc csv output and exit")
     a = ap.parse_args()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
+    if a.summarize_pmc:
+        fd, wd, cfd, cwd, bj, out = a.summarize_pmc
+        summarize_pmc(fd, wd, cfd, cwd, (a.copy_probe or 1024) * (1 << 20), bj, out)
+        return
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the path shards by output frame across GPUs with no exchange, "
               "but this build measures a single MI355X (got --gpus %d)" % a.gpus, file=sys.stderr)
@@ -212,7 +303,7 @@ def main():
     roof, _ = roofline(prof)
     cpu = None
     if not a.no_cpu_baseline:
-        cw = a.cpu_windows or max(1, min(16, os.cpu_count() or 1))
+        cw = a.cpu_windows or max(1, min(2, os.cpu_count() or 1))
         cpu, _ = cpu_baseline(noisy, cw)
     out = {
         "metric": METRIC,
@@ -234,6 +325,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v["ms"] / a.steps for k, v in prof.items()},
+        "stage_units_per_step": {k: v["units"] / a.steps for k, v in prof.items()},
         "lambda_median": float(np.median(est[:, 0])),
     }
     print(json.dumps(out))

@@ -81,6 +81,28 @@ const char* pgsvt_last_error(void);
 /* Library version string. */
 const char* pgsvt_version(void);
 
+/* pgsvt_denoise with X and Y in device memory of the current HIP device
+ * (same shapes, layout and element types); `estimates` stays a host
+ * n_frames x 4 column-major float64 matrix.  The reference has no device
+ * variant: it is the PGURESVT<T1,double> call (src/pguresvt.hpp:17-172) for
+ * callers whose frames are already resident in HBM, so no host<->device copy
+ * of the cube happens.  Returns after Y is complete. */
+int pgsvt_denoise_device(const void* X_dev, int dtype, uint32_t n_rows,
+                         uint32_t n_cols, uint32_t n_frames,
+                         const pgsvt_params* p, double* Y_dev,
+                         double* estimates);
+
+/* Per-stage timing of later pgsvt_denoise* calls (no reference
+ * counterpart; the reference prints wall times only, src/pguresvt.hpp:168).
+ * pgsvt_profile_enable(1) clears and starts collection, (0) stops it.
+ * GPU stages are timed with HIP events on the library's stream.
+ * pgsvt_profile_read writes a JSON object
+ *   {"<stage>": {"launches": n, "ms": total, "units": work units}, ...}
+ * into buf and returns 0, or returns the buffer size needed (bytes,
+ * including the terminating NUL) when buf is NULL or len is too small. */
+void pgsvt_profile_enable(int on);
+int pgsvt_profile_read(char* buf, uint64_t len);
+
 /* ------------------------------------------------------------------------
  * Window-level entry points.  They expose the stages of one time window of
  * the driver (src/pguresvt.hpp:90-166) so a caller can check each stage on

@@ -661,6 +661,21 @@ hipError_t launch_weights(const int32_t* off, const uint8_t* d2, uint32_t nb, co
   return hipGetLastError();
 }
 
+// Workgroups of a launch are handed to the 8 XCDs round-robin by linear
+// id, and each XCD has its own L2.  xcd_block maps the linear id so that XCD
+// x runs one contiguous range of logical (x, y) blocks: neighbouring blocks,
+// which read overlapping patches, then meet in the same L2.  A bijection on
+// the 2-D grid; returns the logical (x, y).
+__device__ __forceinline__ void xcd_block(uint32_t& bx, uint32_t& by) {
+  constexpr uint32_t kXcd = 8;
+  const uint32_t total = gridDim.x * gridDim.y;
+  const uint32_t lin = blockIdx.y * gridDim.x + blockIdx.x;
+  const uint32_t x = lin % kXcd, idx = lin / kXcd, q = total / kXcd, r = total % kXcd;
+  const uint32_t logical = x * q + min(x, r) + idx;
+  bx = logical % gridDim.x;
+  by = logical / gridDim.x;
+}
+
 // --------------------------------------------- threshold + reconstruct --
 // One lambda evaluation of the thresholded reconstruction of every patch
 // (src/svt.hpp:121-146 with src/utils.hpp:96-106):
@@ -670,10 +685,11 @@ hipError_t launch_weights(const int32_t* off, const uint8_t* d2, uint32_t nb, co
 // only their rows are read.  A group of L lanes owns one (patch, variant),
 // lane q one row of B.
 // nvar_run == 3 (PGURE): the groups of a block take (patch, variant) pairs
-// for the variants U, U + eps2 delta2, U - eps2 delta2; the U group stores
-// its B in `B0` ([slot][frame][patch in bucket order][row]) for the gather
-// and adds sum_c (B1_c - 2 B0_c + B2_c) delta2_v / W_v, the divergence sum
-// of src/pgure.hpp:136 (linear in the overlap-add), reduced in FP64.
+// for the variants U, U + eps2 delta2, U - eps2 delta2 and exchange their
+// reconstructions through LDS; then, frame by frame, the U row goes to `B0`
+// ([slot][frame][patch in bucket order][row]) for the gather and
+// sum_c (B1_c - 2 B0_c + B2_c) delta2_v / W_v, the divergence sum of
+// src/pgure.hpp:136 (linear in the overlap-add), is accumulated in FP64.
 // nvar_run == 1: every group reconstructs U of its own patch.
 template <int BS, int N, int L>
 __global__ __launch_bounds__(256) void k_eval(
@@ -683,20 +699,22 @@ __global__ __launch_bounds__(256) void k_eval(
     uint32_t fstride, int exp_w, const double* __restrict__ dw, const int32_t* __restrict__ inv,
     float* __restrict__ B0, double* __restrict__ partial) {
   constexpr int M = BS * BS, G = 256 / L, SR = (N + 3) & ~3, P3 = G / 3;
-  __shared__ float xch[P3][2][M][N];  // reconstructions of the two perturbed variants
+  __shared__ float xch[P3][3][M][N];  // reconstructions of the three variants
   __shared__ double red[4];
-  const uint32_t slot = blockIdx.y, b = wl[slot];
+  uint32_t bx, slot;
+  xcd_block(bx, slot);
+  const uint32_t b = wl[slot];
   const int grp = threadIdx.x / L, lane = threadIdx.x % L;
   int pl, var;
   uint32_t p;
   if (nvar_run == 3) {
     pl = grp / 3;
     var = (grp < 3 * P3) ? grp % 3 : -1;
-    p = blockIdx.x * P3 + pl;
+    p = bx * P3 + pl;
   } else {
     pl = grp;
     var = 0;
-    p = blockIdx.x * G + grp;
+    p = bx * G + grp;
   }
   const bool valid = var >= 0 && p < p_sel;
   const bool row = valid && lane < M;
@@ -731,38 +749,49 @@ __global__ __launch_bounds__(256) void k_eval(
       if (4 * j + 3 < N) bq[4 * j + 3] = fmaf(y, t.w, bq[4 * j + 3]);
     }
   }
-  if (nvar_run == 3 && row && var > 0) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) xch[pl][var - 1][lane][k] = bq[k];
-  }
-  if (nvar_run == 3) __syncthreads();
   double s4 = 0.0;
-  if (row && var == 0) {
+  if (nvar_run == 3) {
+    // The three groups of a patch share the per-frame work: group `var`
+    // takes frames var, var + 3, ...: the U row goes to its bucket-sorted
+    // place in B0 and (B1 - 2 B0 + B2) delta2_v / W_v is accumulated.
+    if (row) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) xch[pl][var][lane][k] = bq[k];
+    }
+    __syncthreads();
+    if (row) {
+      const int32_t* I = inv + (size_t)b * nw * p_sel + p;
+      const uint32_t it = sel[p];
+      const int rr = lane % BS, cc = lane / BS;
+      const double* DW = dw + (size_t)b * vol;
+#pragma unroll
+      for (int i = 0; i < (N + 2) / 3; ++i) {
+        const int k = var + 3 * i;
+        if (k < N) {
+          const float x0 = xch[pl][0][lane][k];
+          B0[(((size_t)slot * nw + k) * p_sel + I[(size_t)k * p_sel]) * M + lane] = x0;
+          const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
+          const uint32_t r0 = min((uint32_t)(pk & 0xffff), lim), c0 = min((uint32_t)(pk >> 16), lim);
+          const uint32_t v = (uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc);
+          const double c2 =
+              ((double)xch[pl][1][lane][k] + (double)xch[pl][2][lane][k]) - 2.0 * (double)x0;
+          s4 = fma(c2, DW[v], s4);
+        }
+      }
+    }
+  } else if (row) {
     const int32_t* I = inv + (size_t)b * nw * p_sel + p;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
       // row of patch p for frame k at its bucket-sorted place
       B0[(((size_t)slot * nw + k) * p_sel + I[(size_t)k * p_sel]) * M + lane] = bq[k];
     }
-    if (nvar_run == 3) {
-      const uint32_t it = sel[p];
-      const int rr = lane % BS, cc = lane / BS;
-      const double* DW = dw + (size_t)b * vol;
-#pragma unroll
-      for (int k = 0; k < N; ++k) {
-        const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
-        const uint32_t r0 = min((uint32_t)(pk & 0xffff), lim), c0 = min((uint32_t)(pk >> 16), lim);
-        const uint32_t v = (uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc);
-        const double c2 = ((double)xch[pl][0][lane][k] + (double)xch[pl][1][lane][k]) - 2.0 * (double)bq[k];
-        s4 = fma(c2, DW[v], s4);
-      }
-    }
   }
   if (nvar_run != 3) return;
   for (int o = 32; o > 0; o >>= 1) s4 += __shfl_xor(s4, o);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = s4;
   __syncthreads();
-  if (threadIdx.x == 0) partial[(size_t)slot * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+  if (threadIdx.x == 0) partial[(size_t)slot * gridDim.x + bx] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 // ----------------------------------------------------------- gather --
@@ -786,10 +815,12 @@ __global__ __launch_bounds__(256) void k_gather_direct(const float* __restrict__
                                                        double* __restrict__ partial) {
   constexpr int M = BS * BS;
   __shared__ double sm[5][4];
-  const uint32_t slot = blockIdx.y, b = wl[slot];
+  uint32_t bx, slot;
+  xcd_block(bx, slot);
+  const uint32_t b = wl[slot];
   const WinMeta m = meta[b];
   const uint32_t span = (mode == 1) ? fs : vol;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = bx * blockDim.x + threadIdx.x;
   double a_q = 0, a_h = 0, a_hu = 0, a_u = 0, a_u2 = 0;
   if (i < span) {
     const uint32_t v = (mode == 1) ? m.ref * fs + i : i;
@@ -797,25 +828,47 @@ __global__ __launch_bounds__(256) void k_gather_direct(const float* __restrict__
     const int r = (int)(pix % nr), c = (int)(pix / nr);
     const int32_t* O = off + ((size_t)b * nw + k) * ((size_t)n1 * n1 + 1);
     const float* Bk = B0 + ((size_t)slot * nw + k) * p_sel * M;
+    // Patch columns in chunks of CH (all at once for small patches): bucket
+    // bounds of positions (r - ra, c - cb), bnd[.][e] for rows r-BS+1+e
+    // (clamped to the frame), e = BS the end of row r; then the first
+    // contribution of every bucket in one batch of independent loads, and
+    // further ones (buckets holding several patches) in order.
+    constexpr int CH = (BS <= 5) ? BS : 1;
     double acc = 0.0;
 #pragma unroll
-    for (int cb = 0; cb < BS; ++cb) {
-      const int pc = c - cb;
-      if (pc < 0 || pc >= (int)n1) continue;
-      const size_t colq = (size_t)n1 * pc;
-      // bucket bounds of rows r-ra, ra = 0..BS-1 (clamped to the column)
-      int32_t bnd[BS + 1];
+    for (int cb0 = 0; cb0 < BS; cb0 += CH) {
+      int32_t bnd[CH][BS + 1];
 #pragma unroll
-      for (int e = 0; e <= BS; ++e) {
-        const int pr = r - (BS - 1) + e;  // e = 0 -> row r-BS+1, e = BS -> end of row r
-        const int prc = min(max(pr, 0), (int)n1);
-        bnd[e] = (prc >= (int)n1) ? O[colq + n1] : O[colq + prc];
-      }
+      for (int h = 0; h < CH; ++h) {
+        const int pc = c - (cb0 + h);
+        const bool in = pc >= 0 && pc < (int)n1;
+        const size_t colq = (size_t)n1 * (in ? pc : 0);
 #pragma unroll
-      for (int ra = 0; ra < BS; ++ra) {
-        const int e = BS - 1 - ra;  // position r - ra
-        for (int32_t j = bnd[e]; j < bnd[e + 1]; ++j) acc += (double)Bk[(size_t)j * M + ra + BS * cb];
+        for (int e = 0; e <= BS; ++e) {
+          const int prc = min(max(r - (BS - 1) + e, 0), (int)n1);
+          bnd[h][e] = in ? O[colq + prc] : 0;
+        }
       }
+      float first[CH][BS];
+#pragma unroll
+      for (int h = 0; h < CH; ++h)
+#pragma unroll
+        for (int ra = 0; ra < BS; ++ra) {
+          const int e = BS - 1 - ra;  // position r - ra
+          const int32_t j = bnd[h][e];
+          first[h][ra] = (j < bnd[h][e + 1]) ? Bk[(size_t)j * M + ra + BS * (cb0 + h)] : 0.f;
+        }
+#pragma unroll
+      for (int h = 0; h < CH; ++h)
+#pragma unroll
+        for (int ra = 0; ra < BS; ++ra) {
+          const int e = BS - 1 - ra;
+          const int32_t j0 = bnd[h][e], j1 = bnd[h][e + 1];
+          if (j0 < j1) {
+            acc += (double)first[h][ra];
+            for (int32_t j = j0 + 1; j < j1; ++j) acc += (double)Bk[(size_t)j * M + ra + BS * (cb0 + h)];
+          }
+        }
     }
     const double uh = acc * invw[(size_t)b * vol + v];
     if (mode == 0) {
@@ -850,7 +903,143 @@ __global__ __launch_bounds__(256) void k_gather_direct(const float* __restrict__
   }
   __syncthreads();
   if (threadIdx.x < 5)
-    partial[((size_t)slot * gridDim.x + blockIdx.x) * 5 + threadIdx.x] =
+    partial[((size_t)slot * gridDim.x + bx) * 5 + threadIdx.x] =
+        ((sm[threadIdx.x][0] + sm[threadIdx.x][1]) + sm[threadIdx.x][2]) + sm[threadIdx.x][3];
+}
+
+// Tiled gather for patches up to 4 x 4, same sums in the same order as
+// k_gather_direct (so the results are identical).  A block covers a tile of
+// TR x TC voxels of one frame; the patches that can cover the tile sit in
+// one contiguous bucket range per patch column, which the block copies into
+// LDS with coalesced loads (one plane per patch row, so lanes on
+// neighbouring voxels read neighbouring words); then one thread per voxel
+// sums from LDS.  A tile whose ranges hold more than ROWS patches (many
+// trajectories meeting) reads them from global memory instead.
+template <int BS>
+__global__ __launch_bounds__(256) void k_gather_tile(const float* __restrict__ B0, const int32_t* __restrict__ off,
+                                                     const double* __restrict__ invw,
+                                                     const double* __restrict__ u64,
+                                                     const WinMeta* __restrict__ meta,
+                                                     const uint32_t* __restrict__ wl, uint32_t nr, uint32_t fs,
+                                                     uint32_t nw, uint32_t vol, uint32_t n1, uint32_t p_sel,
+                                                     int mode, double* __restrict__ out,
+                                                     double* __restrict__ partial) {
+  constexpr int M = BS * BS, TR = 32, TC = 8, NC = TC + BS - 1, NRW = TR + BS, ROWS = 576, RS = ROWS + 1;
+  __shared__ float sB[M * RS];
+  __shared__ int32_t sO[NC][NRW];  // bucket offsets of the tile's positions (+ one past the last row)
+  __shared__ int32_t sl[NC + 1];   // first LDS row of each patch column
+  __shared__ double sm[5][4];
+  uint32_t bx, slot;
+  xcd_block(bx, slot);
+  const uint32_t b = wl[slot];
+  const WinMeta m = meta[b];
+  const uint32_t ntr = (nr + TR - 1) / TR, ntc = (nr + TC - 1) / TC;  // square frames
+  const uint32_t tr = bx % ntr, tc = (bx / ntr) % ntc;
+  const uint32_t k = (mode == 1) ? m.ref : bx / (ntr * ntc);
+  const int r0 = (int)(tr * TR), c0 = (int)(tc * TC);
+  const int r = r0 + (int)(threadIdx.x % TR), c = c0 + (int)(threadIdx.x / TR);
+  const bool live = r < (int)nr && c < (int)nr;
+  const int32_t* O = off + ((size_t)b * nw + k) * ((size_t)n1 * n1 + 1);
+  const float* Bk = B0 + ((size_t)slot * nw + k) * p_sel * M;
+  // positions that can cover the tile: rows prlo..prhi, columns pclo..pchi
+  const int prlo = max(r0 - (BS - 1), 0), prhi = min(r0 + TR - 1, (int)n1 - 1);
+  const int pclo = max(c0 - (BS - 1), 0), pchi = min(c0 + TC - 1, (int)n1 - 1);
+  const int ncol = pchi - pclo + 1, nrow = prhi - prlo + 2;  // nrow counts the end offset
+  for (int e = threadIdx.x; e < ncol * nrow; e += blockDim.x) {
+    const int lc = e / nrow, i = e % nrow;
+    sO[lc][i] = O[(size_t)n1 * (pclo + lc) + prlo + i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t t = 0;
+    for (int lc = 0; lc < ncol; ++lc) {
+      sl[lc] = t;
+      t += sO[lc][nrow - 1] - sO[lc][0];
+    }
+    sl[ncol] = t;
+  }
+  __syncthreads();
+  const int32_t tot = sl[ncol];
+  const bool staged = tot <= ROWS;
+  if (staged) {
+    constexpr int W = (M % 4 == 0) ? 4 : 1;  // floats per load
+    for (int e = threadIdx.x; e < tot * (M / W); e += blockDim.x) {
+      const int lr = e / (M / W), q = (e % (M / W)) * W;
+      int lc = 0;
+      while (lc + 1 < ncol && lr >= sl[lc + 1]) ++lc;
+      const int32_t j = sO[lc][0] + (lr - sl[lc]);
+      if constexpr (W == 4) {
+        const float4 v = *reinterpret_cast<const float4*>(Bk + (size_t)j * M + q);
+        sB[(q + 0) * RS + lr] = v.x;
+        sB[(q + 1) * RS + lr] = v.y;
+        sB[(q + 2) * RS + lr] = v.z;
+        sB[(q + 3) * RS + lr] = v.w;
+      } else {
+        sB[q * RS + lr] = Bk[(size_t)j * M + q];
+      }
+    }
+  }
+  __syncthreads();
+  double a_q = 0, a_h = 0, a_hu = 0, a_u = 0, a_u2 = 0;
+  if (live) {
+    double acc = 0.0;
+#pragma unroll
+    for (int cb = 0; cb < BS; ++cb) {
+      const int pc = c - cb;
+      if (pc < 0 || pc >= (int)n1) continue;
+      const int lc = pc - pclo;
+      const int32_t base = sl[lc] - sO[lc][0];  // bucket index j -> LDS row j + base
+      int32_t bnd[BS + 1];
+#pragma unroll
+      for (int e = 0; e <= BS; ++e) {
+        const int prc = min(max(r - (BS - 1) + e, 0), (int)n1);
+        bnd[e] = sO[lc][prc - prlo];
+      }
+#pragma unroll
+      for (int ra = 0; ra < BS; ++ra) {
+        const int e = BS - 1 - ra;  // position r - ra
+        const int q = ra + BS * cb;
+        if (staged) {
+          for (int32_t j = bnd[e]; j < bnd[e + 1]; ++j) acc += (double)sB[q * RS + j + base];
+        } else {
+          for (int32_t j = bnd[e]; j < bnd[e + 1]; ++j) acc += (double)Bk[(size_t)j * M + q];
+        }
+      }
+    }
+    const uint32_t pix = (uint32_t)r + nr * (uint32_t)c, v = k * fs + pix;
+    const double uh = acc * invw[(size_t)b * vol + v];
+    if (mode == 0) {
+      const double U = u64[(size_t)b * vol + v];
+      a_q = uh * uh;
+      a_h = uh;
+      a_hu = uh * U;
+      a_u = U;
+      a_u2 = U * U;
+    } else if (mode == 1) {
+      out[(size_t)m.t * fs + pix] = uh * m.uMax;
+    } else {
+      out[v] = uh;
+    }
+  }
+  if (mode != 0) return;
+  for (int o = 32; o > 0; o >>= 1) {
+    a_q += __shfl_xor(a_q, o);
+    a_h += __shfl_xor(a_h, o);
+    a_hu += __shfl_xor(a_hu, o);
+    a_u += __shfl_xor(a_u, o);
+    a_u2 += __shfl_xor(a_u2, o);
+  }
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) {
+    sm[0][w] = a_q;
+    sm[1][w] = a_h;
+    sm[2][w] = a_hu;
+    sm[3][w] = a_u;
+    sm[4][w] = a_u2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5)
+    partial[((size_t)slot * gridDim.x + bx) * 5 + threadIdx.x] =
         ((sm[threadIdx.x][0] + sm[threadIdx.x][1]) + sm[threadIdx.x][2]) + sm[threadIdx.x][3];
 }
 
@@ -935,19 +1124,25 @@ hipError_t launch_eval(const float* factors, const int32_t* pos, const uint32_t*
   return hipErrorInvalidValue;
 }
 
-uint32_t gather_blocks(const Shape& sh, int mode) { return ((mode == 1 ? sh.fs : sh.vol) + 255) / 256; }
+// tiled gather for patches up to 4 x 4 (32 x 8 voxel tiles), else one
+// 256-voxel run per block
+uint32_t gather_blocks(const Shape& sh, int mode) {
+  if (sh.bs <= 4) return ((sh.nr + 31) / 32) * ((sh.nr + 7) / 8) * (mode == 1 ? 1 : sh.nw);
+  return ((mode == 1 ? sh.fs : sh.vol) + 255) / 256;
+}
 
 hipError_t launch_gather(const float* B0, const int32_t* off, const double* invw, const double* u64,
                          const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int mode, double* out,
                          double* partial, hipStream_t s) {
   const uint32_t nblk = gather_blocks(sh, mode);
-#define PG_GA(B)                                                                                                 \
+#define PG_GA(B, K)                                                                                              \
   case B:                                                                                                      \
-    hipLaunchKernelGGL(k_gather_direct<B>, dim3(nblk, na), dim3(256), 0, s, B0, off, invw, u64, meta, wl,       \
-                       sh.nr, sh.fs, sh.nw, sh.vol, sh.n1, sh.p_sel, mode, out, partial);                      \
+    hipLaunchKernelGGL(K<B>, dim3(nblk, na), dim3(256), 0, s, B0, off, invw, u64, meta, wl, sh.nr, sh.fs,       \
+                       sh.nw, sh.vol, sh.n1, sh.p_sel, mode, out, partial);                                    \
     break;
   switch (sh.bs) {
-    PG_GA(2) PG_GA(3) PG_GA(4) PG_GA(5) PG_GA(6) PG_GA(7) PG_GA(8)
+    PG_GA(2, k_gather_tile) PG_GA(3, k_gather_tile) PG_GA(4, k_gather_tile)
+    PG_GA(5, k_gather_direct) PG_GA(6, k_gather_direct) PG_GA(7, k_gather_direct) PG_GA(8, k_gather_direct)
     default: return hipErrorInvalidValue;
   }
 #undef PG_GA
