@@ -23,13 +23,17 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <fstream>
 #include <future>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <random>
 #include <string>
 #include <thread>
 #include <vector>
+
+#include <sched.h>
 
 #include "../../include/pgsvt.h"
 #include "common.hpp"
@@ -327,33 +331,40 @@ uint64_t seed_of(const pgsvt_params& p) {
 
 // ------------------------------------------------------- host job pool --
 // Noise estimation and lambda start points of every window, computed on
-// host threads while the GPU works; results are awaited per window.
+// host threads while the GPU works.  The work is one ordered job list: the
+// leaf statistics of each distinct (frame, normalisation) a window needs,
+// then the window's own fit, window after window.  Threads take jobs in
+// list order, so a window's fit only waits for leaf jobs taken before it,
+// and windows become ready roughly in order, as the search admits them.
 struct WindowPrep {
   double alpha = -1, mu = -1, sigma = -1, start = 0;
+  std::string err;  // non-empty: the estimate failed
 };
 
 class PrepPool {
  public:
-  template <class F>
-  PrepPool(uint32_t count, uint32_t threads, F fn) : res_(count), ready_(count, 0) {
-    fn_ = fn;
-    for (uint32_t k = 0; k < std::max(1u, threads); ++k)
-      th_.emplace_back([this, count] {
+  PrepPool(std::vector<std::function<void(PrepPool&)>> jobs, uint32_t windows, uint32_t threads)
+      : jobs_(std::move(jobs)), res_(windows), ready_(windows, 0) {
+    const uint32_t n = (uint32_t)jobs_.size();
+    for (uint32_t k = 0; k < std::max(1u, std::min(threads, n)); ++k)
+      th_.emplace_back([this, n] {
         for (;;) {
-          const uint32_t t = next_.fetch_add(1);
-          if (t >= count) return;
-          WindowPrep r = fn_(t);
-          {
-            std::lock_guard<std::mutex> g(mu_);
-            res_[t] = r;
-            ready_[t] = 1;
-          }
-          cv_.notify_all();
+          const uint32_t j = next_.fetch_add(1);
+          if (j >= n) return;
+          jobs_[j](*this);
         }
       });
   }
   ~PrepPool() {
     for (auto& t : th_) t.join();
+  }
+  void publish(uint32_t t, WindowPrep r) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      res_[t] = std::move(r);
+      ready_[t] = 1;
+    }
+    cv_.notify_all();
   }
   WindowPrep wait(uint32_t t) {
     std::unique_lock<std::mutex> g(mu_);
@@ -366,7 +377,7 @@ class PrepPool {
   }
 
  private:
-  std::function<WindowPrep(uint32_t)> fn_;
+  std::vector<std::function<void(PrepPool&)>> jobs_;
   std::vector<WindowPrep> res_;
   std::vector<char> ready_;
   std::vector<std::thread> th_;
@@ -662,10 +673,22 @@ struct Pipeline {
   }
 };
 
+// Host threads for n_jobs (-1 = all): the CPUs this process may run on,
+// capped by a cgroup CPU quota when one is set.
 uint32_t thread_count(int64_t n_jobs) {
   if (n_jobs == 0 || n_jobs == 1) return 1;
   if (n_jobs > 0) return (uint32_t)n_jobs;
-  return std::max(1u, std::thread::hardware_concurrency());
+  uint32_t n = std::max(1u, std::thread::hardware_concurrency());
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof set, &set) == 0) n = std::max(1, CPU_COUNT(&set));
+  std::ifstream q("/sys/fs/cgroup/cpu.max");
+  std::string quota;
+  double period = 0;
+  if (q >> quota >> period && quota != "max" && period > 0) {
+    const double cpus = std::atof(quota.c_str()) / period;
+    if (cpus >= 1) n = std::min(n, (uint32_t)std::ceil(cpus));
+  }
+  return n;
 }
 
 int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, const pgsvt_params* pp, double* Y,
@@ -695,26 +718,65 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     P.alloc_batch(nf);
   }
 
-  LeafCache cache;
+  // leaf statistics per distinct (frame, window maximum), computed once
+  struct LeafJob {
+    uint32_t frame;
+    double scale;
+    std::promise<std::shared_ptr<const FrameLeaves>> pr;
+    std::shared_future<std::shared_ptr<const FrameLeaves>> fu;
+  };
+  std::vector<std::unique_ptr<LeafJob>> leaf_jobs;
   std::unique_ptr<PrepPool> pool;
   if (P.optimize) {
-    pool.reset(new PrepPool(nf, thread_count(p.n_jobs), [&](uint32_t t) {
-      WindowPrep w;
-      w.alpha = alpha0;
-      w.mu = mu0;
-      w.sigma = sigma0;
+    std::map<std::pair<uint32_t, uint64_t>, uint32_t> key_of;
+    std::vector<std::function<void(PrepPool&)>> jobs;
+    for (uint32_t t = 0; t < nf; ++t) {
       const WinMeta m = P.meta_of(t);
-      std::vector<std::shared_ptr<const FrameLeaves>> leaves;
+      uint64_t sbits;
+      std::memcpy(&sbits, &m.uMax, 8);
+      std::vector<uint32_t> need;
       for (uint32_t k = 0; k < g.Nw; ++k) {
-        const uint32_t f = m.s0 + k;
-        leaves.push_back(cache.get(f, m.uMax, nr, [&](double* buf) { P.host_frame(f, m.uMax, buf); }));
+        const auto key = std::make_pair(m.s0 + k, sbits);
+        auto it = key_of.find(key);
+        if (it == key_of.end()) {
+          const uint32_t id = (uint32_t)leaf_jobs.size();
+          leaf_jobs.emplace_back(new LeafJob{m.s0 + k, m.uMax, {}, {}});
+          LeafJob* J = leaf_jobs.back().get();
+          J->fu = J->pr.get_future().share();
+          jobs.push_back([J, &P, nr](PrepPool&) {
+            try {
+              std::vector<double> buf((size_t)nr * nr);
+              P.host_frame(J->frame, J->scale, buf.data());
+              J->pr.set_value(std::make_shared<const FrameLeaves>(frame_leaves(buf.data(), nr)));
+            } catch (...) {
+              J->pr.set_exception(std::current_exception());
+            }
+          });
+          it = key_of.emplace(key, id).first;
+        }
+        need.push_back(it->second);
       }
-      estimate_from_leaves(leaves, nr, p.noise_method, w.alpha, w.mu, w.sigma);
-      const double OoNxNyNt = 1.0 / (nc * nr * g.Nt);
-      const double start = (lambda0 >= 0.0) ? lambda0 : P.host_window_accu(t) * OoNxNyNt;
-      w.start = std::max(0.0, start);
-      return w;
-    }));
+      jobs.push_back([t, need, &leaf_jobs, &P, &p, g, nr, nc, lambda0, alpha0, mu0, sigma0](PrepPool& pool) {
+        WindowPrep w;
+        w.alpha = alpha0;
+        w.mu = mu0;
+        w.sigma = sigma0;
+        try {
+          std::vector<std::shared_ptr<const FrameLeaves>> leaves;
+          for (uint32_t id : need) leaves.push_back(leaf_jobs[id]->fu.get());
+          estimate_from_leaves(leaves, nr, p.noise_method, w.alpha, w.mu, w.sigma);
+          const double OoNxNyNt = 1.0 / (nc * nr * g.Nt);
+          const double start = (lambda0 >= 0.0) ? lambda0 : P.host_window_accu(t) * OoNxNyNt;
+          w.start = std::max(0.0, start);
+        } catch (const std::exception& e) {
+          w.err = std::string("noise estimate: ") + e.what();
+        } catch (...) {
+          w.err = "noise estimate failed";
+        }
+        pool.publish(t, std::move(w));
+      });
+    }
+    pool.reset(new PrepPool(std::move(jobs), nf, thread_count(p.n_jobs)));
   }
 
   int status = 0;
@@ -738,6 +800,7 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
       uint32_t n_started = 0;
       auto admit = [&](uint32_t b) {
         prep[b] = pool->wait(t0 + b);
+        if (!prep[b].err.empty()) throw std::runtime_error(prep[b].err);
         const double start = prep[b].start;
         const double ub = std::max(100.0, start);
         opt[b].reset(new Sbplx1D(start, 0.0, ub, std::sqrt(start), p.tol, 1e-12, (int)p.max_iter));

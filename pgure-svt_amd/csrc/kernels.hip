@@ -402,13 +402,16 @@ hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs
 // --------------------------------------------------- batched Jacobi SVD --
 // One-sided (Hestenes) Jacobi on the m x n Casorati matrix of each patch
 // (arma::svd_econ at src/svt.hpp:111).  Output per patch, sorted by
-// descending singular value: V^T (n x n), W^T = (X V)^T (n x m) and s (n).  A group of L lanes holds one matrix,
-// lane q owning row q of X and row q of V; column dot products are
-// butterflies across the group (DPP inside a 16-lane row, then lane swaps
-// for 32/64-lane groups).  Every lane of a group ends the butterfly with the
-// bit-identical sum, so all lanes apply the same rotation.  Column norms are
-// carried through the sweep with the exact Jacobi update (a -= t c, b += t c)
-// and refreshed at the start of every sweep, so a pair costs one butterfly.
+// descending singular value: V^T (n x n), W^T = (X V)^T (n x m) and s (n).
+// A group of L lanes holds one matrix; lane l owns rows l, l + L, l + 2L ...
+// of X and of V (R and VR rows), so the rotation parameters, which every
+// lane of the group computes, are shared by R rows.  Column dot products
+// are a sum over the lane's rows, then a butterfly across the group (DPP
+// inside a 16-lane row, lane swaps for 32/64-lane groups).  Every lane ends
+// the butterfly with the bit-identical sum, so all lanes apply the same
+// rotation.  Column norms are carried through the sweep with the exact
+// Jacobi update (a -= t c, b += t c) and refreshed at the start of every
+// sweep, so a pair costs one butterfly.
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
@@ -416,10 +419,10 @@ __device__ __forceinline__ float dpp_f(float v) {
 
 template <int L>
 __device__ __forceinline__ float group_sum(float v) {
-  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
-  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
-  v += dpp_f<0x141>(v);  // row_half_mirror
-  v += dpp_f<0x140>(v);  // row_mirror
+  if (L >= 2) v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+  if (L >= 4) v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+  if (L >= 8) v += dpp_f<0x141>(v);  // row_half_mirror
+  if (L >= 16) v += dpp_f<0x140>(v); // row_mirror
   if (L >= 32) v += __shfl_xor(v, 16);
   if (L >= 64) v += __shfl_xor(v, 32);
   return v;
@@ -431,8 +434,8 @@ __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ u32, cons
                                              uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol,
                                              uint32_t lim, uint32_t fstride, int max_sweeps, float tol2,
                                              float* __restrict__ factors, uint32_t* __restrict__ stats) {
-  constexpr int M = BS * BS;
-  static_assert(M <= L && N <= L, "group too small");
+  constexpr int M = BS * BS, R = (M + L - 1) / L, VR = (N + L - 1) / L, SR = (N + 3) & ~3;
+  static_assert(L * VR >= SR, "V^T padding not covered");
   const uint32_t b = blockIdx.y;
   const int lane = threadIdx.x % L;
   const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / L;
@@ -440,26 +443,31 @@ __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ u32, cons
   const bool valid = var < nvar;  // uniform per group
   const uint32_t it = valid ? sel[p] : 0;
 
-  float x[N], v[N];
-  const int rr = lane % BS, cc = lane / BS;
+  float x[R][N], v[VR][N];
   const float* U = u32 + ((size_t)b * nvar + (valid ? var : 0)) * vol;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    float val = 0.f;
-    if (valid && lane < M) {
-      const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
-      const uint32_t row = min((uint32_t)(pk & 0xffff), lim), col = min((uint32_t)(pk >> 16), lim);
-      val = U[(size_t)k * fs + (row + rr) + (size_t)nr * (col + cc)];
+    const int32_t pk = valid ? pos[((size_t)b * nw + k) * p_all + it] : 0;
+    const uint32_t row = min((uint32_t)(pk & 0xffff), lim), col = min((uint32_t)(pk >> 16), lim);
+#pragma unroll
+    for (int h = 0; h < R; ++h) {
+      const int q = lane + L * h;
+      x[h][k] = (valid && q < M) ? U[(size_t)k * fs + (row + q % BS) + (size_t)nr * (col + q / BS)] : 0.f;
     }
-    x[k] = val;
-    v[k] = (lane == k) ? 1.f : 0.f;
+#pragma unroll
+    for (int h = 0; h < VR; ++h) v[h][k] = (lane + L * h == k) ? 1.f : 0.f;
   }
 
   float a[N];
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) a[k] = group_sum<L>(x[k] * x[k]);
+    for (int k = 0; k < N; ++k) {
+      float t = x[0][k] * x[0][k];
+#pragma unroll
+      for (int h = 1; h < R; ++h) t = fmaf(x[h][k], x[h][k], t);
+      a[k] = group_sum<L>(t);
+    }
     // sum of t^2 over the sweep: zero exactly when no pair rotated.  A
     // floating-point chain (never reassociated) keeps this in one register.
     float tsum = 0.f;
@@ -469,29 +477,37 @@ __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ u32, cons
       for (int j = i + 1; j < N; ++j) {
         // rotation zeroing the (i, j) inner product: zeta = (b - a) / 2c,
         // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), cs = 1/sqrt(1+t^2)
-        const float c = group_sum<L>(x[i] * x[j]);
+        float cp = x[0][i] * x[0][j];
+#pragma unroll
+        for (int h = 1; h < R; ++h) cp = fmaf(x[h][i], x[h][j], cp);
+        const float c = group_sum<L>(cp);
         const bool rot = c * c > tol2 * a[i] * a[j];
         const float zeta = (a[j] - a[i]) * __builtin_amdgcn_rcpf(c + c);
         const float den = fabsf(zeta) + __builtin_amdgcn_sqrtf(fmaf(zeta, zeta, 1.f));
         const float t = rot ? copysignf(__builtin_amdgcn_rcpf(den), zeta) : 0.f;
         const float cs = __builtin_amdgcn_rsqf(fmaf(t, t, 1.f));
         const float sn = cs * t;
-        const float xi = x[i], xj = x[j];
-        x[i] = cs * xi - sn * xj;
-        x[j] = sn * xi + cs * xj;
-        const float vi = v[i], vj = v[j];
-        v[i] = cs * vi - sn * vj;
-        v[j] = sn * vi + cs * vj;
+#pragma unroll
+        for (int h = 0; h < R; ++h) {
+          const float xi = x[h][i], xj = x[h][j];
+          x[h][i] = cs * xi - sn * xj;
+          x[h][j] = sn * xi + cs * xj;
+        }
+#pragma unroll
+        for (int h = 0; h < VR; ++h) {
+          const float vi = v[h][i], vj = v[h][j];
+          v[h][i] = cs * vi - sn * vj;
+          v[h][j] = sn * vi + cs * vj;
+          // Pin the V update next to its rotation.  V only feeds the next
+          // sweep, so without this the compiler sinks every V update to the
+          // end of the sweep and keeps (cs, sn) of all rotations live at
+          // once.  The empty statement only names the two registers; it has
+          // no instruction and no memory access.
+          asm volatile("" : "+v"(v[h][i]), "+v"(v[h][j]));
+        }
         a[i] = fmaf(-t, c, a[i]);
         a[j] = fmaf(t, c, a[j]);
         tsum = fmaf(t, t, tsum);
-        // Pin the V update next to its rotation.  V only feeds the next
-        // sweep, so without this the compiler sinks every V update to the
-        // end of the sweep and keeps (cs, sn) of all rotations live at once
-        // (about 2 n(n-1)/2 extra registers).  The empty statement only
-        // names the two registers; it has no instruction and no memory
-        // access.
-        asm volatile("" : "+v"(v[i]), "+v"(v[j]));
       }
     }
     if (!__any(tsum > 0.f)) break;
@@ -499,14 +515,18 @@ __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ u32, cons
   if (stats && lane == 0 && valid) atomicMax(stats, (uint32_t)sweep + 1);
   // final column norms (identical in every lane of the group)
 #pragma unroll
-  for (int k = 0; k < N; ++k) a[k] = group_sum<L>(x[k] * x[k]);
+  for (int k = 0; k < N; ++k) {
+    float t = x[0][k] * x[0][k];
+#pragma unroll
+    for (int h = 1; h < R; ++h) t = fmaf(x[h][k], x[h][k], t);
+    a[k] = group_sum<L>(t);
+  }
   if (!valid) return;
   // Store in descending singular-value order, transposed: row r of V^T is
   // the right singular vector of the r-th largest singular value (row
   // stride SR, padded with zeros), row r of W^T is the matching column of
   // W = X V = U S (the rotated Casorati columns), then the sorted values.
   // rank(l) counts the values that sort before column l (ties by index).
-  constexpr int SR = (N + 3) & ~3;
   float* F = factors + (((size_t)b * nvar + var) * p_sel + p) * fstride;
   float* FW = F + N * SR;
   float* FS = FW + N * M;
@@ -515,9 +535,17 @@ __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ u32, cons
     int rank = 0;
 #pragma unroll
     for (int m = 0; m < N; ++m) rank += (a[m] > a[l] || (a[m] == a[l] && m < l)) ? 1 : 0;
-    if (lane < N) F[rank * SR + lane] = v[l];
-    if (lane >= N && lane < SR) F[rank * SR + lane] = 0.f;
-    if (lane < M) FW[rank * M + lane] = x[l];
+#pragma unroll
+    for (int h = 0; h < VR; ++h) {
+      const int vr = lane + L * h;
+      if (vr < N) F[rank * SR + vr] = v[h][l];
+      else if (vr < SR) F[rank * SR + vr] = 0.f;
+    }
+#pragma unroll
+    for (int h = 0; h < R; ++h) {
+      const int q = lane + L * h;
+      if (q < M) FW[rank * M + q] = x[h][l];
+    }
     if (lane == 0) FS[rank] = sqrtf(a[l]);
   }
 }
@@ -1070,14 +1098,15 @@ __global__ __launch_bounds__(256) void k_risk_reduce(const double* __restrict__ 
 }
 
 // shape table of the instantiated SVD / evaluation kernels (patch size,
-// window frames, lanes per matrix)
+// window frames, lanes per patch in the evaluation, lanes per matrix in the
+// SVD)
 #define PG_SHAPES(X) \
-  X(4, 3, 16) X(4, 5, 16) X(4, 7, 16) X(4, 9, 16) X(4, 11, 16) X(4, 13, 16) X(4, 15, 16) \
-  X(2, 3, 16) X(3, 3, 16) X(3, 5, 16) X(3, 7, 16) X(3, 9, 16) \
-  X(6, 7, 64) X(6, 15, 64) X(6, 31, 64) X(8, 7, 64) X(8, 15, 64) X(8, 31, 64)
+  X(4, 3, 16, 4) X(4, 5, 16, 4) X(4, 7, 16, 4) X(4, 9, 16, 4) X(4, 11, 16, 4) X(4, 13, 16, 4) X(4, 15, 16, 4) \
+  X(2, 3, 16, 4) X(3, 3, 16, 4) X(3, 5, 16, 4) X(3, 7, 16, 4) X(3, 9, 16, 4) \
+  X(6, 7, 64, 16) X(6, 15, 64, 16) X(6, 31, 64, 32) X(8, 7, 64, 16) X(8, 15, 64, 32) X(8, 31, 64, 64)
 
 bool svd_shape_supported(uint32_t bs, uint32_t nw) {
-#define PG_CHK(B, N, L) \
+#define PG_CHK(B, N, L, LS) \
   if (bs == B && nw == N) return true;
   PG_SHAPES(PG_CHK)
 #undef PG_CHK
@@ -1088,7 +1117,7 @@ hipError_t launch_svd(const float* u32, const int32_t* pos, const uint32_t* sel,
                       const Shape& sh, float* factors, int max_sweeps, float tol, uint32_t* stats, hipStream_t s) {
   const uint32_t lim = sh.nr - sh.bs;
   const float tol2 = tol * tol;
-#define PG_SVD(B, N, L)                                                                                          \
+#define PG_SVD(B, N, LE, L)                                                                                          \
   if (sh.bs == B && sh.nw == N) {                                                                              \
     const uint32_t groups = nvar * sh.p_sel;                                                                   \
     const dim3 grid((groups * L + 255) / 256, nb);                                                             \
@@ -1112,7 +1141,7 @@ hipError_t launch_eval(const float* factors, const int32_t* pos, const uint32_t*
                        int exp_w, const double* dw, const int32_t* inv, float* B0, double* partial, hipStream_t s) {
   const uint32_t lim = sh.nr - sh.bs;
   const uint32_t nblk = eval_blocks(sh, nvar_run);
-#define PG_EV(B, N, L)                                                                                           \
+#define PG_EV(B, N, L, LS)                                                                                           \
   if (sh.bs == B && sh.nw == N) {                                                                              \
     hipLaunchKernelGGL((k_eval<B, N, L>), dim3(nblk, na), dim3(256), 0, s, factors, pos, sel, meta, wl,       \
                        nvar_stored, nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, \

@@ -10,7 +10,6 @@
 //     sorting r in every IRLS iteration;
 //   * medians and order statistics use nth_element instead of full sorts;
 //   * repeated quadtree nodes are evaluated once.
-// Leaf statistics of a frame are cached per (frame, normalisation).
 #include "noise.hpp"
 
 #include <algorithm>
@@ -83,19 +82,43 @@ double robust_mean(const std::vector<double>& A) {
   const double tol = 1e-6, eps = 1e-12;
   double m = 0, m0 = 1e12;
   std::vector<double> w(N, 1.0);
+  // Each sum keeps its own even/odd accumulator pair in index order, so
+  // fusing the passes changes no rounding: pass 1 forms sum(w A) and
+  // sum(w); pass 2 forms sum |A - m| and the next weights, which are only
+  // used when the loop goes on.
   for (uint32_t it = 0; it < 10000; ++it) {
-    m = accu2f(N, [&](size_t k) { return w[k] * A[k]; });
-    const double aux = accu2f(N, [&](size_t k) { return w[k]; });
+    double wa1 = 0, wa2 = 0, w1 = 0, w2 = 0;
+    size_t k = 0;
+    for (; k + 1 < N; k += 2) {
+      wa1 += w[k] * A[k];
+      wa2 += w[k + 1] * A[k + 1];
+      w1 += w[k];
+      w2 += w[k + 1];
+    }
+    if (k < N) {
+      wa1 += w[k] * A[k];
+      w1 += w[k];
+    }
+    m = wa1 + wa2;
+    const double aux = w1 + w2;
     m = (std::fabs(aux) < eps) ? m0 : m / aux;
-    const double e = accu2f(N, [&](size_t k) { return std::fabs(A[k] - m); }) / (double)N;
-    if (std::fabs(m0 - m) < tol || e < tol) break;
-    m0 = m;
     const double d = ((a_hi - m) - (a_lo - m)) + eps;
     const double inv = 1. / d;
-    for (size_t k = 0; k < N; ++k) {
-      const double a = std::fabs((A[k] - m) * inv);
-      w[k] = (a < 0.75) ? 1. : 0.75 / a;
+    double e1 = 0, e2 = 0;
+    auto step = [&](size_t q) {
+      const double r = A[q] - m;
+      const double a = std::fabs(r * inv);
+      w[q] = (a < 0.75) ? 1. : 0.75 / a;
+      return std::fabs(r);
+    };
+    for (k = 0; k + 1 < N; k += 2) {
+      e1 += step(k);
+      e2 += step(k + 1);
     }
+    if (k < N) e1 += step(k);
+    const double e = (e1 + e2) / (double)N;
+    if (std::fabs(m0 - m) < tol || e < tol) break;
+    m0 = m;
   }
   return m;
 }
@@ -131,10 +154,34 @@ bool split_block(const double* F, uint32_t n, uint32_t i0, uint32_t j0, uint32_t
     a[k] = A((uint32_t)(k % N), (uint32_t)(k / N));
     rs[k] = res(k);
   }
-  const double accuZ = accu2f(R, [&](size_t k) { return a[k]; }) * OoR;
-  const double Sz = accu2f(R, [&](size_t k) { const double t = a[k] - accuZ; return t * t; }) * OoRm1;
-  const double accuR = accu2f(R, [&](size_t k) { return rs[k]; }) * OoR;
-  const double Se = accu2f(R, [&](size_t k) { const double t = rs[k] - accuR; return t * t; }) * OoRm1;
+  // four sums, each with its own even/odd accumulator pair (as accu)
+  double z1 = 0, z2 = 0, r1 = 0, r2 = 0;
+  for (size_t k = 0; k + 1 < R; k += 2) {
+    z1 += a[k];
+    z2 += a[k + 1];
+    r1 += rs[k];
+    r2 += rs[k + 1];
+  }
+  if (R % 2) {
+    z1 += a[R - 1];
+    r1 += rs[R - 1];
+  }
+  const double accuZ = (z1 + z2) * OoR, accuR = (r1 + r2) * OoR;
+  double s1 = 0, s2 = 0, e1 = 0, e2 = 0;
+  for (size_t k = 0; k + 1 < R; k += 2) {
+    const double t1 = a[k] - accuZ, t2 = a[k + 1] - accuZ;
+    const double u1 = rs[k] - accuR, u2 = rs[k + 1] - accuR;
+    s1 += t1 * t1;
+    s2 += t2 * t2;
+    e1 += u1 * u1;
+    e2 += u2 * u2;
+  }
+  if (R % 2) {
+    const double t1 = a[R - 1] - accuZ, u1 = rs[R - 1] - accuR;
+    s1 += t1 * t1;
+    e1 += u1 * u1;
+  }
+  const double Sz = (s1 + s2) * OoRm1, Se = (e1 + e2) * OoRm1;
   const double stat = (Sz > Se) ? Sz / Se : Se / Sz;
   for (int k = 0; k < 12; ++k)
     if (kDof[k] == N) return stat > kF0025[k];
@@ -259,25 +306,6 @@ FrameLeaves frame_leaves(const double* F, uint32_t n) {
     out.var.push_back(stat[hit].second);
   }
   return out;
-}
-
-std::shared_ptr<const FrameLeaves> LeafCache::get(uint32_t frame, double scale, uint32_t n,
-                                                  const std::function<void(double*)>& fill) {
-  uint64_t bits;
-  std::memcpy(&bits, &scale, 8);
-  const uint64_t key = (uint64_t)frame * 0x9E3779B97F4A7C15ULL ^ bits;
-  {
-    std::lock_guard<std::mutex> g(mu_);
-    auto it = map_.find(key);
-    if (it != map_.end() && scale_of_[key] == scale) return it->second;
-  }
-  std::vector<double> buf((size_t)n * n);
-  fill(buf.data());
-  auto r = std::make_shared<const FrameLeaves>(frame_leaves(buf.data(), n));
-  std::lock_guard<std::mutex> g(mu_);
-  map_[key] = r;
-  scale_of_[key] = scale;
-  return r;
 }
 
 void estimate_from_leaves(const std::vector<std::shared_ptr<const FrameLeaves>>& frames, uint32_t n,

@@ -2,10 +2,7 @@
 #pragma once
 
 #include <cstdint>
-#include <functional>
 #include <memory>
-#include <mutex>
-#include <unordered_map>
 #include <vector>
 
 namespace pgsvt {
@@ -14,21 +11,6 @@ namespace pgsvt {
 // leaf order (src/noise.hpp:55-100).
 struct FrameLeaves {
   std::vector<double> mean, var;
-};
-
-// Per-frame leaf statistics depend only on the frame and on the window
-// maximum it is normalised by, so windows that share both share the result.
-class LeafCache {
- public:
-  // leaves of `frame` normalised by `scale`; `fill(buf)` writes the
-  // normalised n x n frame into buf and is only called on a miss
-  std::shared_ptr<const FrameLeaves> get(uint32_t frame, double scale, uint32_t n,
-                                         const std::function<void(double*)>& fill);
-
- private:
-  std::mutex mu_;
-  std::unordered_map<uint64_t, std::shared_ptr<const FrameLeaves>> map_;
-  std::unordered_map<uint64_t, double> scale_of_;
 };
 
 // NoiseEstimator::Estimate (src/noise.hpp:35-153) for a window of nt frames,
