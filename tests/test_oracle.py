@@ -185,3 +185,64 @@ def test_reference_quality_threshold_fixture():
     Y = Y.astype(np.uint16)
     Z, est = O.denoise(Y, random_seed=101, motion_filter=5, n_jobs=4)
     assert nsed(X, Z) < 0.025
+
+
+def _smooth_texture(n, seed):
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(seed)
+    return gaussian_filter(rng.standard_normal((n, n)), 2.0, mode="wrap") * 50 + 100
+
+
+@pytest.mark.parametrize("dy,dx", [(1, 2), (-2, 1), (0, -3), (3, 0)])
+@pytest.mark.parametrize("t", [0, 7, 15])
+def test_arps_recovers_known_shift(dy, dx, t):
+    # A smooth texture translated by (dy, dx) per frame.  ARPS moves every
+    # block from its grid position in the source frame (src/arps.hpp:186-190,
+    # 370-373), so frame k of the trajectory is the grid position plus one
+    # shift forward of the reference frame and minus one shift behind it.
+    # ARPS is a local search, so (1) most interior blocks land exactly there
+    # and (2) every block ends on a rood local minimum of its cost inside the
+    # search window: costs only fall along the LDSP/SDSP path and ties keep
+    # the centre (src/arps.hpp:289-294, 299-368).
+    n, nf, bs, mw = 48, 17, 4, 7
+    base = _smooth_texture(n, 100 + dy * 7 + dx)
+    X = np.asfortranarray(np.stack([np.roll(base, (dy * k, dx * k), axis=(0, 1)) for k in range(nf)], axis=2))
+    tr = O.window_trajectories(X, t, motion_estimation=1, motion_window=mw)
+    n1 = n - bs + 1
+    s0, ref = (0, t) if t < 7 else ((t - 7, 7) if t < nf - 7 else (nf - 15, t - (nf - 15)))
+    u = X[:, :, s0:s0 + 15] / X[:, :, s0:s0 + 15].max()
+    it = np.arange(n1 * n1)
+    gi, gj = it % n1, it // n1
+    inner = (gi >= 8) & (gi <= n1 - 9) & (gj >= 8) & (gj <= n1 - 9)
+    for k in range(15):
+        if k == ref:
+            assert (tr[k, :, 0] == gi).all() and (tr[k, :, 1] == gj).all()
+            continue
+        sgn = 1 if k > ref else -1
+        ok = (tr[k, inner, 0] == gi[inner] + sgn * dy) & (tr[k, inner, 1] == gj[inner] + sgn * dx)
+        assert ok.mean() > 0.75, (k, ref, ok.mean())
+        src = k - 1 if k > ref else k + 1  # the block comes from the neighbouring frame towards ref
+        for b in np.where(inner)[0][::7]:
+            i, j = gi[b], gj[b]
+            A = u[i:i + bs, j:j + bs, src]
+            cost = lambda y, x: ((A - u[y:y + bs, x:x + bs, k]) ** 2).sum() / (bs * bs)
+            y, x = tr[k, b]
+            c0 = cost(y, x)
+            for ny, nx in ((y - 1, x), (y + 1, x), (y, x - 1), (y, x + 1)):
+                if 0 <= ny < n1 and 0 <= nx < n1 and abs(ny - i) <= mw and abs(nx - j) <= mw:
+                    assert c0 <= cost(ny, nx) + 1e-12, (k, b, (y, x), (ny, nx))
+
+
+def test_arps_motion_window_clamps_search():
+    # a shift beyond the search window cannot be reached: no interior block
+    # may report a displacement larger than motion_window (src/arps.hpp:299-368)
+    n, nf, bs, mw = 48, 16, 4, 3
+    base = _smooth_texture(n, 11)
+    X = np.asfortranarray(np.stack([np.roll(base, (5 * k, 0), axis=(0, 1)) for k in range(nf)], axis=2))
+    tr = O.window_trajectories(X, 7, motion_estimation=1, motion_window=mw)
+    n1 = n - bs + 1
+    it = np.arange(n1 * n1)
+    gi, gj = it % n1, it // n1
+    for k in range(15):
+        assert np.abs(tr[k, :, 0] - gi).max() <= mw and np.abs(tr[k, :, 1] - gj).max() <= mw
