@@ -92,6 +92,12 @@ int pgsvt_denoise_device(const void* X_dev, int dtype, uint32_t n_rows,
                          const pgsvt_params* p, double* Y_dev,
                          double* estimates);
 
+/* Device memory is cached per device between calls (buffers of one call
+ * are reused by the next call of the same shape).  pgsvt_release_cache
+ * waits for the current device and frees its cache.  No reference
+ * counterpart (the reference allocates per window on the host). */
+int pgsvt_release_cache(void);
+
 /* Per-stage timing of later pgsvt_denoise* calls (no reference
  * counterpart; the reference prints wall times only, src/pguresvt.hpp:168).
  * pgsvt_profile_enable(1) clears and starts collection, (0) stops it.

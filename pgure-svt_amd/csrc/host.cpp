@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <fstream>
 #include <future>
@@ -151,7 +152,76 @@ struct HostTimer {
   }
 };
 
-// device buffer owning its allocation
+// Device allocations are cached per device across calls: a released
+// buffer goes back to a free list and a later request of at least its size
+// (and at most 5/4 of it) takes it again, so repeated calls of one shape
+// allocate and free nothing.  pgsvt_release_cache() frees the list; a
+// failing hipMalloc empties it and retries once.
+class DeviceCache {
+ public:
+  static DeviceCache& get() {
+    static DeviceCache* c = new DeviceCache;  // never destroyed: outlives every caller
+    return *c;
+  }
+  void* take(size_t bytes) {
+    int dev = 0;
+    hipchk(hipGetDevice(&dev), "hipGetDevice");
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      auto& fl = free_[dev];
+      auto it = fl.lower_bound(bytes);
+      if (it != fl.end() && it->first <= bytes + bytes / 4) {
+        void* p = it->second;
+        held_[dev] -= it->first;
+        size_of_[p] = it->first;
+        fl.erase(it);
+        return p;
+      }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      release(dev);
+      hipchk(hipMalloc(&p, bytes), "hipMalloc");
+    }
+    std::lock_guard<std::mutex> g(mu_);
+    size_of_[p] = bytes;
+    return p;
+  }
+  void give(void* p) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = size_of_.find(p);
+    if (it == size_of_.end()) return;
+    free_[dev].emplace(it->second, p);
+    held_[dev] += it->second;
+    size_of_.erase(it);
+  }
+  // free every cached block of device `dev`
+  void release(int dev) {
+    std::multimap<size_t, void*> fl;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      fl.swap(free_[dev]);
+      held_[dev] = 0;
+    }
+    for (auto& kv : fl) (void)hipFree(kv.second);
+  }
+  size_t held(int dev) {
+    std::lock_guard<std::mutex> g(mu_);
+    return held_[dev];
+  }
+
+ private:
+  std::mutex mu_;
+  std::map<int, std::multimap<size_t, void*>> free_;
+  std::map<int, size_t> held_;
+  std::map<void*, size_t> size_of_;
+};
+
+// device buffer owning its (cached) allocation
 template <class T>
 struct DBuf {
   T* p = nullptr;
@@ -163,11 +233,17 @@ struct DBuf {
   void alloc(size_t count) {
     release();
     if (count == 0) return;
-    hipchk(hipMalloc((void**)&p, count * sizeof(T)), "hipMalloc");
+    p = (T*)DeviceCache::get().take(count * sizeof(T));
     n = count;
   }
+  // Normal paths release a buffer only after the stream that used it was
+  // synchronised; while an exception unwinds, kernels may still be queued,
+  // so the device is drained before the block becomes reusable.
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (std::uncaught_exceptions() > 0) (void)hipDeviceSynchronize();
+      DeviceCache::get().give(p);
+    }
     p = nullptr;
     n = 0;
   }
@@ -421,6 +497,8 @@ struct Pipeline {
   DBuf<uint32_t> stats;
 
   ~Pipeline() {
+    // drain the stream before the member buffers go back to the cache
+    if (s) (void)hipStreamSynchronize(s);
     prof.reset();
     if (s) (void)hipStreamDestroy(s);
   }
@@ -531,6 +609,9 @@ struct Pipeline {
   void alloc_batch(uint32_t want) {
     size_t freeb = 0, total = 0;
     hipchk(hipMemGetInfo(&freeb, &total), "hipMemGetInfo");
+    int dev = 0;
+    hipchk(hipGetDevice(&dev), "hipGetDevice");
+    freeb += DeviceCache::get().held(dev);  // cached blocks are ours to reuse
     const size_t budget = std::min<size_t>((size_t)(freeb * 0.6), 64ull << 30);
     const size_t per = bytes_per_window();
     NB = (uint32_t)std::max<size_t>(1, std::min<size_t>(want, budget / per));
@@ -929,6 +1010,16 @@ void pgsvt_default_params(pgsvt_params* p) {
 const char* pgsvt_last_error(void) { return g_err.c_str(); }
 
 const char* pgsvt_version(void) { return "pgsvt 0.1.0 (gfx950)"; }
+
+int pgsvt_release_cache(void) {
+  return guarded([&] {
+    int dev = 0;
+    hipchk(hipGetDevice(&dev), "hipGetDevice");
+    hipchk(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    DeviceCache::get().release(dev);
+    return 0;
+  });
+}
 
 int pgsvt_denoise(const void* X, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
                   const pgsvt_params* p, double* Y, double* estimates) {
