@@ -750,10 +750,30 @@ __global__ __launch_bounds__(256) void k_eval(
   const float* F = factors + (((size_t)b * nvar_stored + (valid ? var : 0)) * p_sel + (valid ? p : 0)) * fstride;
   const float* FW = F + N * SR;
   const float* FS = FW + N * M;
+  // Loads that do not depend on the reconstruction go out first: the
+  // singular values, then (PGURE) the bucket places and trajectory of the
+  // frames this group finishes (var, var + 3, ...).
+  constexpr int KP = (N + 2) / 3;
+  const bool pg = nvar_run == 3 && row;
+  float sl = 0.f, smax = 0.f;
+  if (valid && lane < N) {
+    sl = FS[lane];
+    smax = FS[0];
+  }
+  const uint32_t it = pg ? sel[p] : 0;
+  int32_t ib[KP], pk[KP];
+  if (pg) {
+    const int32_t* I = inv + (size_t)b * nw * p_sel + p;
+#pragma unroll
+    for (int i = 0; i < KP; ++i) {
+      const int k = min(var + 3 * i, N - 1);
+      ib[i] = I[(size_t)k * p_sel];
+      pk[i] = pos[((size_t)b * nw + k) * p_all + it];
+    }
+  }
   // shrink factor g(s_r)/s_r in lane r < N; s_0 is the largest
   float dl = 0.f;
   if (valid && lane < N) {
-    const float sl = FS[lane], smax = FS[0];
     const float w = exp_w ? fabsf(smax * expf((float)(-0.5 * lam) * (sl * sl))) : (float)lam;
     dl = sl > 0.f ? fmaxf(sl - w, 0.f) / sl : 0.f;
   }
@@ -764,13 +784,25 @@ __global__ __launch_bounds__(256) void k_eval(
   float bq[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) bq[k] = 0.f;
-  for (int r = 0; r < nkeep; ++r) {
-    const float d = __shfl(dl, r, L);
-    const float y = row ? d * FW[r * M + lane] : 0.f;
+  // rank loop, loads of rank r + 1 issued before the products of rank r
+  float wn = 0.f;
+  float4 vn[SR / 4];
+  auto fetch = [&](int r) {
+    wn = row ? FW[r * M + lane] : 0.f;
     const float4* vr = reinterpret_cast<const float4*>(F + r * SR);
 #pragma unroll
+    for (int j = 0; j < SR / 4; ++j) vn[j] = vr[j];
+  };
+  if (nkeep > 0) fetch(0);
+  for (int r = 0; r < nkeep; ++r) {
+    const float y = __shfl(dl, r, L) * wn;
+    float4 vc[SR / 4];
+#pragma unroll
+    for (int j = 0; j < SR / 4; ++j) vc[j] = vn[j];
+    if (r + 1 < nkeep) fetch(r + 1);
+#pragma unroll
     for (int j = 0; j < SR / 4; ++j) {
-      const float4 t = vr[j];
+      const float4 t = vc[j];
       if (4 * j < N) bq[4 * j] = fmaf(y, t.x, bq[4 * j]);
       if (4 * j + 1 < N) bq[4 * j + 1] = fmaf(y, t.y, bq[4 * j + 1]);
       if (4 * j + 2 < N) bq[4 * j + 2] = fmaf(y, t.z, bq[4 * j + 2]);
@@ -782,28 +814,30 @@ __global__ __launch_bounds__(256) void k_eval(
     // The three groups of a patch share the per-frame work: group `var`
     // takes frames var, var + 3, ...: the U row goes to its bucket-sorted
     // place in B0 and (B1 - 2 B0 + B2) delta2_v / W_v is accumulated.
-    if (row) {
+    double dwv[KP];
+    const int rr = lane % BS, cc = lane / BS;
+    if (pg) {
+      const double* DW = dw + (size_t)b * vol;
+#pragma unroll
+      for (int i = 0; i < KP; ++i) {
+        const int k = min(var + 3 * i, N - 1);
+        const uint32_t r0 = min((uint32_t)(pk[i] & 0xffff), lim), c0 = min((uint32_t)(pk[i] >> 16), lim);
+        dwv[i] = DW[(uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc)];
+      }
 #pragma unroll
       for (int k = 0; k < N; ++k) xch[pl][var][lane][k] = bq[k];
     }
     __syncthreads();
-    if (row) {
-      const int32_t* I = inv + (size_t)b * nw * p_sel + p;
-      const uint32_t it = sel[p];
-      const int rr = lane % BS, cc = lane / BS;
-      const double* DW = dw + (size_t)b * vol;
+    if (pg) {
 #pragma unroll
-      for (int i = 0; i < (N + 2) / 3; ++i) {
+      for (int i = 0; i < KP; ++i) {
         const int k = var + 3 * i;
         if (k < N) {
           const float x0 = xch[pl][0][lane][k];
-          B0[(((size_t)slot * nw + k) * p_sel + I[(size_t)k * p_sel]) * M + lane] = x0;
-          const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
-          const uint32_t r0 = min((uint32_t)(pk & 0xffff), lim), c0 = min((uint32_t)(pk >> 16), lim);
-          const uint32_t v = (uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc);
+          B0[(((size_t)slot * nw + k) * p_sel + ib[i]) * M + lane] = x0;
           const double c2 =
               ((double)xch[pl][1][lane][k] + (double)xch[pl][2][lane][k]) - 2.0 * (double)x0;
-          s4 = fma(c2, DW[v], s4);
+          s4 = fma(c2, dwv[i], s4);
         }
       }
     }
@@ -935,14 +969,17 @@ __global__ __launch_bounds__(256) void k_gather_direct(const float* __restrict__
         ((sm[threadIdx.x][0] + sm[threadIdx.x][1]) + sm[threadIdx.x][2]) + sm[threadIdx.x][3];
 }
 
-// Tiled gather for patches up to 4 x 4, same sums in the same order as
-// k_gather_direct (so the results are identical).  A block covers a tile of
-// TR x TC voxels of one frame; the patches that can cover the tile sit in
-// one contiguous bucket range per patch column, which the block copies into
-// LDS with coalesced loads (one plane per patch row, so lanes on
-// neighbouring voxels read neighbouring words); then one thread per voxel
-// sums from LDS.  A tile whose ranges hold more than ROWS patches (many
-// trajectories meeting) reads them from global memory instead.
+// Tiled gather for patches up to 4 x 4.  A block covers a tile of TR x TC
+// voxels of one frame; the patches that can cover the tile sit in one
+// contiguous bucket range per patch column, which the block copies into LDS
+// with coalesced float4 loads: patch column lc gets CAP rows, stored as one
+// plane per patch row (lanes on neighbouring voxels read neighbouring
+// words).  Each thread then sums its voxel: first the leading patch of each
+// of its bs x bs buckets (one LDS read and a select, no branch), then, only
+// in waves where a bucket holds several patches, the remaining ones.  The
+// order is fixed, so the sums are reproducible.  A tile with a column range
+// longer than CAP patches (trajectories piling up) reads global memory in
+// the order of k_gather_direct instead.
 template <int BS>
 __global__ __launch_bounds__(256) void k_gather_tile(const float* __restrict__ B0, const int32_t* __restrict__ off,
                                                      const double* __restrict__ invw,
@@ -952,10 +989,11 @@ __global__ __launch_bounds__(256) void k_gather_tile(const float* __restrict__ B
                                                      uint32_t nw, uint32_t vol, uint32_t n1, uint32_t p_sel,
                                                      int mode, double* __restrict__ out,
                                                      double* __restrict__ partial) {
-  constexpr int M = BS * BS, TR = 32, TC = 8, NC = TC + BS - 1, NRW = TR + BS, ROWS = 576, RS = ROWS + 1;
+  constexpr int M = BS * BS, TR = 32, TC = 8, NC = TC + BS - 1, NRW = TR + BS, CAP = 48;
+  constexpr int RS = NC * CAP + 1;           // plane stride (odd: spreads the planes over banks)
+  constexpr int W = (M % 4 == 0) ? 4 : 1;    // floats per staging load
   __shared__ float sB[M * RS];
   __shared__ int32_t sO[NC][NRW];  // bucket offsets of the tile's positions (+ one past the last row)
-  __shared__ int32_t sl[NC + 1];   // first LDS row of each patch column
   __shared__ double sm[5][4];
   uint32_t bx, slot;
   xcd_block(bx, slot);
@@ -969,6 +1007,10 @@ __global__ __launch_bounds__(256) void k_gather_tile(const float* __restrict__ B
   const bool live = r < (int)nr && c < (int)nr;
   const int32_t* O = off + ((size_t)b * nw + k) * ((size_t)n1 * n1 + 1);
   const float* Bk = B0 + ((size_t)slot * nw + k) * p_sel * M;
+  // the voxel's 1/W and U are independent of the staging: load them first
+  const uint32_t pix = (uint32_t)min(r, (int)nr - 1) + nr * (uint32_t)min(c, (int)nr - 1), v = k * fs + pix;
+  const double iw = invw[(size_t)b * vol + v];
+  const double U = (mode == 0) ? u64[(size_t)b * vol + v] : 0.0;
   // positions that can cover the tile: rows prlo..prhi, columns pclo..pchi
   const int prlo = max(r0 - (BS - 1), 0), prhi = min(r0 + TR - 1, (int)n1 - 1);
   const int pclo = max(c0 - (BS - 1), 0), pchi = min(c0 + TC - 1, (int)n1 - 1);
@@ -978,32 +1020,24 @@ __global__ __launch_bounds__(256) void k_gather_tile(const float* __restrict__ B
     sO[lc][i] = O[(size_t)n1 * (pclo + lc) + prlo + i];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int32_t t = 0;
-    for (int lc = 0; lc < ncol; ++lc) {
-      sl[lc] = t;
-      t += sO[lc][nrow - 1] - sO[lc][0];
-    }
-    sl[ncol] = t;
-  }
-  __syncthreads();
-  const int32_t tot = sl[ncol];
-  const bool staged = tot <= ROWS;
-  if (staged) {
-    constexpr int W = (M % 4 == 0) ? 4 : 1;  // floats per load
-    for (int e = threadIdx.x; e < tot * (M / W); e += blockDim.x) {
-      const int lr = e / (M / W), q = (e % (M / W)) * W;
-      int lc = 0;
-      while (lc + 1 < ncol && lr >= sl[lc + 1]) ++lc;
-      const int32_t j = sO[lc][0] + (lr - sl[lc]);
-      if constexpr (W == 4) {
-        const float4 v = *reinterpret_cast<const float4*>(Bk + (size_t)j * M + q);
-        sB[(q + 0) * RS + lr] = v.x;
-        sB[(q + 1) * RS + lr] = v.y;
-        sB[(q + 2) * RS + lr] = v.z;
-        sB[(q + 3) * RS + lr] = v.w;
-      } else {
-        sB[q * RS + lr] = Bk[(size_t)j * M + q];
+  bool fits = true;
+  for (int lc = 0; lc < ncol; ++lc) fits = fits && (sO[lc][nrow - 1] - sO[lc][0] <= CAP);
+  if (fits) {
+    constexpr int PER = CAP * (M / W);  // staging elements per patch column
+    for (int e = threadIdx.x; e < ncol * PER; e += blockDim.x) {
+      const int lc = e / PER, rem = e % PER, lr = rem / (M / W), q = (rem % (M / W)) * W;
+      const int32_t j0 = sO[lc][0];
+      if (lr < sO[lc][nrow - 1] - j0) {
+        const int dst = lc * CAP + lr;
+        if constexpr (W == 4) {
+          const float4 v = *reinterpret_cast<const float4*>(Bk + (size_t)(j0 + lr) * M + q);
+          sB[(q + 0) * RS + dst] = v.x;
+          sB[(q + 1) * RS + dst] = v.y;
+          sB[(q + 2) * RS + dst] = v.z;
+          sB[(q + 3) * RS + dst] = v.w;
+        } else {
+          sB[q * RS + dst] = Bk[(size_t)(j0 + lr) * M + q];
+        }
       }
     }
   }
@@ -1011,33 +1045,59 @@ __global__ __launch_bounds__(256) void k_gather_tile(const float* __restrict__ B
   double a_q = 0, a_h = 0, a_hu = 0, a_u = 0, a_u2 = 0;
   if (live) {
     double acc = 0.0;
+    // bucket bounds bnd[cb][e] of positions (r - BS + 1 + e, c - cb),
+    // clamped to the frame; e = BS is the end of row r
+    int32_t bnd[BS][BS + 1];
+    int lcs[BS];
 #pragma unroll
     for (int cb = 0; cb < BS; ++cb) {
       const int pc = c - cb;
-      if (pc < 0 || pc >= (int)n1) continue;
-      const int lc = pc - pclo;
-      const int32_t base = sl[lc] - sO[lc][0];  // bucket index j -> LDS row j + base
-      int32_t bnd[BS + 1];
+      const bool in = pc >= 0 && pc < (int)n1;
+      lcs[cb] = in ? pc - pclo : 0;
 #pragma unroll
       for (int e = 0; e <= BS; ++e) {
         const int prc = min(max(r - (BS - 1) + e, 0), (int)n1);
-        bnd[e] = sO[lc][prc - prlo];
-      }
-#pragma unroll
-      for (int ra = 0; ra < BS; ++ra) {
-        const int e = BS - 1 - ra;  // position r - ra
-        const int q = ra + BS * cb;
-        if (staged) {
-          for (int32_t j = bnd[e]; j < bnd[e + 1]; ++j) acc += (double)sB[q * RS + j + base];
-        } else {
-          for (int32_t j = bnd[e]; j < bnd[e + 1]; ++j) acc += (double)Bk[(size_t)j * M + q];
-        }
+        bnd[cb][e] = in ? sO[lcs[cb]][prc - prlo] : 0;
       }
     }
-    const uint32_t pix = (uint32_t)r + nr * (uint32_t)c, v = k * fs + pix;
-    const double uh = acc * invw[(size_t)b * vol + v];
+    if (fits) {
+      bool multi = false;
+#pragma unroll
+      for (int cb = 0; cb < BS; ++cb) {
+        const int base = lcs[cb] * CAP - sO[lcs[cb]][0];  // bucket index j -> LDS row j + base
+#pragma unroll
+        for (int ra = 0; ra < BS; ++ra) {
+          const int e = BS - 1 - ra;  // position r - ra
+          const int32_t j0 = bnd[cb][e], n = bnd[cb][e + 1] - j0;
+          const int row = min(max(j0 + base, 0), NC * CAP - 1);  // in bounds even for empty buckets
+          const float f = sB[(ra + BS * cb) * RS + row];
+          acc += (n > 0) ? (double)f : 0.0;
+          multi = multi || n > 1;
+        }
+      }
+      if (__any(multi)) {
+#pragma unroll
+        for (int cb = 0; cb < BS; ++cb) {
+          const int base = lcs[cb] * CAP - sO[lcs[cb]][0];
+#pragma unroll
+          for (int ra = 0; ra < BS; ++ra) {
+            const int e = BS - 1 - ra;
+            for (int32_t j = bnd[cb][e] + 1; j < bnd[cb][e + 1]; ++j)
+              acc += (double)sB[(ra + BS * cb) * RS + j + base];
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int cb = 0; cb < BS; ++cb)
+#pragma unroll
+        for (int ra = 0; ra < BS; ++ra) {
+          const int e = BS - 1 - ra;
+          for (int32_t j = bnd[cb][e]; j < bnd[cb][e + 1]; ++j) acc += (double)Bk[(size_t)j * M + ra + BS * cb];
+        }
+    }
+    const double uh = acc * iw;
     if (mode == 0) {
-      const double U = u64[(size_t)b * vol + v];
       a_q = uh * uh;
       a_h = uh;
       a_hu = uh * U;
