@@ -164,6 +164,30 @@ int pgsvt_noise_estimate(const double* u, uint32_t n_rows, uint32_t n_cols,
  * (-sqrt(vQ/vP)), 0 where it is +sqrt(vP/vQ); n_vox values. */
 int pgsvt_delta2_signs(int64_t seed, uint64_t n_vox, uint8_t* out);
 
+/* ------------------------------------------------------------------------
+ * Front-end support (the reference's command-line program, src/PGURE-SVT.cpp)
+ * ------------------------------------------------------------------------ */
+
+/* HotPixelFilter (src/hotpixel.hpp:19-64) on a host uint16 cube
+ * n_rows x n_cols x n_frames, in place, on the GPU: pixels further than
+ * threshold * MAD from the frame median are replaced by the median of their
+ * 8 neighbours (border pixels by the frame median), in column-major order.
+ * Deviation: |x - median| is the exact distance (the reference's uint16
+ * arithmetic wraps below the median). */
+int pgsvt_hotpixel_filter(uint16_t* seq, uint32_t n_rows, uint32_t n_cols,
+                          uint32_t n_frames, double threshold);
+
+/* Multi-page grayscale TIFF (uncompressed, 8/16-bit) as the front end
+ * reads and writes it through libtiff (src/PGURE-SVT.cpp:118-174,
+ * 204-240).  Cubes are height x width x pages, column-major, element
+ * (r, c, k) = row r, column c of page k. */
+int pgsvt_tiff_info(const char* path, uint32_t* width, uint32_t* height,
+                    uint32_t* bits, uint32_t* pages);
+int pgsvt_tiff_read_u16(const char* path, uint32_t first, uint32_t count,
+                        uint16_t* out);
+int pgsvt_tiff_write_u16(const char* path, const uint16_t* cube,
+                         uint32_t width, uint32_t height, uint32_t pages);
+
 #ifdef __cplusplus
 }
 #endif

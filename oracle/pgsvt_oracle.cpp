@@ -1195,6 +1195,53 @@ static std::unique_ptr<std::vector<double>> maybe_median(const void* X, int dtyp
 // ================================================================ C API ====
 using namespace orc;
 
+// ------------------------------------------ hot-pixel filter (front end) --
+// HotPixelFilter (src/hotpixel.hpp:19-64) on uint16 frames, in place.
+// Armadillo's median of integers: odd n -> element n/2 of the sorted values;
+// even n -> upper + (lower - upper) / 2 in int arithmetic (rounded towards
+// the upper middle).  The frame median is the median of the column medians
+// (arma::median applied twice), likewise the MAD of |x - median| (times
+// 1 / 0.6745).  Stated deviation from the reference: |x - median| is the
+// exact distance; the reference's uint16 subtraction wraps below the median.
+static uint32_t int_median(std::vector<uint32_t> v) {
+  const size_t n = v.size(), h = n / 2;
+  std::sort(v.begin(), v.end());
+  if (n % 2) return v[h];
+  return (uint32_t)((int)v[h] + ((int)v[h - 1] - (int)v[h]) / 2);
+}
+
+static void hot_pixel_frame(uint16_t* F, uint32_t nr, uint32_t nc, double threshold) {
+  std::vector<uint32_t> col(nr), meds(nc);
+  for (uint32_t c = 0; c < nc; ++c) {
+    for (uint32_t r = 0; r < nr; ++r) col[r] = F[r + (size_t)nr * c];
+    meds[c] = int_median(col);
+  }
+  const uint32_t med = int_median(meds);
+  auto dist = [&](uint32_t x) { return x > med ? x - med : med - x; };
+  for (uint32_t c = 0; c < nc; ++c) {
+    for (uint32_t r = 0; r < nr; ++r) col[r] = dist(F[r + (size_t)nr * c]);
+    meds[c] = int_median(col);
+  }
+  const double mad = (double)int_median(meds) * (1.0 / 0.6745);
+  std::vector<size_t> outliers;  // found before any replacement (arma::find)
+  for (size_t i = 0; i < (size_t)nr * nc; ++i)
+    if ((double)dist(F[i]) > threshold * mad) outliers.push_back(i);
+  for (size_t i : outliers) {  // replaced in index order, in place
+    const uint32_t r = (uint32_t)(i % nr), c = (uint32_t)(i / nr);
+    if (r > 0 && r < nr - 1 && c > 0 && c < nc - 1) {
+      double w[8];
+      int q = 0;
+      for (int dc = -1; dc <= 1; ++dc)  // the 8 neighbours (their order does not matter: sorted next)
+        for (int dr = -1; dr <= 1; ++dr)
+          if (dr || dc) w[q++] = (double)F[(r + dr) + (size_t)nr * (c + dc)];
+      std::sort(w, w + 8);
+      F[i] = (uint16_t)(0.5 * (w[3] + w[4]));
+    } else {
+      F[i] = (uint16_t)med;
+    }
+  }
+}
+
 extern "C" {
 
 const char* oracle_last_error(void) { return g_err.c_str(); }
@@ -1358,6 +1405,11 @@ int oracle_sbplx(double (*f)(double, void*), void* ctx, double start, double lb,
   *x_last = last;
   *nevals = st.nevals;
   return rc;
+}
+
+int oracle_hotpixel(uint16_t* seq, uint32_t nr, uint32_t nc, uint32_t nf, double threshold) {
+  for (uint32_t k = 0; k < nf; ++k) hot_pixel_frame(seq + (size_t)k * nr * nc, nr, nc, threshold);
+  return 0;
 }
 
 }  // extern "C"

@@ -3,7 +3,8 @@
     python pgure-svt_amd/build.py [--jobs N]
 
 Objects go to pgure-svt_amd/_build/, the library to
-pgure-svt_amd/pguresvt/libpgsvt.so where the Python package loads it.
+pgure-svt_amd/pguresvt/libpgsvt.so where the Python package loads it, and
+the command-line front end to pgure-svt_amd/pgure-svt.
 """
 
 import argparse
@@ -17,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "pguresvt", "libpgsvt.so")
+CLI = os.path.join(HERE, "pgure-svt")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + os.path.join(HERE, "..", "include")]
@@ -27,8 +29,10 @@ HOST = ["-x", "c++", "-pthread"]
 
 SOURCES = [
     ("kernels.hip", DEVICE),
+    ("hotpixel.hip", DEVICE),
     ("host.cpp", DEVICE),
     ("noise.cpp", HOST),
+    ("tiff.cpp", HOST),
 ]
 
 
@@ -58,6 +62,14 @@ def build(jobs=4):
         sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
         raise RuntimeError("link failed")
     shutil.move(tmp, LIB)
+    # command-line front end, linked against the library next to it
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-x", "c++", os.path.join(CSRC, "cli.cpp"), "-x", "none",
+           "-o", CLI + ".tmp", "-L" + os.path.dirname(LIB), "-lpgsvt", "-Wl,-rpath,$ORIGIN/pguresvt"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError("cli build failed")
+    shutil.move(CLI + ".tmp", CLI)
     return LIB
 
 

@@ -41,6 +41,7 @@
 #include "kernels.hpp"
 #include "noise.hpp"
 #include "sbplx.hpp"
+#include "tiff.hpp"
 
 namespace pgsvt {
 namespace {
@@ -1147,6 +1148,74 @@ int pgsvt_noise_estimate(const double* u, uint32_t n_rows, uint32_t n_cols, uint
   if (n_rows != n_cols) return fail(PGSVT_E_ARG, "quadtree noise estimation requires square frames");
   return guarded([&] {
     estimate_noise(u, n_rows, n_t, method, *alpha, *mu, *sigma);
+    return 0;
+  });
+}
+
+int pgsvt_hotpixel_filter(uint16_t* seq, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames, double threshold) {
+  return guarded([&] {
+    if (!seq) return fail(PGSVT_E_ARG, "null sequence");
+    if (n_rows < 1 || n_cols < 1 || n_frames < 1) return fail(PGSVT_E_ARG, "empty sequence");
+    if (!(threshold >= 0.0)) return fail(PGSVT_E_ARG, "hot-pixel threshold must be >= 0");
+    if (int rc = check_device()) return rc;
+    hipStream_t s = nullptr;
+    hipchk(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+      }
+    } guard{s};
+    const size_t n = (size_t)n_rows * n_cols * n_frames;
+    DBuf<uint16_t> d;
+    DBuf<uint32_t> cm;
+    d.alloc(n);
+    cm.alloc((size_t)n_frames * n_cols);
+    hipchk(hipMemcpyAsync(d.p, seq, n * 2, hipMemcpyHostToDevice, s), "H2D seq");
+    hipchk(launch_hotpixel(d.p, n_rows, n_cols, n_frames, threshold, cm.p, s), "hotpixel");
+    hipchk(hipMemcpyAsync(seq, d.p, n * 2, hipMemcpyDeviceToHost, s), "D2H seq");
+    hipchk(hipStreamSynchronize(s), "sync");
+    return 0;
+  });
+}
+
+int pgsvt_tiff_info(const char* path, uint32_t* width, uint32_t* height, uint32_t* bits, uint32_t* pages) {
+  return guarded([&] {
+    if (!path || !width || !height || !bits || !pages) return fail(PGSVT_E_ARG, "null argument");
+    try {
+      const TiffInfo i = tiff_info(path);
+      *width = i.width;
+      *height = i.height;
+      *bits = i.bits;
+      *pages = i.pages;
+    } catch (const std::runtime_error& e) {  // unreadable or unsupported file
+      return fail(PGSVT_E_ARG, e.what());
+    }
+    return 0;
+  });
+}
+
+int pgsvt_tiff_read_u16(const char* path, uint32_t first, uint32_t count, uint16_t* out) {
+  return guarded([&] {
+    if (!path || !out) return fail(PGSVT_E_ARG, "null argument");
+    try {
+      tiff_read_u16(path, first, count, out);
+    } catch (const std::runtime_error& e) {
+      return fail(PGSVT_E_ARG, e.what());
+    }
+    return 0;
+  });
+}
+
+int pgsvt_tiff_write_u16(const char* path, const uint16_t* cube, uint32_t width, uint32_t height, uint32_t pages) {
+  return guarded([&] {
+    if (!path || !cube) return fail(PGSVT_E_ARG, "null argument");
+    try {
+      tiff_write_u16(path, cube, width, height, pages);
+    } catch (const std::runtime_error& e) {
+      return fail(PGSVT_E_ARG, e.what());
+    }
     return 0;
   });
 }

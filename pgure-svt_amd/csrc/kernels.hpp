@@ -64,4 +64,9 @@ hipError_t launch_gather(const float* B0, const int32_t* off, const double* invw
 hipError_t launch_risk_reduce(const double* pa, uint32_t na_blk, const double* pb, uint32_t nb_blk, uint32_t na,
                               double* sums, hipStream_t s);
 
+// hot-pixel filter of uint16 frames in place (one workgroup per frame);
+// scratch: nf * nc uint32
+hipError_t launch_hotpixel(uint16_t* seq, uint32_t nr, uint32_t nc, uint32_t nf, double threshold,
+                           uint32_t* scratch, hipStream_t s);
+
 }  // namespace pgsvt

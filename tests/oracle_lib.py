@@ -79,6 +79,7 @@ def lib():
         L.oracle_noise_estimate.argtypes = [vp, u32, u32, u32, u32, dp, dp, dp]
         L.oracle_pcg64.argtypes = [u64, u64, vp]
         L.oracle_median_filter.argtypes = [vp, C.c_int, u32, u32, u32, C.c_int, vp]
+        L.oracle_hotpixel.argtypes = [vp, u32, u32, u32, C.c_double]
         L.oracle_perturbations.argtypes = [i64, u64, vp, vp]
         L.oracle_patch_indices.argtypes = [u32, u32, u32, vp]
         L.oracle_patch_indices.restype = u32
@@ -217,3 +218,11 @@ def median_filter(X, r):
     Z = np.zeros((nr, nc, nf), order="F")
     _check(lib().oracle_median_filter(X.ctypes.data, dt, nr, nc, nf, r, Z.ctypes.data))
     return Z
+
+
+def hotpixel(seq, threshold):
+    """HotPixelFilter of a uint16 cube (nr, nc, nf), returns a filtered copy."""
+    out = np.asfortranarray(seq.astype(np.uint16, copy=True))
+    nr, nc, nf = out.shape
+    _check(lib().oracle_hotpixel(out.ctypes.data, nr, nc, nf, threshold))
+    return out
