@@ -30,6 +30,7 @@ HOST = ["-x", "c++", "-pthread"]
 SOURCES = [
     ("kernels.hip", DEVICE),
     ("hotpixel.hip", DEVICE),
+    ("mlp.hip", DEVICE),
     ("host.cpp", DEVICE),
     ("noise.cpp", HOST),
     ("tiff.cpp", HOST),

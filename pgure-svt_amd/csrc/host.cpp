@@ -773,6 +773,87 @@ uint32_t thread_count(int64_t n_jobs) {
   return n;
 }
 
+// ------------------------------------------------------- learned lambda --
+// The proposal network n_in -> h -> h -> h -> 1 (GELU between layers).
+// Parameters, flat FP32 in this order: W1 [n_in][h], b1 [h], W2 [h][h],
+// b2 [h], W3 [h][h], b3 [h], W4 [h][1], b4 [1].  Every product runs on the
+// BF16 matrix cores with FP32 sums (launch_gemm).
+size_t mlp_param_count(uint32_t n_in, uint32_t h) {
+  return (size_t)n_in * h + h + 2 * ((size_t)h * h + h) + h + 1;
+}
+
+class Mlp {
+ public:
+  Mlp(uint32_t n_in, uint32_t h, hipStream_t s) : n_in_(n_in), h_(h), s_(s) {
+    P_.alloc(mlp_param_count(n_in, h));
+    G_.alloc(P_.n);
+  }
+  size_t n_params() const { return P_.n; }
+  float* params() { return P_.p; }
+  float* grads() { return G_.p; }
+
+  // forward of B rows of X ([B][n_in], device) into out ([B], device);
+  // activations are kept for backward
+  void forward(const float* X, uint32_t B, float* out) {
+    ensure(B);
+    const uint32_t h = h_;
+    const float *W1 = P_.p, *b1 = W1 + (size_t)n_in_ * h, *W2 = b1 + h, *b2 = W2 + (size_t)h * h, *W3 = b2 + h,
+                *b3 = W3 + (size_t)h * h, *W4 = b3 + h, *b4 = W4 + h;
+    hipchk(launch_gemm(X, n_in_, 1, W1, h, 1, a1_.p, h, B, h, n_in_, b1, 1, p1_.p, s_), "mlp l1");
+    hipchk(launch_gemm(a1_.p, h, 1, W2, h, 1, a2_.p, h, B, h, h, b2, 1, p2_.p, s_), "mlp l2");
+    hipchk(launch_gemm(a2_.p, h, 1, W3, h, 1, a3_.p, h, B, h, h, b3, 1, p3_.p, s_), "mlp l3");
+    hipchk(launch_gemm(a3_.p, h, 1, W4, 1, 1, out, 1, B, 1, h, b4, 0, nullptr, s_), "mlp l4");
+  }
+
+  // gradient of the mean squared error of the last forward's output y
+  // against targets t (device, [B]) into grads(); loss into *loss_dev
+  void backward(const float* X, const float* y, const float* t, uint32_t B, float* loss_dev) {
+    const uint32_t h = h_;
+    const float *W2 = P_.p + (size_t)n_in_ * h + h, *W3 = W2 + (size_t)h * h + h, *W4 = W3 + (size_t)h * h + h;
+    float *gW1 = G_.p, *gb1 = gW1 + (size_t)n_in_ * h, *gW2 = gb1 + h, *gb2 = gW2 + (size_t)h * h, *gW3 = gb2 + h,
+          *gb3 = gW3 + (size_t)h * h, *gW4 = gb3 + h, *gb4 = gW4 + h;
+    hipchk(launch_mse(y, 1, t, (int)B, dy_.p, loss_dev, s_), "mse");
+    // layer 4: y = a3 W4 + b4
+    hipchk(launch_gemm(a3_.p, 1, h, dy_.p, 1, 1, gW4, 1, h, 1, B, nullptr, 0, nullptr, s_), "dW4");
+    hipchk(launch_colsum(dy_.p, B, 1, 1, gb4, s_), "db4");
+    hipchk(launch_gemm(dy_.p, 1, 1, W4, 1, 1, da_.p, h, B, h, 1, nullptr, 0, nullptr, s_), "da3");
+    // layers 3, 2, 1: a = gelu(pre), pre = x W + b
+    struct L {
+      const float *x, *W, *pre;
+      float *gW, *gb;
+      uint32_t k;
+    };
+    const L layers[3] = {{a2_.p, W3, p3_.p, gW3, gb3, h}, {a1_.p, W2, p2_.p, gW2, gb2, h}, {X, nullptr, p1_.p, gW1, gb1, n_in_}};
+    for (int l = 0; l < 3; ++l) {
+      const L& c = layers[l];
+      hipchk(launch_gelu_bwd(c.pre, da_.p, dp_.p, (int64_t)B * h, s_), "gelu bwd");
+      hipchk(launch_gemm(c.x, 1, c.k, dp_.p, h, 1, c.gW, h, c.k, h, B, nullptr, 0, nullptr, s_), "dW");
+      hipchk(launch_colsum(dp_.p, B, h, h, c.gb, s_), "db");
+      if (c.W) hipchk(launch_gemm(dp_.p, h, 1, c.W, 1, h, da_.p, h, B, h, h, nullptr, 0, nullptr, s_), "dx");
+    }
+  }
+
+ private:
+  void ensure(uint32_t B) {
+    if (B <= cap_) return;
+    cap_ = B;
+    for (DBuf<float>* b : {&a1_, &a2_, &a3_, &p1_, &p2_, &p3_, &da_, &dp_}) b->alloc((size_t)B * h_);
+    dy_.alloc(B);
+  }
+  uint32_t n_in_, h_, cap_ = 0;
+  hipStream_t s_;
+  DBuf<float> P_, G_, a1_, a2_, a3_, p1_, p2_, p3_, da_, dp_, dy_;
+};
+
+struct StreamOwner {
+  hipStream_t s = nullptr;
+  StreamOwner() { hipchk(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"); }
+  ~StreamOwner() {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
+};
+
 int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, const pgsvt_params* pp, double* Y,
                 double* est, bool on_device) {
   if (int rc = validate(X, dtype, nr, nc, nf, pp)) return rc;
@@ -1216,6 +1297,109 @@ int pgsvt_tiff_write_u16(const char* path, const uint16_t* cube, uint32_t width,
     } catch (const std::runtime_error& e) {
       return fail(PGSVT_E_ARG, e.what());
     }
+    return 0;
+  });
+}
+
+int pgsvt_mlp_forward(const float* params, uint32_t n_in, uint32_t hidden, const float* X, uint32_t n, float* out) {
+  return guarded([&] {
+    if (!params || !X || !out || n_in == 0 || hidden == 0) return fail(PGSVT_E_ARG, "bad network arguments");
+    if (int rc = check_device()) return rc;
+    StreamOwner so;
+    Mlp net(n_in, hidden, so.s);
+    hipchk(hipMemcpyAsync(net.params(), params, net.n_params() * 4, hipMemcpyHostToDevice, so.s), "H2D params");
+    DBuf<float> x, y;
+    x.alloc((size_t)n * n_in);
+    y.alloc(n);
+    hipchk(hipMemcpyAsync(x.p, X, x.bytes(), hipMemcpyHostToDevice, so.s), "H2D X");
+    net.forward(x.p, n, y.p);
+    hipchk(hipMemcpyAsync(out, y.p, y.bytes(), hipMemcpyDeviceToHost, so.s), "D2H y");
+    hipchk(hipStreamSynchronize(so.s), "sync");
+    return 0;
+  });
+}
+
+int pgsvt_mlp_gradient(const float* params, uint32_t n_in, uint32_t hidden, const float* X, const float* t,
+                       uint32_t n, float* grad, float* loss) {
+  return guarded([&] {
+    if (!params || !X || !t || !grad || !loss || n_in == 0 || hidden == 0 || n == 0)
+      return fail(PGSVT_E_ARG, "bad network arguments");
+    if (int rc = check_device()) return rc;
+    StreamOwner so;
+    Mlp net(n_in, hidden, so.s);
+    hipchk(hipMemcpyAsync(net.params(), params, net.n_params() * 4, hipMemcpyHostToDevice, so.s), "H2D params");
+    DBuf<float> x, y, tt, l;
+    x.alloc((size_t)n * n_in);
+    y.alloc(n);
+    tt.alloc(n);
+    l.alloc(1);
+    hipchk(hipMemcpyAsync(x.p, X, x.bytes(), hipMemcpyHostToDevice, so.s), "H2D X");
+    hipchk(hipMemcpyAsync(tt.p, t, tt.bytes(), hipMemcpyHostToDevice, so.s), "H2D t");
+    net.forward(x.p, n, y.p);
+    net.backward(x.p, y.p, tt.p, n, l.p);
+    hipchk(hipMemcpyAsync(grad, net.grads(), net.n_params() * 4, hipMemcpyDeviceToHost, so.s), "D2H grad");
+    hipchk(hipMemcpyAsync(loss, l.p, 4, hipMemcpyDeviceToHost, so.s), "D2H loss");
+    hipchk(hipStreamSynchronize(so.s), "sync");
+    return 0;
+  });
+}
+
+int pgsvt_mlp_train(const float* X, const float* t, uint32_t n, uint32_t n_in, uint32_t hidden, uint32_t epochs,
+                    uint32_t batch, float lr, uint64_t seed, float* params, float* epoch_loss) {
+  return guarded([&] {
+    if (!X || !t || !params || n == 0 || n_in == 0 || hidden == 0 || batch == 0 || !(lr > 0.f))
+      return fail(PGSVT_E_ARG, "bad training arguments");
+    if (int rc = check_device()) return rc;
+    StreamOwner so;
+    hipStream_t s = so.s;
+    Mlp net(n_in, hidden, s);
+    const size_t np = net.n_params();
+    hipchk(hipMemcpyAsync(net.params(), params, np * 4, hipMemcpyHostToDevice, s), "H2D params");
+    DBuf<float> x, tt, xb, tb, y, m, v, l;
+    DBuf<uint32_t> idx;
+    x.alloc((size_t)n * n_in);
+    tt.alloc(n);
+    const uint32_t B = std::min(batch, n);
+    xb.alloc((size_t)B * n_in);
+    tb.alloc(B);
+    y.alloc(B);
+    idx.alloc(n);
+    l.alloc(1);
+    m.alloc(np);
+    v.alloc(np);
+    hipchk(hipMemcpyAsync(x.p, X, x.bytes(), hipMemcpyHostToDevice, s), "H2D X");
+    hipchk(hipMemcpyAsync(tt.p, t, tt.bytes(), hipMemcpyHostToDevice, s), "H2D t");
+    std::vector<float> zero(np, 0.f);
+    hipchk(hipMemcpyAsync(m.p, zero.data(), np * 4, hipMemcpyHostToDevice, s), "H2D m");
+    hipchk(hipMemcpyAsync(v.p, zero.data(), np * 4, hipMemcpyHostToDevice, s), "H2D v");
+    std::mt19937_64 rng(seed);
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::vector<float> losses;
+    int step = 0;
+    for (uint32_t ep = 0; ep < epochs; ++ep) {
+      std::shuffle(order.begin(), order.end(), rng);
+      hipchk(hipMemcpyAsync(idx.p, order.data(), n * 4, hipMemcpyHostToDevice, s), "H2D order");
+      hipchk(hipStreamSynchronize(s), "sync order");  // `order` is reshuffled next epoch
+      double sum = 0;
+      uint32_t nbatch = 0;
+      for (uint32_t b0 = 0; b0 + B <= n; b0 += B) {  // full batches
+        hipchk(launch_gather_rows(x.p, idx.p + b0, B, n_in, xb.p, s), "gather X");
+        hipchk(launch_gather_rows(tt.p, idx.p + b0, B, 1, tb.p, s), "gather t");
+        net.forward(xb.p, B, y.p);
+        net.backward(xb.p, y.p, tb.p, B, l.p);
+        hipchk(launch_adam(net.params(), net.grads(), m.p, v.p, (int64_t)np, lr, 0.9f, 0.999f, 1e-8f, ++step, s),
+               "adam");
+        float lh = 0;
+        hipchk(hipMemcpyAsync(&lh, l.p, 4, hipMemcpyDeviceToHost, s), "D2H loss");
+        hipchk(hipStreamSynchronize(s), "sync loss");
+        sum += lh;
+        ++nbatch;
+      }
+      if (epoch_loss) epoch_loss[ep] = (float)(sum / std::max(1u, nbatch));
+    }
+    hipchk(hipMemcpyAsync(params, net.params(), np * 4, hipMemcpyDeviceToHost, s), "D2H params");
+    hipchk(hipStreamSynchronize(s), "sync");
     return 0;
   });
 }

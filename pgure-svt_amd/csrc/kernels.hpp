@@ -69,4 +69,22 @@ hipError_t launch_risk_reduce(const double* pa, uint32_t na_blk, const double* p
 hipError_t launch_hotpixel(uint16_t* seq, uint32_t nr, uint32_t nc, uint32_t nf, double threshold,
                            uint32_t* scratch, hipStream_t s);
 
+// learned-lambda network (mlp.hip): C(m, n) = act(sum_k A(m, k) B(k, n) +
+// bias[n]) with A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn], bf16
+// operands, FP32 sums; act 1 = GELU (pre-activation kept in pre if set)
+hipError_t launch_gemm(const float* A, int64_t sam, int64_t sak, const float* B, int64_t sbk, int64_t sbn, float* C,
+                       int64_t ldc, int M, int N, int K, const float* bias, int act, float* pre, hipStream_t s);
+hipError_t launch_gelu_bwd(const float* pre, const float* dy, float* dx, int64_t n, hipStream_t s);
+hipError_t launch_colsum(const float* D, int rows, int cols, int64_t ld, float* out, hipStream_t s);
+hipError_t launch_mse(const float* y, int64_t ldy, const float* t, int n, float* dy, float* loss, hipStream_t s);
+hipError_t launch_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                       float eps, int step, hipStream_t s);
+hipError_t launch_gather_rows(const float* X, const uint32_t* idx, uint32_t rows, uint32_t cols, float* out,
+                              hipStream_t s);
+// log singular values of sampled patches: feat[b*ld + i*n + r], standardised
+// with mean / inv_std when given
+hipError_t launch_features(const float* factors, uint32_t nvar_stored, uint32_t p_sel, uint32_t fstride,
+                           uint32_t s_at, uint32_t n, const uint32_t* sample, uint32_t nsamp, uint32_t nb, uint32_t ld,
+                           const float* mean, const float* inv_std, float* feat, hipStream_t s);
+
 }  // namespace pgsvt
