@@ -101,7 +101,9 @@ PARAMS = dict(trajectory_length=TRAJ, patch_size=PATCH, patch_overlap=1, motion_
               noise_sigma=-1.0, tol=1e-7)
 
 
-def run_gpu(noisy, steps, warmup):
+def run_gpu(noisy, steps, warmup, model=None):
+    """Time `steps` calls of the device entry point (PGURE search, or the
+    learned lambda when a model is given) after `warmup` calls."""
     from pguresvt import _pguresvt as P
 
     L = P.load()
@@ -116,9 +118,19 @@ def run_gpu(noisy, steps, warmup):
     hip.h2d(xd, X)
     p = P.make_params(**PARAMS)
     est = np.zeros((FRAMES, 4), order="F")
+    if model is None:
+        def step():
+            P.check(L.pgsvt_denoise_device(xd, 2, N, N, FRAMES, C.byref(p), yd, est.ctypes.data))
+    else:
+        from pguresvt import learned as Lr
 
-    def step():
-        P.check(L.pgsvt_denoise_device(xd, 2, N, N, FRAMES, C.byref(p), yd, est.ctypes.data))
+        L.pgsvt_denoise_learned_device.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                   C.POINTER(P.pgsvt_params), C.POINTER(Lr._Model), C.c_void_p,
+                                                   C.c_void_p]
+        m = model._c()
+
+        def step():
+            P.check(L.pgsvt_denoise_learned_device(xd, 2, N, N, FRAMES, C.byref(p), C.byref(m), yd, est.ctypes.data))
 
     for _ in range(warmup):
         step()
@@ -139,6 +151,37 @@ def run_gpu(noisy, steps, warmup):
     hip.free(xd)
     hip.free(yd)
     return dt, prof, Y, est.copy()
+
+
+def learned_eval(noisy, steps, warmup, dt_plain, est_plain):
+    """The learned-lambda path on the same workload: frames/s, and the PGURE
+    risk gap of each window's proposed lambda against the searched one
+    (both risks evaluated by the library's risk of that window with the
+    search's noise estimates).  Windows whose gap exceeds 0.5 % fall back to
+    the search: counted at the search's per-window time."""
+    from pguresvt import learned as Lr
+
+    path = Lr.default_model_path()
+    if not os.path.exists(path):
+        return None
+    model = Lr.LambdaModel.load(path)
+    dt, prof, _, est = run_gpu(noisy, steps, warmup, model)
+    gaps = []
+    for t in range(FRAMES):
+        r = Lr.window_risk(noisy, t, est_plain[t, 1:], [est_plain[t, 0], est[t, 0]], **PARAMS)
+        gaps.append((r[1] - r[0]) / abs(r[0]))
+    gaps = np.array(gaps)
+    fb = int((gaps > 0.005).sum())
+    per_step_plain = dt_plain
+    per_step_learned = dt / steps
+    with_fb = per_step_learned + fb / FRAMES * per_step_plain
+    return {"value": FRAMES / per_step_learned, "unit": "frames/s", "speedup": per_step_plain / per_step_learned,
+            "risk_gap_mean": float(gaps.mean()), "risk_gap_median": float(np.median(gaps)),
+            "risk_gap_max": float(gaps.max()), "windows": FRAMES, "windows_gap_over_0.5pct": fb,
+            "risk_gap_mean_after_fallback": float(np.where(gaps > 0.005, 0.0, gaps).mean()),
+            "speedup_with_fallback": per_step_plain / with_fb,
+            "lambda_rel_error_median": float(np.median(np.abs(est[:, 0] / est_plain[:, 0] - 1))),
+            "kernels_ms_per_step": {k: v["ms"] / steps for k, v in prof.items()}}
 
 
 def roofline(prof):
@@ -280,6 +323,7 @@ def main():
     ap.add_argument("--cpu-windows", type=int, default=None,
                     help="output frames in the CPU-baseline sample, one thread each (default 2: about 30 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-lambda comparison")
     ap.add_argument("--copy-probe", type=int, metavar="MIB", default=None,
                     help="only run one MIB-MiB elementwise copy (counter calibration) and exit")
     ap.add_argument("--summarize-pmc", nargs=6, default=None,
@@ -301,6 +345,7 @@ def main():
     dt, prof, Y, est = run_gpu(noisy, a.steps, a.warmup)
     fps = FRAMES * a.steps / dt
     roof, _ = roofline(prof)
+    learned = None if a.no_learned else learned_eval(noisy, a.steps, a.warmup, dt / a.steps, est)
     cpu = None
     if not a.no_cpu_baseline:
         cw = a.cpu_windows or max(1, min(2, os.cpu_count() or 1))
@@ -324,6 +369,7 @@ def main():
                    "patches_per_window": (N - PATCH + 1) ** 2},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "learned_lambda": learned,
         "kernels_ms_per_step": {k: v["ms"] / a.steps for k, v in prof.items()},
         "stage_units_per_step": {k: v["units"] / a.steps for k, v in prof.items()},
         "lambda_median": float(np.median(est[:, 0])),

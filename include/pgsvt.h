@@ -92,6 +92,84 @@ int pgsvt_denoise_device(const void* X_dev, int dtype, uint32_t n_rows,
                          const pgsvt_params* p, double* Y_dev,
                          double* estimates);
 
+/* pgsvt_denoise restricted to output frames t_begin .. t_end-1 (the windows
+ * of those frames read the input frames they need from the whole cube).  Y
+ * holds the t_end - t_begin output frames in order, estimates is a
+ * (t_end - t_begin) x 4 column-major matrix.  This is the per-GPU share when
+ * the frames of one sequence are split across GPUs: the driver loop of
+ * src/pguresvt.hpp:90-166 over a sub-range, with identical results. */
+int pgsvt_denoise_range(const void* X, int dtype, uint32_t n_rows,
+                        uint32_t n_cols, uint32_t n_frames,
+                        const pgsvt_params* p, uint32_t t_begin,
+                        uint32_t t_end, double* Y, double* estimates);
+
+/* ------------------------------------------------------------------------
+ * Learned lambda (no reference counterpart: the reference always searches
+ * lambda, src/pgure.hpp:196-237).  A network proposes each window's lambda
+ * from n_in = 256 * Nw + 3 features: log(max(s, 1e-6)) of the sorted
+ * singular values of 256 fixed-sample patches of the window (patch-major),
+ * then the window's noise estimates (alpha, mu, sigma) from leaf statistics
+ * computed once per frame at the sequence maximum and rescaled to the
+ * window's maximum.  Network: n_in -> hidden -> hidden -> hidden -> 1 with
+ * GELU, BF16 matrix cores, FP32 sums.
+ * ------------------------------------------------------------------------ */
+typedef struct pgsvt_lambda_model {
+  uint32_t n_in;         /* feature count, 256 * Nw + 3 */
+  uint32_t hidden;       /* hidden width */
+  const float* mean;     /* [n_in] feature means */
+  const float* inv_std;  /* [n_in] reciprocal feature standard deviations */
+  const float* params;   /* W1 [n_in][hidden], b1, W2 [hidden][hidden], b2,
+                            W3 [hidden][hidden], b3, W4 [hidden][1], b4 */
+  float y_mean, y_std;   /* log(lambda) = y_mean + y_std * network output */
+} pgsvt_lambda_model;
+
+/* The raw (unstandardised) features of output frames t_begin .. t_end-1
+ * into features [t_end - t_begin][256 * Nw + 3]; estimates as in
+ * pgsvt_denoise_range with lambda = 0. */
+int pgsvt_learned_features(const void* X, int dtype, uint32_t n_rows,
+                           uint32_t n_cols, uint32_t n_frames,
+                           const pgsvt_params* p, uint32_t t_begin,
+                           uint32_t t_end, float* features,
+                           double* estimates);
+
+/* pgsvt_denoise with each window's lambda proposed by the model instead of
+ * searched (optimize_pgure and lambda1 are ignored; the proposal is clamped
+ * to the search's bounds [0, max(100, start)]).  estimates holds the
+ * proposed lambda and the noise features' (alpha, mu, sigma). */
+int pgsvt_denoise_learned(const void* X, int dtype, uint32_t n_rows,
+                          uint32_t n_cols, uint32_t n_frames,
+                          const pgsvt_params* p,
+                          const pgsvt_lambda_model* model, double* Y,
+                          double* estimates);
+/* the same with X and Y in device memory */
+int pgsvt_denoise_learned_device(const void* X_dev, int dtype, uint32_t n_rows,
+                                 uint32_t n_cols, uint32_t n_frames,
+                                 const pgsvt_params* p,
+                                 const pgsvt_lambda_model* model,
+                                 double* Y_dev, double* estimates);
+
+/* The network alone, for training and its tests: forward of n rows of X
+ * ([n][n_in], host) into out [n]; gradient of mean((y - t)^2) into grad
+ * (parameter layout) and the loss; Adam training for `epochs` passes of
+ * shuffled mini-batches (params in: initial, out: trained; epoch_loss
+ * [epochs] optional). */
+int pgsvt_mlp_forward(const float* params, uint32_t n_in, uint32_t hidden,
+                      const float* X, uint32_t n, float* out);
+int pgsvt_mlp_gradient(const float* params, uint32_t n_in, uint32_t hidden,
+                       const float* X, const float* t, uint32_t n,
+                       float* grad, float* loss);
+int pgsvt_mlp_train(const float* X, const float* t, uint32_t n,
+                    uint32_t n_in, uint32_t hidden, uint32_t epochs,
+                    uint32_t batch, float lr, uint64_t seed, float* params,
+                    float* epoch_loss);
+
+/* Arithmetic of the SVD input, the Jacobi SVD, the stored factors and the
+ * reconstructions for calls started afterwards, process-wide: 32 (default,
+ * FP32 with FP64 risk sums) or 64 (everything FP64: the GPU precision
+ * reference).  The learned-lambda path runs in 32 only.  No reference
+ * counterpart (the reference is FP64 throughout). */
+int pgsvt_set_precision(int bits);
+
 /* Device memory is cached per device between calls (buffers of one call
  * are reused by the next call of the same shape).  pgsvt_release_cache
  * waits for the current device and frees its cache.  No reference

@@ -31,7 +31,7 @@ struct WinMeta {
   uint32_t t;      // output frame index
   uint32_t s0;     // first input frame of the window
   uint32_t ref;    // reference (output) frame inside the window
-  uint32_t pad;
+  uint32_t out;    // slot of the output frame in the caller's Y
   double uMax;     // max of the raw window (src/pguresvt.hpp:116)
   double wMax;     // max of the motion-estimation window (:117)
   double lambda;   // threshold of the current evaluation

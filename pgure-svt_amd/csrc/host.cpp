@@ -48,6 +48,10 @@ namespace {
 
 thread_local std::string g_err;
 
+// arithmetic of the SVD / reconstruction stages for calls that start from
+// now on: 32 (default) or 64 bits, see pgsvt_set_precision
+std::atomic<int> g_precision{32};
+
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -472,6 +476,8 @@ struct Pipeline {
   Geometry g{};
   Shape sh{};
   bool optimize = false, median = false;
+  bool host_noise = false;  // the noise estimator runs on the host (search or learned lambda)
+  uint32_t tb = 0;          // first output window of the call (output slot = t - tb)
   uint32_t nvar = 1;
   double eps1 = 1e-4, eps2 = 1e-2, d2neg = 0, d2pos = 0;
   uint64_t seed = 0;
@@ -491,7 +497,9 @@ struct Pipeline {
   DBuf<WinMeta> meta_d;
   DBuf<uint32_t> wl_d;
   DBuf<double> u64, w64, invw, dw, partial, partial_b, sums;
-  DBuf<float> u32, factors, B0;
+  // SVD input variants, factors and U reconstructions: FP32, or FP64 for
+  // the precision study (sh.fp64); sized in bytes
+  DBuf<char> uvar, factors, B0;
   DBuf<int32_t> pos, cnt, off, list, inv;
   DBuf<int16_t> jobs_d;
   uint32_t max_jobs = 0, gather_blocks = 0;
@@ -531,6 +539,7 @@ struct Pipeline {
     sh.p_sel = (uint32_t)sel.size();
     const uint32_t sr = (g.Nw + 3) & ~3u;  // padded V^T row
     sh.fstride = (g.Nw * sr + g.Nw * sh.m + g.Nw + 3) & ~3u;
+    sh.fp64 = g_precision.load() == 64;
     const double vP = 0.5 + 0.5 * 1. / std::sqrt(1. * 1. + 4);
     const double vQ = 1 - vP;
     d2neg = -1 * std::sqrt(vQ / vP);
@@ -542,7 +551,7 @@ struct Pipeline {
     const size_t N = (size_t)sh.fs * nf;
     const void* src = X;
     if (on_device) {
-      if (optimize) {  // the noise estimator runs on the host
+      if (optimize || host_noise) {  // the noise estimator runs on the host
         host_copy.resize(N * dtype_size(dtype));
         hipchk(hipMemcpyAsync(host_copy.data(), X, host_copy.size(), hipMemcpyDeviceToHost, s), "D2H X");
         X = host_copy.data();
@@ -581,6 +590,7 @@ struct Pipeline {
   WinMeta meta_of(uint32_t t) const {
     WinMeta m{};
     m.t = t;
+    m.out = t - tb;
     window_of(t, nf, g, m.s0, m.ref);
     double um = fmaxX[m.s0], wm = median ? fmaxZ[m.s0] : 0;
     for (uint32_t k = 1; k < g.Nw; ++k) {
@@ -594,16 +604,17 @@ struct Pipeline {
   }
 
   size_t bytes_per_window() const {
+    const size_t es = sh.fp64 ? 8 : 4;
     size_t b = 0;
     b += 8ull * sh.vol;                                  // u64
     if (median) b += 8ull * sh.vol;                      // w64
-    b += 4ull * sh.vol * nvar;                           // u32
+    b += es * sh.vol * nvar;                             // SVD input variants
     b += 4ull * sh.nw * sh.p_all;                        // pos
     b += 8ull * sh.nw * ((size_t)sh.n1 * sh.n1 + 1);     // bucket counts + offsets
     b += 8ull * sh.nw * sh.p_sel;                        // bucket lists + inverse
     b += 8ull * sh.vol * (optimize ? 2 : 1);             // 1/W, delta2/W
-    b += 4ull * sh.nw * (size_t)sh.p_sel * sh.m;         // B0
-    b += 4ull * nvar * (size_t)sh.p_sel * sh.fstride;    // factors
+    b += es * sh.nw * (size_t)sh.p_sel * sh.m;           // B0
+    b += es * nvar * (size_t)sh.p_sel * sh.fstride;      // factors
     return b;
   }
 
@@ -621,7 +632,8 @@ struct Pipeline {
     wl_d.alloc(NB);
     u64.alloc((size_t)NB * sh.vol);
     if (median) w64.alloc((size_t)NB * sh.vol);
-    u32.alloc((size_t)NB * sh.vol * nvar);
+    const size_t es = sh.fp64 ? 8 : 4;
+    uvar.alloc((size_t)NB * sh.vol * nvar * es);
     pos.alloc((size_t)NB * sh.nw * sh.p_all);
     const size_t npos = (size_t)sh.n1 * sh.n1;
     cnt.alloc((size_t)NB * sh.nw * npos);
@@ -630,8 +642,8 @@ struct Pipeline {
     inv.alloc((size_t)NB * sh.nw * sh.p_sel);
     invw.alloc((size_t)NB * sh.vol);
     if (optimize) dw.alloc((size_t)NB * sh.vol);
-    B0.alloc((size_t)NB * sh.nw * sh.p_sel * sh.m);
-    factors.alloc((size_t)NB * nvar * sh.p_sel * sh.fstride);
+    B0.alloc((size_t)NB * sh.nw * sh.p_sel * sh.m * es);
+    factors.alloc((size_t)NB * nvar * sh.p_sel * sh.fstride * es);
     gather_blocks = pgsvt::gather_blocks(sh, 0);
     partial.alloc((size_t)NB * eval_blocks(sh, 3));
     partial_b.alloc((size_t)NB * gather_blocks * 5);
@@ -647,7 +659,7 @@ struct Pipeline {
     const uint32_t nb = (uint32_t)ms.size();
     hipchk(hipMemcpyAsync(meta_d.p, ms.data(), nb * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
     hipchk(launch_prepare(Xd.p, median ? Zd.p : nullptr, meta_d.p, nb, d2.p, sh, nvar, eps2, d2neg, d2pos, u64.p,
-                          median ? w64.p : nullptr, u32.p, s),
+                          median ? w64.p : nullptr, uvar.p, s),
            "prepare");
     hipchk(hipMemsetAsync(pos.p, 0, (size_t)nb * sh.nw * sh.p_all * sizeof(int32_t), s), "memset pos");
     hipchk(launch_fill_ref(meta_d.p, nb, sh, pos.p, s), "fill_ref");
@@ -673,7 +685,10 @@ struct Pipeline {
     });
     // units: matrices decomposed
     prof->run("svd_jacobi", (double)nb * nvar_run * sh.p_sel, [&] {
-      hipchk(launch_svd(u32.p, pos.p, sel_d.p, nb, nvar_run, sh, factors.p, 30, 1e-6f, stats.p, s), "svd");
+      // rotation threshold |c| > tol sqrt(ab): 1e-6 in FP32, 1e-12 in FP64
+      hipchk(launch_svd(uvar.p, pos.p, sel_d.p, nb, nvar_run, sh, factors.p, 30, sh.fp64 ? 1e-12 : 1e-6, stats.p,
+                        s),
+             "svd");
     });
   }
 
@@ -854,34 +869,91 @@ struct StreamOwner {
   }
 };
 
-int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, const pgsvt_params* pp, double* Y,
-                double* est, bool on_device) {
+// fixed sample of patch indices for the learned-lambda features: nsamp
+// distinct indices of [0, p_sel) (all of them, repeated, when p_sel is
+// smaller), sorted; the same for every window and call
+std::vector<uint32_t> feature_sample(uint32_t p_sel, uint32_t nsamp) {
+  std::vector<uint32_t> idx(p_sel);
+  for (uint32_t i = 0; i < p_sel; ++i) idx[i] = i;
+  std::vector<uint32_t> out(nsamp);
+  if (p_sel >= nsamp) {
+    std::mt19937_64 rng(0x5EEDF00Dull);
+    for (uint32_t i = 0; i < nsamp; ++i) {
+      const uint32_t j = i + (uint32_t)(rng() % (p_sel - i));
+      std::swap(idx[i], idx[j]);
+      out[i] = idx[i];
+    }
+    std::sort(out.begin(), out.end());
+  } else {
+    for (uint32_t i = 0; i < nsamp; ++i) out[i] = i % p_sel;
+  }
+  return out;
+}
+
+constexpr uint32_t kFeatureSamples = 256;
+
+// What one call computes.  mode 0: the reference driver (PGURE search or the
+// fixed lambda); 1: lambda proposed by the learned network; 2: only the
+// learned-lambda features of each window (no output frames).
+struct RunSpec {
+  uint32_t tb = 0, te = 0;  // output windows [tb, te)
+  int mode = 0;
+  const pgsvt_lambda_model* model = nullptr;
+  float* features = nullptr;  // mode 2: [te - tb][n_in]
+};
+
+// A1 driver (src/pguresvt.hpp:90-166) over output windows [rs.tb, rs.te):
+// Y holds their frames in order (Y slot t - tb), est their (lambda, alpha,
+// mu, sigma) as a (te - tb) x 4 column-major matrix.
+int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, const pgsvt_params* pp, double* Y,
+             double* est, bool on_device, const RunSpec& rs) {
   if (int rc = validate(X, dtype, nr, nc, nf, pp)) return rc;
+  if (!(rs.tb < rs.te && rs.te <= nf)) return fail(PGSVT_E_ARG, "window range must satisfy t_begin < t_end <= n_frames");
+  if (!est || (rs.mode != 2 && !Y) || (rs.mode == 2 && !rs.features)) return fail(PGSVT_E_ARG, "null output pointer");
+  if (rs.mode == 1 && (!rs.model || !rs.model->params || !rs.model->mean || !rs.model->inv_std))
+    return fail(PGSVT_E_ARG, "learned lambda needs a model");
   if (int rc = check_device()) return rc;
   HostTimer total_timer("host_total_call");
+  const bool learned = rs.mode != 0;
+  const uint32_t count = rs.te - rs.tb;
+  pgsvt_params pl = *pp;
+  if (learned) pl.optimize_pgure = 0;  // no perturbed decompositions, no search
   Pipeline P;
+  P.tb = rs.tb;
+  P.host_noise = learned;
   {
     HostTimer t("host_setup");
-    P.setup(X, dtype, nr, nc, nf, *pp, on_device);
+    P.setup(X, dtype, nr, nc, nf, pl, on_device);
   }
   const pgsvt_params& p = P.p;
   const Geometry& g = P.g;
+  const uint32_t n_in = kFeatureSamples * g.Nw + 3;
+  if (rs.mode == 1 && rs.model->n_in != n_in)
+    return fail(PGSVT_E_ARG, "learned-lambda model expects " + std::to_string(rs.model->n_in) +
+                                 " features, this shape gives " + std::to_string(n_in));
+  if (learned && P.sh.fp64) return fail(PGSVT_E_ARG, "the learned-lambda path runs in FP32 only");
   const double lambda0 = (p.lambda1 >= 0.0) ? p.lambda1 : -1.0;
   const double alpha0 = (p.noise_alpha >= 0.0) ? p.noise_alpha : -1.0;
   const double mu0 = (p.noise_mu >= 0.0) ? p.noise_mu : -1.0;
   const double sigma0 = (p.noise_sigma >= 0.0) ? p.noise_sigma : -1.0;
-  if (on_device) {
-    P.Yout = Y;
-  } else {
-    P.Yd.alloc((size_t)P.sh.fs * nf);
-    P.Yout = P.Yd.p;
+  if (rs.mode != 2) {
+    if (on_device) {
+      P.Yout = Y;
+    } else {
+      P.Yd.alloc((size_t)P.sh.fs * count);
+      P.Yout = P.Yd.p;
+    }
   }
   {
     HostTimer t("host_alloc_batch");
-    P.alloc_batch(nf);
+    P.alloc_batch(count);
   }
 
-  // leaf statistics per distinct (frame, window maximum), computed once
+  // Noise estimates on host threads.  Reference path: leaf statistics per
+  // distinct (frame, window maximum), as the reference normalises each
+  // window by its own maximum.  Learned path: leaf statistics once per frame,
+  // normalised by the sequence maximum, then rescaled to each window's
+  // maximum (means by the ratio, variances by its square) before the fit.
   struct LeafJob {
     uint32_t frame;
     double scale;
@@ -890,20 +962,22 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
   };
   std::vector<std::unique_ptr<LeafJob>> leaf_jobs;
   std::unique_ptr<PrepPool> pool;
-  if (P.optimize) {
+  const double gmax = *std::max_element(P.fmaxX.begin(), P.fmaxX.end());
+  if (P.optimize || learned) {
     std::map<std::pair<uint32_t, uint64_t>, uint32_t> key_of;
     std::vector<std::function<void(PrepPool&)>> jobs;
-    for (uint32_t t = 0; t < nf; ++t) {
+    for (uint32_t t = rs.tb; t < rs.te; ++t) {
       const WinMeta m = P.meta_of(t);
+      const double scale = learned ? gmax : m.uMax;
       uint64_t sbits;
-      std::memcpy(&sbits, &m.uMax, 8);
+      std::memcpy(&sbits, &scale, 8);
       std::vector<uint32_t> need;
       for (uint32_t k = 0; k < g.Nw; ++k) {
         const auto key = std::make_pair(m.s0 + k, sbits);
         auto it = key_of.find(key);
         if (it == key_of.end()) {
           const uint32_t id = (uint32_t)leaf_jobs.size();
-          leaf_jobs.emplace_back(new LeafJob{m.s0 + k, m.uMax, {}, {}});
+          leaf_jobs.emplace_back(new LeafJob{m.s0 + k, scale, {}, {}});
           LeafJob* J = leaf_jobs.back().get();
           J->fu = J->pr.get_future().share();
           jobs.push_back([J, &P, nr](PrepPool&) {
@@ -919,14 +993,25 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
         }
         need.push_back(it->second);
       }
-      jobs.push_back([t, need, &leaf_jobs, &P, &p, g, nr, nc, lambda0, alpha0, mu0, sigma0](PrepPool& pool) {
+      const double ratio = gmax / m.uMax;
+      jobs.push_back([t, need, ratio, learned, &leaf_jobs, &P, &p, &rs, g, nr, nc, lambda0, alpha0, mu0,
+                      sigma0](PrepPool& pool) {
         WindowPrep w;
         w.alpha = alpha0;
         w.mu = mu0;
         w.sigma = sigma0;
         try {
           std::vector<std::shared_ptr<const FrameLeaves>> leaves;
-          for (uint32_t id : need) leaves.push_back(leaf_jobs[id]->fu.get());
+          for (uint32_t id : need) {
+            auto lv = leaf_jobs[id]->fu.get();
+            if (learned && ratio != 1.0) {
+              auto sc = std::make_shared<FrameLeaves>(*lv);
+              for (double& v : sc->mean) v *= ratio;
+              for (double& v : sc->var) v *= ratio * ratio;
+              lv = sc;
+            }
+            leaves.push_back(lv);
+          }
           estimate_from_leaves(leaves, nr, p.noise_method, w.alpha, w.mu, w.sigma);
           const double OoNxNyNt = 1.0 / (nc * nr * g.Nt);
           const double start = (lambda0 >= 0.0) ? lambda0 : P.host_window_accu(t) * OoNxNyNt;
@@ -936,21 +1021,87 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
         } catch (...) {
           w.err = "noise estimate failed";
         }
-        pool.publish(t, std::move(w));
+        pool.publish(t - rs.tb, std::move(w));
       });
     }
-    pool.reset(new PrepPool(std::move(jobs), nf, thread_count(p.n_jobs)));
+    pool.reset(new PrepPool(std::move(jobs), count, thread_count(p.n_jobs)));
+  }
+
+  // learned lambda: fixed patch sample, feature buffer, network
+  std::unique_ptr<Mlp> net;
+  DBuf<uint32_t> sample_d;
+  DBuf<float> feat_d, mean_d, istd_d, yout_d;
+  const uint32_t nlog = kFeatureSamples * g.Nw;
+  if (learned) {
+    const std::vector<uint32_t> smp = feature_sample(P.sh.p_sel, kFeatureSamples);
+    sample_d.alloc(smp.size());
+    hipchk(hipMemcpyAsync(sample_d.p, smp.data(), smp.size() * 4, hipMemcpyHostToDevice, P.s), "H2D sample");
+    feat_d.alloc((size_t)P.NB * n_in);
+    yout_d.alloc(P.NB);
+    if (rs.mode == 1) {
+      net.reset(new Mlp(n_in, rs.model->hidden, P.s));
+      hipchk(hipMemcpyAsync(net->params(), rs.model->params, net->n_params() * 4, hipMemcpyHostToDevice, P.s),
+             "H2D model");
+      mean_d.alloc(n_in);
+      istd_d.alloc(n_in);
+      hipchk(hipMemcpyAsync(mean_d.p, rs.model->mean, n_in * 4, hipMemcpyHostToDevice, P.s), "H2D mean");
+      hipchk(hipMemcpyAsync(istd_d.p, rs.model->inv_std, n_in * 4, hipMemcpyHostToDevice, P.s), "H2D inv_std");
+    }
+    hipchk(hipStreamSynchronize(P.s), "model sync");  // host vectors above are local
   }
 
   int status = 0;
   std::string warn;
-  for (uint32_t t0 = 0; t0 < nf; t0 += P.NB) {
-    const uint32_t nb = std::min(P.NB, nf - t0);
+  for (uint32_t t0 = rs.tb; t0 < rs.te; t0 += P.NB) {
+    const uint32_t nb = std::min(P.NB, rs.te - t0);
     std::vector<WinMeta> ms(nb);
     for (uint32_t b = 0; b < nb; ++b) ms[b] = P.meta_of(t0 + b);
     P.decompose(ms, P.nvar);
     std::vector<WindowPrep> prep(nb);
-    if (P.optimize) {
+    if (learned) {
+      // log singular values of the sampled patches (standardised for the
+      // network), then the noise features of each window as they arrive
+      const uint32_t s_at = g.Nw * ((g.Nw + 3) & ~3u) + g.Nw * P.sh.m;
+      P.prof->run("learned_features", (double)nb, [&] {
+        hipchk(launch_features((const float*)P.factors.p, P.nvar, P.sh.p_sel, P.sh.fstride, s_at, g.Nw, sample_d.p,
+                               kFeatureSamples, nb, n_in, rs.mode == 1 ? mean_d.p : nullptr,
+                               rs.mode == 1 ? istd_d.p : nullptr, feat_d.p, P.s),
+               "features");
+      });
+      std::vector<float> nz((size_t)nb * 3);
+      {
+        HostTimer tw("host_noise_wait");
+        for (uint32_t b = 0; b < nb; ++b) {
+          prep[b] = pool->wait(t0 + b - rs.tb);
+          if (!prep[b].err.empty()) throw std::runtime_error(prep[b].err);
+          const double raw[3] = {prep[b].alpha, prep[b].mu, prep[b].sigma};
+          for (int q = 0; q < 3; ++q) {
+            float v = (float)raw[q];
+            if (rs.mode == 1) v = (v - rs.model->mean[nlog + q]) * rs.model->inv_std[nlog + q];
+            nz[(size_t)b * 3 + q] = v;
+          }
+        }
+      }
+      hipchk(hipMemcpy2DAsync(feat_d.p + nlog, (size_t)n_in * 4, nz.data(), 12, 12, nb, hipMemcpyHostToDevice, P.s),
+             "H2D noise features");
+      if (rs.mode == 2) {
+        hipchk(hipMemcpyAsync(rs.features + (size_t)(t0 - rs.tb) * n_in, feat_d.p, (size_t)nb * n_in * 4,
+                              hipMemcpyDeviceToHost, P.s),
+               "D2H features");
+        hipchk(hipStreamSynchronize(P.s), "features sync");
+        for (uint32_t b = 0; b < nb; ++b) ms[b].lambda = 0.0;
+      } else {
+        std::vector<float> yh(nb);
+        P.prof->run("learned_mlp", (double)nb, [&] { net->forward(feat_d.p, nb, yout_d.p); });
+        hipchk(hipMemcpyAsync(yh.data(), yout_d.p, nb * 4, hipMemcpyDeviceToHost, P.s), "D2H lambda");
+        hipchk(hipStreamSynchronize(P.s), "mlp sync");  // also orders the noise-feature upload
+        for (uint32_t b = 0; b < nb; ++b) {
+          const double l = std::exp((double)rs.model->y_mean + (double)rs.model->y_std * (double)yh[b]);
+          ms[b].lambda = std::min(std::max(l, 0.0), std::max(100.0, prep[b].start));  // the search's bounds
+        }
+        P.prof->flush();
+      }
+    } else if (P.optimize) {
       std::vector<std::unique_ptr<Sbplx1D>> opt(nb);
       std::vector<int> code(nb, Sbplx1D::kNeed);
       {
@@ -962,7 +1113,7 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
       std::vector<char> started(nb, 0);
       uint32_t n_started = 0;
       auto admit = [&](uint32_t b) {
-        prep[b] = pool->wait(t0 + b);
+        prep[b] = pool->wait(t0 + b - rs.tb);
         if (!prep[b].err.empty()) throw std::runtime_error(prep[b].err);
         const double start = prep[b].start;
         const double ub = std::max(100.0, start);
@@ -974,7 +1125,7 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
       std::vector<double> sums;
       for (;;) {
         for (uint32_t b = 0; b < nb; ++b)
-          if (!started[b] && pool->ready(t0 + b)) admit(b);
+          if (!started[b] && pool->ready(t0 + b - rs.tb)) admit(b);
         std::vector<uint32_t> list;
         for (uint32_t b = 0; b < nb; ++b) {
           if (!started[b] || code[b] != Sbplx1D::kNeed) continue;
@@ -1019,19 +1170,28 @@ int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
         prep[b].sigma = sigma0;
       }
     }
-    P.finish(ms);
+    if (rs.mode != 2) P.finish(ms);
     for (uint32_t b = 0; b < nb; ++b) {
-      const uint32_t t = t0 + b;
-      est[t + (size_t)nf * 0] = ms[b].lambda;
-      est[t + (size_t)nf * 1] = prep[b].alpha;
-      est[t + (size_t)nf * 2] = prep[b].mu;
-      est[t + (size_t)nf * 3] = prep[b].sigma;
+      const uint32_t i = t0 + b - rs.tb;
+      est[i + (size_t)count * 0] = ms[b].lambda;
+      est[i + (size_t)count * 1] = prep[b].alpha;
+      est[i + (size_t)count * 2] = prep[b].mu;
+      est[i + (size_t)count * 3] = prep[b].sigma;
     }
   }
-  if (!on_device) hipchk(hipMemcpyAsync(Y, P.Yd.p, P.Yd.bytes(), hipMemcpyDeviceToHost, P.s), "D2H Y");
+  if (rs.mode != 2 && !on_device)
+    hipchk(hipMemcpyAsync(Y, P.Yd.p, P.Yd.bytes(), hipMemcpyDeviceToHost, P.s), "D2H Y");
   hipchk(hipStreamSynchronize(P.s), "final sync");
   if (status) return fail(status, warn);
   return 0;
+}
+
+int run_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, const pgsvt_params* pp, double* Y,
+                double* est, bool on_device) {
+  RunSpec rs;
+  rs.tb = 0;
+  rs.te = nf;
+  return run_core(X, dtype, nr, nc, nf, pp, Y, est, on_device, rs);
 }
 
 template <class F>
@@ -1093,6 +1253,12 @@ const char* pgsvt_last_error(void) { return g_err.c_str(); }
 
 const char* pgsvt_version(void) { return "pgsvt 0.1.0 (gfx950)"; }
 
+int pgsvt_set_precision(int bits) {
+  if (bits != 32 && bits != 64) return fail(PGSVT_E_ARG, "precision must be 32 or 64 bits");
+  g_precision.store(bits);
+  return 0;
+}
+
 int pgsvt_release_cache(void) {
   return guarded([&] {
     int dev = 0;
@@ -1113,6 +1279,54 @@ int pgsvt_denoise_device(const void* X_dev, int dtype, uint32_t n_rows, uint32_t
                          const pgsvt_params* p, double* Y_dev, double* estimates) {
   if (!Y_dev || !estimates) return fail(PGSVT_E_ARG, "null output pointer");
   return guarded([&] { return run_denoise(X_dev, dtype, n_rows, n_cols, n_frames, p, Y_dev, estimates, true); });
+}
+
+int pgsvt_denoise_range(const void* X, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
+                        const pgsvt_params* p, uint32_t t_begin, uint32_t t_end, double* Y, double* estimates) {
+  return guarded([&] {
+    RunSpec rs;
+    rs.tb = t_begin;
+    rs.te = t_end;
+    return run_core(X, dtype, n_rows, n_cols, n_frames, p, Y, estimates, false, rs);
+  });
+}
+
+int pgsvt_learned_features(const void* X, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
+                           const pgsvt_params* p, uint32_t t_begin, uint32_t t_end, float* features,
+                           double* estimates) {
+  return guarded([&] {
+    RunSpec rs;
+    rs.tb = t_begin;
+    rs.te = t_end;
+    rs.mode = 2;
+    rs.features = features;
+    return run_core(X, dtype, n_rows, n_cols, n_frames, p, nullptr, estimates, false, rs);
+  });
+}
+
+int pgsvt_denoise_learned(const void* X, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
+                          const pgsvt_params* p, const pgsvt_lambda_model* model, double* Y, double* estimates) {
+  return guarded([&] {
+    RunSpec rs;
+    rs.tb = 0;
+    rs.te = n_frames;
+    rs.mode = 1;
+    rs.model = model;
+    return run_core(X, dtype, n_rows, n_cols, n_frames, p, Y, estimates, false, rs);
+  });
+}
+
+int pgsvt_denoise_learned_device(const void* X_dev, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
+                                 const pgsvt_params* p, const pgsvt_lambda_model* model, double* Y_dev,
+                                 double* estimates) {
+  return guarded([&] {
+    RunSpec rs;
+    rs.tb = 0;
+    rs.te = n_frames;
+    rs.mode = 1;
+    rs.model = model;
+    return run_core(X_dev, dtype, n_rows, n_cols, n_frames, p, Y_dev, estimates, true, rs);
+  });
 }
 
 void pgsvt_profile_enable(int on) {
@@ -1207,14 +1421,18 @@ int pgsvt_window_singular_values(const void* X, int dtype, uint32_t n_rows, uint
     Pipeline P;
     std::vector<WinMeta> ms;
     if (int rc = one_window(X, dtype, n_rows, n_cols, n_frames, p, t, false, P, ms)) return rc;
-    std::vector<float> f((size_t)P.sh.p_sel * P.sh.fstride);
-    hipchk(hipMemcpyAsync(f.data(), P.factors.p, f.size() * 4, hipMemcpyDeviceToHost, P.s), "D2H factors");
+    const size_t es = P.sh.fp64 ? 8 : 4, nval = (size_t)P.sh.p_sel * P.sh.fstride;
+    std::vector<char> f(nval * es);
+    hipchk(hipMemcpyAsync(f.data(), P.factors.p, f.size(), hipMemcpyDeviceToHost, P.s), "D2H factors");
     hipchk(hipStreamSynchronize(P.s), "sync");
+    auto at = [&](size_t i) {
+      return P.sh.fp64 ? reinterpret_cast<const double*>(f.data())[i] : (double)reinterpret_cast<const float*>(f.data())[i];
+    };
     const uint32_t n = P.sh.nw;
     const uint32_t s_at = n * ((n + 3) & ~3u) + n * P.sh.m;  // after V^T and W^T
     for (uint32_t k = 0; k < P.sh.p_sel; ++k) {
       std::vector<double> v(n);
-      for (uint32_t j = 0; j < n; ++j) v[j] = f[(size_t)k * P.sh.fstride + s_at + j];
+      for (uint32_t j = 0; j < n; ++j) v[j] = at((size_t)k * P.sh.fstride + s_at + j);
       std::sort(v.begin(), v.end(), [](double a, double b) { return a > b; });
       std::memcpy(s + (size_t)k * n, v.data(), n * sizeof(double));
     }

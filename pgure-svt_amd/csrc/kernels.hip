@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -171,10 +172,11 @@ hipError_t launch_delta2(int64_t seed, uint64_t nvox, uint8_t* out, hipStream_t 
 // ------------------------------------------------------ window volumes --
 // src/pguresvt.hpp:100-120 (u = window / max, w likewise for ARPS) and
 // src/pgure.hpp:80-82 (U2p = U + delta2*eps2, U2m = U - delta2*eps2).
+template <class T>
 __global__ void k_prepare(const double* __restrict__ X, const double* __restrict__ Z, const WinMeta* __restrict__ meta,
                           const uint8_t* __restrict__ d2, uint32_t fs, uint32_t vol, uint32_t nvar, double eps2,
                           double d2neg, double d2pos, double* __restrict__ u64, double* __restrict__ w64,
-                          float* __restrict__ u32) {
+                          T* __restrict__ u32) {
   const uint32_t b = blockIdx.y;
   const WinMeta m = meta[b];
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < vol; v += gridDim.x * blockDim.x) {
@@ -182,22 +184,26 @@ __global__ void k_prepare(const double* __restrict__ X, const double* __restrict
     const double u = X[src] / m.uMax;
     u64[(size_t)b * vol + v] = u;
     if (Z) w64[(size_t)b * vol + v] = Z[src] / m.wMax;
-    float* o = u32 + (size_t)b * nvar * vol + v;
-    o[0] = (float)u;
+    T* o = u32 + (size_t)b * nvar * vol + v;
+    o[0] = (T)u;
     if (nvar == 3) {
       const double dl = d2[v] ? d2neg : d2pos;
-      o[vol] = (float)(u + dl * eps2);
-      o[2 * (size_t)vol] = (float)(u - dl * eps2);
+      o[vol] = (T)(u + dl * eps2);
+      o[2 * (size_t)vol] = (T)(u - dl * eps2);
     }
   }
 }
 
 hipError_t launch_prepare(const double* X, const double* Z, const WinMeta* meta, uint32_t nb, const uint8_t* d2,
                           const Shape& sh, uint32_t nvar, double eps2, double d2neg, double d2pos, double* u64,
-                          double* w64, float* u32, hipStream_t s) {
+                          double* w64, void* uvar, hipStream_t s) {
   const dim3 grid(std::min<uint32_t>((sh.vol + 255) / 256, 1024), nb);
-  hipLaunchKernelGGL(k_prepare, grid, dim3(256), 0, s, X, Z, meta, d2, sh.fs, sh.vol, nvar, eps2, d2neg, d2pos,
-                     u64, w64, u32);
+  if (sh.fp64)
+    hipLaunchKernelGGL(k_prepare<double>, grid, dim3(256), 0, s, X, Z, meta, d2, sh.fs, sh.vol, nvar, eps2, d2neg,
+                       d2pos, u64, w64, (double*)uvar);
+  else
+    hipLaunchKernelGGL(k_prepare<float>, grid, dim3(256), 0, s, X, Z, meta, d2, sh.fs, sh.vol, nvar, eps2, d2neg,
+                       d2pos, u64, w64, (float*)uvar);
   return hipGetLastError();
 }
 
@@ -428,12 +434,30 @@ __device__ __forceinline__ float group_sum(float v) {
   return v;
 }
 
-template <int BS, int N, int L>
-__global__ __launch_bounds__(256) void k_svd(const float* __restrict__ u32, const int32_t* __restrict__ pos,
+// FP64 (the precision-study path): a lane-swap butterfly, same property
+// (every lane of the group ends with the bit-identical sum)
+template <int L>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = 1; o < L; o <<= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// reciprocal, square root and reciprocal square root: the hardware
+// approximations in FP32, IEEE-correct operations in FP64
+__device__ __forceinline__ float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_(double x) { return 1.0 / x; }
+__device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+__device__ __forceinline__ float rsq_(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsq_(double x) { return 1.0 / sqrt(x); }
+
+template <int BS, int N, int L, class T>
+__global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const int32_t* __restrict__ pos,
                                              const uint32_t* __restrict__ sel, uint32_t nvar, uint32_t p_sel,
                                              uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol,
-                                             uint32_t lim, uint32_t fstride, int max_sweeps, float tol2,
-                                             float* __restrict__ factors, uint32_t* __restrict__ stats) {
+                                             uint32_t lim, uint32_t fstride, int max_sweeps, T tol2,
+                                             T* __restrict__ factors, uint32_t* __restrict__ stats) {
   constexpr int M = BS * BS, R = (M + L - 1) / L, VR = (N + L - 1) / L, SR = (N + 3) & ~3;
   static_assert(L * VR >= SR, "V^T padding not covered");
   const uint32_t b = blockIdx.y;
@@ -443,8 +467,8 @@ __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ u32, cons
   const bool valid = var < nvar;  // uniform per group
   const uint32_t it = valid ? sel[p] : 0;
 
-  float x[R][N], v[VR][N];
-  const float* U = u32 + ((size_t)b * nvar + (valid ? var : 0)) * vol;
+  T x[R][N], v[VR][N];
+  const T* U = u32 + ((size_t)b * nvar + (valid ? var : 0)) * vol;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const int32_t pk = valid ? pos[((size_t)b * nw + k) * p_all + it] : 0;
@@ -452,50 +476,50 @@ __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ u32, cons
 #pragma unroll
     for (int h = 0; h < R; ++h) {
       const int q = lane + L * h;
-      x[h][k] = (valid && q < M) ? U[(size_t)k * fs + (row + q % BS) + (size_t)nr * (col + q / BS)] : 0.f;
+      x[h][k] = (valid && q < M) ? U[(size_t)k * fs + (row + q % BS) + (size_t)nr * (col + q / BS)] : T(0);
     }
 #pragma unroll
-    for (int h = 0; h < VR; ++h) v[h][k] = (lane + L * h == k) ? 1.f : 0.f;
+    for (int h = 0; h < VR; ++h) v[h][k] = (lane + L * h == k) ? T(1) : T(0);
   }
 
-  float a[N];
+  T a[N];
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-      float t = x[0][k] * x[0][k];
+      T t = x[0][k] * x[0][k];
 #pragma unroll
-      for (int h = 1; h < R; ++h) t = fmaf(x[h][k], x[h][k], t);
+      for (int h = 1; h < R; ++h) t = fma(x[h][k], x[h][k], t);
       a[k] = group_sum<L>(t);
     }
     // sum of t^2 over the sweep: zero exactly when no pair rotated.  A
     // floating-point chain (never reassociated) keeps this in one register.
-    float tsum = 0.f;
+    T tsum = T(0);
 #pragma unroll
     for (int i = 0; i < N - 1; ++i) {
 #pragma unroll
       for (int j = i + 1; j < N; ++j) {
         // rotation zeroing the (i, j) inner product: zeta = (b - a) / 2c,
         // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), cs = 1/sqrt(1+t^2)
-        float cp = x[0][i] * x[0][j];
+        T cp = x[0][i] * x[0][j];
 #pragma unroll
-        for (int h = 1; h < R; ++h) cp = fmaf(x[h][i], x[h][j], cp);
-        const float c = group_sum<L>(cp);
+        for (int h = 1; h < R; ++h) cp = fma(x[h][i], x[h][j], cp);
+        const T c = group_sum<L>(cp);
         const bool rot = c * c > tol2 * a[i] * a[j];
-        const float zeta = (a[j] - a[i]) * __builtin_amdgcn_rcpf(c + c);
-        const float den = fabsf(zeta) + __builtin_amdgcn_sqrtf(fmaf(zeta, zeta, 1.f));
-        const float t = rot ? copysignf(__builtin_amdgcn_rcpf(den), zeta) : 0.f;
-        const float cs = __builtin_amdgcn_rsqf(fmaf(t, t, 1.f));
-        const float sn = cs * t;
+        const T zeta = (a[j] - a[i]) * rcp_(c + c);
+        const T den = fabs(zeta) + sqrt_(fma(zeta, zeta, T(1)));
+        const T t = rot ? copysign(rcp_(den), zeta) : T(0);
+        const T cs = rsq_(fma(t, t, T(1)));
+        const T sn = cs * t;
 #pragma unroll
         for (int h = 0; h < R; ++h) {
-          const float xi = x[h][i], xj = x[h][j];
+          const T xi = x[h][i], xj = x[h][j];
           x[h][i] = cs * xi - sn * xj;
           x[h][j] = sn * xi + cs * xj;
         }
 #pragma unroll
         for (int h = 0; h < VR; ++h) {
-          const float vi = v[h][i], vj = v[h][j];
+          const T vi = v[h][i], vj = v[h][j];
           v[h][i] = cs * vi - sn * vj;
           v[h][j] = sn * vi + cs * vj;
           // Pin the V update next to its rotation.  V only feeds the next
@@ -505,20 +529,20 @@ __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ u32, cons
           // no instruction and no memory access.
           asm volatile("" : "+v"(v[h][i]), "+v"(v[h][j]));
         }
-        a[i] = fmaf(-t, c, a[i]);
-        a[j] = fmaf(t, c, a[j]);
-        tsum = fmaf(t, t, tsum);
+        a[i] = fma(-t, c, a[i]);
+        a[j] = fma(t, c, a[j]);
+        tsum = fma(t, t, tsum);
       }
     }
-    if (!__any(tsum > 0.f)) break;
+    if (!__any(tsum > T(0))) break;
   }
   if (stats && lane == 0 && valid) atomicMax(stats, (uint32_t)sweep + 1);
   // final column norms (identical in every lane of the group)
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    float t = x[0][k] * x[0][k];
+    T t = x[0][k] * x[0][k];
 #pragma unroll
-    for (int h = 1; h < R; ++h) t = fmaf(x[h][k], x[h][k], t);
+    for (int h = 1; h < R; ++h) t = fma(x[h][k], x[h][k], t);
     a[k] = group_sum<L>(t);
   }
   if (!valid) return;
@@ -527,9 +551,9 @@ __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ u32, cons
   // stride SR, padded with zeros), row r of W^T is the matching column of
   // W = X V = U S (the rotated Casorati columns), then the sorted values.
   // rank(l) counts the values that sort before column l (ties by index).
-  float* F = factors + (((size_t)b * nvar + var) * p_sel + p) * fstride;
-  float* FW = F + N * SR;
-  float* FS = FW + N * M;
+  T* F = factors + (((size_t)b * nvar + var) * p_sel + p) * fstride;
+  T* FW = F + N * SR;
+  T* FS = FW + N * M;
 #pragma unroll
   for (int l = 0; l < N; ++l) {
     int rank = 0;
@@ -539,14 +563,14 @@ __global__ __launch_bounds__(256) void k_svd(const float* __restrict__ u32, cons
     for (int h = 0; h < VR; ++h) {
       const int vr = lane + L * h;
       if (vr < N) F[rank * SR + vr] = v[h][l];
-      else if (vr < SR) F[rank * SR + vr] = 0.f;
+      else if (vr < SR) F[rank * SR + vr] = T(0);
     }
 #pragma unroll
     for (int h = 0; h < R; ++h) {
       const int q = lane + L * h;
       if (q < M) FW[rank * M + q] = x[h][l];
     }
-    if (lane == 0) FS[rank] = sqrtf(a[l]);
+    if (lane == 0) FS[rank] = sqrt(a[l]);
   }
 }
 
@@ -719,15 +743,15 @@ __device__ __forceinline__ void xcd_block(uint32_t& bx, uint32_t& by) {
 // sum_c (B1_c - 2 B0_c + B2_c) delta2_v / W_v, the divergence sum of
 // src/pgure.hpp:136 (linear in the overlap-add), is accumulated in FP64.
 // nvar_run == 1: every group reconstructs U of its own patch.
-template <int BS, int N, int L>
+template <int BS, int N, int L, class T>
 __global__ __launch_bounds__(256) void k_eval(
-    const float* __restrict__ factors, const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel,
+    const T* __restrict__ factors, const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel,
     const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl, uint32_t nvar_stored, uint32_t nvar_run,
     uint32_t p_sel, uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim,
     uint32_t fstride, int exp_w, const double* __restrict__ dw, const int32_t* __restrict__ inv,
-    float* __restrict__ B0, double* __restrict__ partial) {
+    T* __restrict__ B0, double* __restrict__ partial) {
   constexpr int M = BS * BS, G = 256 / L, SR = (N + 3) & ~3, P3 = G / 3;
-  __shared__ float xch[P3][3][M][N];  // reconstructions of the three variants
+  __shared__ T xch[P3][3][M][N];  // reconstructions of the three variants
   __shared__ double red[4];
   uint32_t bx, slot;
   xcd_block(bx, slot);
@@ -747,15 +771,15 @@ __global__ __launch_bounds__(256) void k_eval(
   const bool valid = var >= 0 && p < p_sel;
   const bool row = valid && lane < M;
   const double lam = meta[b].lambda;
-  const float* F = factors + (((size_t)b * nvar_stored + (valid ? var : 0)) * p_sel + (valid ? p : 0)) * fstride;
-  const float* FW = F + N * SR;
-  const float* FS = FW + N * M;
+  const T* F = factors + (((size_t)b * nvar_stored + (valid ? var : 0)) * p_sel + (valid ? p : 0)) * fstride;
+  const T* FW = F + N * SR;
+  const T* FS = FW + N * M;
   // Loads that do not depend on the reconstruction go out first: the
   // singular values, then (PGURE) the bucket places and trajectory of the
   // frames this group finishes (var, var + 3, ...).
   constexpr int KP = (N + 2) / 3;
   const bool pg = nvar_run == 3 && row;
-  float sl = 0.f, smax = 0.f;
+  T sl = T(0), smax = T(0);
   if (valid && lane < N) {
     sl = FS[lane];
     smax = FS[0];
@@ -772,41 +796,48 @@ __global__ __launch_bounds__(256) void k_eval(
     }
   }
   // shrink factor g(s_r)/s_r in lane r < N; s_0 is the largest
-  float dl = 0.f;
+  T dl = T(0);
   if (valid && lane < N) {
-    const float w = exp_w ? fabsf(smax * expf((float)(-0.5 * lam) * (sl * sl))) : (float)lam;
-    dl = sl > 0.f ? fmaxf(sl - w, 0.f) / sl : 0.f;
+    const T w = exp_w ? fabs(smax * exp((T)(-0.5 * lam) * (sl * sl))) : (T)lam;
+    dl = sl > T(0) ? fmax(sl - w, T(0)) / sl : T(0);
   }
   // kept ranks of this group: 1 + highest lane with g > 0
-  const uint64_t bal = __ballot(dl > 0.f);
+  const uint64_t bal = __ballot(dl > T(0));
   const uint32_t gbits = (uint32_t)((bal >> ((threadIdx.x % 64) / L * L)) & ((L >= 64) ? ~0ull : ((1ull << L) - 1)));
   const int nkeep = gbits ? 32 - __clz(gbits) : 0;
-  float bq[N];
+  T bq[N];
 #pragma unroll
-  for (int k = 0; k < N; ++k) bq[k] = 0.f;
+  for (int k = 0; k < N; ++k) bq[k] = T(0);
   // rank loop, loads of rank r + 1 issued before the products of rank r
-  float wn = 0.f;
-  float4 vn[SR / 4];
+  // (FP32: the V^T row as SR/4 float4 loads; FP64: SR scalar loads)
+  constexpr int VW = (sizeof(T) == 4) ? 4 : 1, NV = SR / VW;
+  typedef typename std::conditional<sizeof(T) == 4, float4, double>::type VT;
+  T wn = T(0);
+  VT vn[NV];
   auto fetch = [&](int r) {
-    wn = row ? FW[r * M + lane] : 0.f;
-    const float4* vr = reinterpret_cast<const float4*>(F + r * SR);
+    wn = row ? FW[r * M + lane] : T(0);
+    const VT* vr = reinterpret_cast<const VT*>(F + r * SR);
 #pragma unroll
-    for (int j = 0; j < SR / 4; ++j) vn[j] = vr[j];
+    for (int j = 0; j < NV; ++j) vn[j] = vr[j];
   };
   if (nkeep > 0) fetch(0);
   for (int r = 0; r < nkeep; ++r) {
-    const float y = __shfl(dl, r, L) * wn;
-    float4 vc[SR / 4];
+    const T y = __shfl(dl, r, L) * wn;
+    VT vc[NV];
 #pragma unroll
-    for (int j = 0; j < SR / 4; ++j) vc[j] = vn[j];
+    for (int j = 0; j < NV; ++j) vc[j] = vn[j];
     if (r + 1 < nkeep) fetch(r + 1);
 #pragma unroll
-    for (int j = 0; j < SR / 4; ++j) {
-      const float4 t = vc[j];
-      if (4 * j < N) bq[4 * j] = fmaf(y, t.x, bq[4 * j]);
-      if (4 * j + 1 < N) bq[4 * j + 1] = fmaf(y, t.y, bq[4 * j + 1]);
-      if (4 * j + 2 < N) bq[4 * j + 2] = fmaf(y, t.z, bq[4 * j + 2]);
-      if (4 * j + 3 < N) bq[4 * j + 3] = fmaf(y, t.w, bq[4 * j + 3]);
+    for (int j = 0; j < NV; ++j) {
+      if constexpr (VW == 4) {
+        const float4 t = vc[j];
+        if (4 * j < N) bq[4 * j] = fma(y, t.x, bq[4 * j]);
+        if (4 * j + 1 < N) bq[4 * j + 1] = fma(y, t.y, bq[4 * j + 1]);
+        if (4 * j + 2 < N) bq[4 * j + 2] = fma(y, t.z, bq[4 * j + 2]);
+        if (4 * j + 3 < N) bq[4 * j + 3] = fma(y, t.w, bq[4 * j + 3]);
+      } else {
+        if (j < N) bq[j] = fma(y, (T)vc[j], bq[j]);
+      }
     }
   }
   double s4 = 0.0;
@@ -833,7 +864,7 @@ __global__ __launch_bounds__(256) void k_eval(
       for (int i = 0; i < KP; ++i) {
         const int k = var + 3 * i;
         if (k < N) {
-          const float x0 = xch[pl][0][lane][k];
+          const T x0 = xch[pl][0][lane][k];
           B0[(((size_t)slot * nw + k) * p_sel + ib[i]) * M + lane] = x0;
           const double c2 =
               ((double)xch[pl][1][lane][k] + (double)xch[pl][2][lane][k]) - 2.0 * (double)x0;
@@ -861,13 +892,13 @@ __global__ __launch_bounds__(256) void k_eval(
 // position by position (patch column cb, then row ra) and bucket by bucket
 // in ascending patch order, summed in FP64.
 //   mode 0: risk sums per block: sum Uhat^2, sum Uhat, sum Uhat U, sum U, sum U^2
-//   mode 1: final frame: Y(:,:,t) = uMax * Uhat(:,:,ref)
+//   mode 1: final frame: Y(:,:,out) = uMax * Uhat(:,:,ref)
 //   mode 2: whole window volume of slot 0 into out
 // One thread per voxel.  For each patch column cb the buckets of positions
 // (r-bs+1 .. r, c-cb) are one contiguous run of rows in bucket order, read
 // straight from memory (neighbouring voxels share them in cache).
-template <int BS>
-__global__ __launch_bounds__(256) void k_gather_direct(const float* __restrict__ B0, const int32_t* __restrict__ off,
+template <int BS, class T>
+__global__ __launch_bounds__(256) void k_gather_direct(const T* __restrict__ B0, const int32_t* __restrict__ off,
                                                        const double* __restrict__ invw,
                                                        const double* __restrict__ u64,
                                                        const WinMeta* __restrict__ meta,
@@ -889,7 +920,7 @@ __global__ __launch_bounds__(256) void k_gather_direct(const float* __restrict__
     const uint32_t k = v / fs, pix = v % fs;
     const int r = (int)(pix % nr), c = (int)(pix / nr);
     const int32_t* O = off + ((size_t)b * nw + k) * ((size_t)n1 * n1 + 1);
-    const float* Bk = B0 + ((size_t)slot * nw + k) * p_sel * M;
+    const T* Bk = B0 + ((size_t)slot * nw + k) * p_sel * M;
     // Patch columns in chunks of CH (all at once for small patches): bucket
     // bounds of positions (r - ra, c - cb), bnd[.][e] for rows r-BS+1+e
     // (clamped to the frame), e = BS the end of row r; then the first
@@ -911,14 +942,14 @@ __global__ __launch_bounds__(256) void k_gather_direct(const float* __restrict__
           bnd[h][e] = in ? O[colq + prc] : 0;
         }
       }
-      float first[CH][BS];
+      T first[CH][BS];
 #pragma unroll
       for (int h = 0; h < CH; ++h)
 #pragma unroll
         for (int ra = 0; ra < BS; ++ra) {
           const int e = BS - 1 - ra;  // position r - ra
           const int32_t j = bnd[h][e];
-          first[h][ra] = (j < bnd[h][e + 1]) ? Bk[(size_t)j * M + ra + BS * (cb0 + h)] : 0.f;
+          first[h][ra] = (j < bnd[h][e + 1]) ? Bk[(size_t)j * M + ra + BS * (cb0 + h)] : T(0);
         }
 #pragma unroll
       for (int h = 0; h < CH; ++h)
@@ -941,7 +972,7 @@ __global__ __launch_bounds__(256) void k_gather_direct(const float* __restrict__
       a_u = U;
       a_u2 = U * U;
     } else if (mode == 1) {
-      out[(size_t)m.t * fs + i] = uh * m.uMax;
+      out[(size_t)m.out * fs + i] = uh * m.uMax;
     } else {
       out[v] = uh;
     }
@@ -980,8 +1011,8 @@ __global__ __launch_bounds__(256) void k_gather_direct(const float* __restrict__
 // order is fixed, so the sums are reproducible.  A tile with a column range
 // longer than CAP patches (trajectories piling up) reads global memory in
 // the order of k_gather_direct instead.
-template <int BS>
-__global__ __launch_bounds__(256) void k_gather_tile(const float* __restrict__ B0, const int32_t* __restrict__ off,
+template <int BS, class T>
+__global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, const int32_t* __restrict__ off,
                                                      const double* __restrict__ invw,
                                                      const double* __restrict__ u64,
                                                      const WinMeta* __restrict__ meta,
@@ -989,10 +1020,13 @@ __global__ __launch_bounds__(256) void k_gather_tile(const float* __restrict__ B
                                                      uint32_t nw, uint32_t vol, uint32_t n1, uint32_t p_sel,
                                                      int mode, double* __restrict__ out,
                                                      double* __restrict__ partial) {
-  constexpr int M = BS * BS, TR = 32, TC = 8, NC = TC + BS - 1, NRW = TR + BS, CAP = 48;
-  constexpr int RS = NC * CAP + 1;           // plane stride (odd: spreads the planes over banks)
-  constexpr int W = (M % 4 == 0) ? 4 : 1;    // floats per staging load
-  __shared__ float sB[M * RS];
+  // CAP: staged patches per patch column (FP64 keeps the LDS below 64 KB)
+  constexpr int M = BS * BS, TR = 32, TC = 8, NC = TC + BS - 1, NRW = TR + BS, CAP = sizeof(T) == 4 ? 48 : 40;
+  constexpr int RS = NC * CAP + 1;             // plane stride (odd: spreads the planes over banks)
+  constexpr int VW = 16 / (int)sizeof(T);      // values per 16-byte load
+  constexpr int W = (M % VW == 0) ? VW : 1;    // values per staging load
+  typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type VT;
+  __shared__ T sB[M * RS];
   __shared__ int32_t sO[NC][NRW];  // bucket offsets of the tile's positions (+ one past the last row)
   __shared__ double sm[5][4];
   uint32_t bx, slot;
@@ -1006,7 +1040,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(const float* __restrict__ B
   const int r = r0 + (int)(threadIdx.x % TR), c = c0 + (int)(threadIdx.x / TR);
   const bool live = r < (int)nr && c < (int)nr;
   const int32_t* O = off + ((size_t)b * nw + k) * ((size_t)n1 * n1 + 1);
-  const float* Bk = B0 + ((size_t)slot * nw + k) * p_sel * M;
+  const T* Bk = B0 + ((size_t)slot * nw + k) * p_sel * M;
   // the voxel's 1/W and U are independent of the staging: load them first
   const uint32_t pix = (uint32_t)min(r, (int)nr - 1) + nr * (uint32_t)min(c, (int)nr - 1), v = k * fs + pix;
   const double iw = invw[(size_t)b * vol + v];
@@ -1029,12 +1063,11 @@ __global__ __launch_bounds__(256) void k_gather_tile(const float* __restrict__ B
       const int32_t j0 = sO[lc][0];
       if (lr < sO[lc][nrow - 1] - j0) {
         const int dst = lc * CAP + lr;
-        if constexpr (W == 4) {
-          const float4 v = *reinterpret_cast<const float4*>(Bk + (size_t)(j0 + lr) * M + q);
-          sB[(q + 0) * RS + dst] = v.x;
-          sB[(q + 1) * RS + dst] = v.y;
-          sB[(q + 2) * RS + dst] = v.z;
-          sB[(q + 3) * RS + dst] = v.w;
+        if constexpr (W > 1) {
+          const VT v = *reinterpret_cast<const VT*>(Bk + (size_t)(j0 + lr) * M + q);
+          const T* vv = reinterpret_cast<const T*>(&v);
+#pragma unroll
+          for (int w = 0; w < W; ++w) sB[(q + w) * RS + dst] = vv[w];
         } else {
           sB[q * RS + dst] = Bk[(size_t)(j0 + lr) * M + q];
         }
@@ -1070,7 +1103,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(const float* __restrict__ B
           const int e = BS - 1 - ra;  // position r - ra
           const int32_t j0 = bnd[cb][e], n = bnd[cb][e + 1] - j0;
           const int row = min(max(j0 + base, 0), NC * CAP - 1);  // in bounds even for empty buckets
-          const float f = sB[(ra + BS * cb) * RS + row];
+          const T f = sB[(ra + BS * cb) * RS + row];
           acc += (n > 0) ? (double)f : 0.0;
           multi = multi || n > 1;
         }
@@ -1104,7 +1137,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(const float* __restrict__ B
       a_u = U;
       a_u2 = U * U;
     } else if (mode == 1) {
-      out[(size_t)m.t * fs + pix] = uh * m.uMax;
+      out[(size_t)m.out * fs + pix] = uh * m.uMax;
     } else {
       out[v] = uh;
     }
@@ -1173,16 +1206,25 @@ bool svd_shape_supported(uint32_t bs, uint32_t nw) {
   return false;
 }
 
-hipError_t launch_svd(const float* u32, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
-                      const Shape& sh, float* factors, int max_sweeps, float tol, uint32_t* stats, hipStream_t s) {
+hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
+                      const Shape& sh, void* factors, int max_sweeps, double tol, uint32_t* stats, hipStream_t s) {
   const uint32_t lim = sh.nr - sh.bs;
-  const float tol2 = tol * tol;
-#define PG_SVD(B, N, LE, L)                                                                                          \
+#define PG_SVD(B, N, LE, L)                                                                                      \
   if (sh.bs == B && sh.nw == N) {                                                                              \
     const uint32_t groups = nvar * sh.p_sel;                                                                   \
-    const dim3 grid((groups * L + 255) / 256, nb);                                                             \
-    hipLaunchKernelGGL((k_svd<B, N, L>), grid, dim3(256), 0, s, u32, pos, sel, nvar, sh.p_sel, sh.p_all,      \
-                       sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps, tol2, factors, stats);        \
+    if (sh.fp64) {                                                                                             \
+      /* FP64: at least 16 lanes per matrix (fewer rows per lane fit the registers) */                        \
+      constexpr int LD = (L < 16) ? 16 : L;                                                                    \
+      const dim3 grid((groups * LD + 255) / 256, nb);                                                          \
+      hipLaunchKernelGGL((k_svd<B, N, LD, double>), grid, dim3(256), 0, s, (const double*)uvar, pos, sel, nvar, \
+                         sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps, tol * tol, \
+                         (double*)factors, stats);                                                             \
+    } else {                                                                                                   \
+      const dim3 grid((groups * L + 255) / 256, nb);                                                           \
+      hipLaunchKernelGGL((k_svd<B, N, L, float>), grid, dim3(256), 0, s, (const float*)uvar, pos, sel, nvar,   \
+                         sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps,          \
+                         (float)(tol * tol), (float*)factors, stats);                                          \
+    }                                                                                                          \
     return hipGetLastError();                                                                                  \
   }
   PG_SHAPES(PG_SVD)
@@ -1196,16 +1238,21 @@ uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run) {
   return (sh.p_sel + per - 1) / per;
 }
 
-hipError_t launch_eval(const float* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
+hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
                        const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh,
-                       int exp_w, const double* dw, const int32_t* inv, float* B0, double* partial, hipStream_t s) {
+                       int exp_w, const double* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s) {
   const uint32_t lim = sh.nr - sh.bs;
   const uint32_t nblk = eval_blocks(sh, nvar_run);
-#define PG_EV(B, N, L, LS)                                                                                           \
+#define PG_EV(B, N, L, LS)                                                                                       \
   if (sh.bs == B && sh.nw == N) {                                                                              \
-    hipLaunchKernelGGL((k_eval<B, N, L>), dim3(nblk, na), dim3(256), 0, s, factors, pos, sel, meta, wl,       \
-                       nvar_stored, nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, \
-                       exp_w, dw, inv, B0, partial);                                                           \
+    if (sh.fp64)                                                                                               \
+      hipLaunchKernelGGL((k_eval<B, N, L, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)factors,    \
+                         pos, sel, meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw,    \
+                         sh.vol, lim, sh.fstride, exp_w, dw, inv, (double*)B0, partial);                       \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_eval<B, N, L, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors, pos, \
+                         sel, meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, \
+                         lim, sh.fstride, exp_w, dw, inv, (float*)B0, partial);                                \
     return hipGetLastError();                                                                                  \
   }
   PG_SHAPES(PG_EV)
@@ -1220,14 +1267,18 @@ uint32_t gather_blocks(const Shape& sh, int mode) {
   return ((mode == 1 ? sh.fs : sh.vol) + 255) / 256;
 }
 
-hipError_t launch_gather(const float* B0, const int32_t* off, const double* invw, const double* u64,
+hipError_t launch_gather(const void* B0, const int32_t* off, const double* invw, const double* u64,
                          const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int mode, double* out,
                          double* partial, hipStream_t s) {
   const uint32_t nblk = gather_blocks(sh, mode);
 #define PG_GA(B, K)                                                                                              \
   case B:                                                                                                      \
-    hipLaunchKernelGGL(K<B>, dim3(nblk, na), dim3(256), 0, s, B0, off, invw, u64, meta, wl, sh.nr, sh.fs,       \
-                       sh.nw, sh.vol, sh.n1, sh.p_sel, mode, out, partial);                                    \
+    if (sh.fp64)                                                                                               \
+      hipLaunchKernelGGL((K<B, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)B0, off, invw, u64,    \
+                         meta, wl, sh.nr, sh.fs, sh.nw, sh.vol, sh.n1, sh.p_sel, mode, out, partial);           \
+    else                                                                                                       \
+      hipLaunchKernelGGL((K<B, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)B0, off, invw, u64,      \
+                         meta, wl, sh.nr, sh.fs, sh.nw, sh.vol, sh.n1, sh.p_sel, mode, out, partial);           \
     break;
   switch (sh.bs) {
     PG_GA(2, k_gather_tile) PG_GA(3, k_gather_tile) PG_GA(4, k_gather_tile)

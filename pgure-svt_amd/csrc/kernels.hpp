@@ -20,7 +20,8 @@ struct Shape {
   uint32_t n1;          // nr - bs + 1 (grid positions per axis)
   uint32_t p_all;       // n1 * n1
   uint32_t p_sel;       // selected patches (src/svt.hpp:61-92)
-  uint32_t fstride;     // floats per stored factor block (V^T n x sr, W^T n x m, s n)
+  uint32_t fstride;     // values per stored factor block (V^T n x sr, W^T n x m, s n)
+  bool fp64 = false;    // SVD input, factors and reconstructions in FP64 (precision study)
 };
 
 bool svd_shape_supported(uint32_t bs, uint32_t nw);
@@ -29,14 +30,15 @@ hipError_t launch_to_double(const void* raw, int dtype, double* out, size_t n, h
 hipError_t launch_frame_max(const double* X, uint32_t fs, uint32_t nf, double* out, hipStream_t s);
 hipError_t launch_median(const double* X, double* Z, uint32_t nr, uint32_t nc, uint32_t nf, int r, hipStream_t s);
 hipError_t launch_delta2(int64_t seed, uint64_t nvox, uint8_t* out, hipStream_t s);
+// uvar: the SVD input variants, float, or double when sh.fp64
 hipError_t launch_prepare(const double* X, const double* Z, const WinMeta* meta, uint32_t nb, const uint8_t* d2,
                           const Shape& sh, uint32_t nvar, double eps2, double d2neg, double d2pos, double* u64,
-                          double* w64, float* u32, hipStream_t s);
+                          double* w64, void* uvar, hipStream_t s);
 hipError_t launch_fill_ref(const WinMeta* meta, uint32_t nb, const Shape& sh, int32_t* pos, hipStream_t s);
 hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs, uint32_t nb, const Shape& sh,
                        int mw, int32_t* pos, hipStream_t s);
-hipError_t launch_svd(const float* u32, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
-                      const Shape& sh, float* factors, int max_sweeps, float tol, uint32_t* stats, hipStream_t s);
+hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
+                      const Shape& sh, void* factors, int max_sweeps, double tol, uint32_t* stats, hipStream_t s);
 // position buckets of every (window, frame): cnt and off are [nb][nw][n1*n1
 // (+1)], list [nb][nw][p_sel]
 // inv [nb][nw][p_sel]: bucket-sorted place of every patch
@@ -49,14 +51,14 @@ hipError_t launch_weights(const int32_t* off, const uint8_t* d2, uint32_t nb, co
 // bucket order); partial
 // [na][eval_blocks] (divergence sums, only with nvar_run == 3)
 uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run);
-hipError_t launch_eval(const float* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
+hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
                        const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh,
-                       int exp_w, const double* dw, const int32_t* inv, float* B0, double* partial, hipStream_t s);
+                       int exp_w, const double* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s);
 // overlap-add gather: mode 0 risk sums into partial [na][gather_blocks][5]; mode 1
 // final reference frames into out (the output cube); mode 2 the whole
 // window of slot 0 into out
 uint32_t gather_blocks(const Shape& sh, int mode);
-hipError_t launch_gather(const float* B0, const int32_t* off, const double* invw, const double* u64,
+hipError_t launch_gather(const void* B0, const int32_t* off, const double* invw, const double* u64,
                          const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int mode, double* out,
                          double* partial, hipStream_t s);
 // sums [na][6] = (sum Uhat^2, sum U, sum U^2, sum Uhat U,
