@@ -220,7 +220,7 @@ def roofline(prof):
 STAGE_KERNELS = {
     "svd_jacobi": "pgsvt::k_svd<",
     "eval_reconstruct": "pgsvt::k_eval<",
-    "gather_risk": "pgsvt::k_gather_direct<",
+    "gather_risk": "pgsvt::k_gather_",
 }
 
 
