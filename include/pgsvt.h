@@ -104,6 +104,40 @@ int pgsvt_denoise_range(const void* X, int dtype, uint32_t n_rows,
                         uint32_t t_end, double* Y, double* estimates);
 
 /* ------------------------------------------------------------------------
+ * Lambda grid and batches (no reference counterpart: the reference searches
+ * lambda with SBPLX, src/pgure.hpp:196-237).  Each window's PGURE risk is
+ * evaluated at n_lambda candidates log-spaced over [0.01, 10] times a scale
+ * from the median singular value of 256 fixed-sample patches (1 / median^2
+ * with exponential weighting, the median without); the first candidate of
+ * least risk is used.  All GPU work of a batch of windows (decomposition,
+ * grid, risks, choice, final frames) runs as one captured HIP graph; the
+ * SVD's lanes-per-matrix mapping is autotuned per shape on first use.
+ * ------------------------------------------------------------------------ */
+int pgsvt_denoise_grid(const void* X, int dtype, uint32_t n_rows,
+                       uint32_t n_cols, uint32_t n_frames,
+                       const pgsvt_params* p, uint32_t n_lambda, double* Y,
+                       double* estimates);
+
+/* One sequence of a batch: the arguments of pgsvt_denoise, and its status
+ * on return. */
+typedef struct pgsvt_job {
+  const void* X;
+  int dtype;
+  uint32_t n_rows, n_cols, n_frames;
+  pgsvt_params params;
+  double* Y;
+  double* estimates;
+  int status;
+} pgsvt_job;
+
+/* Independent sequences of uneven shapes: `concurrency` at a time (largest
+ * first), each on its own stream; n_lambda > 0 uses the lambda grid, 0 the
+ * reference search.  Returns 0, or the first failing job's status with its
+ * index in the message; every job's own status is set. */
+int pgsvt_denoise_batch(pgsvt_job* jobs, uint32_t n_jobs, uint32_t n_lambda,
+                        uint32_t concurrency);
+
+/* ------------------------------------------------------------------------
  * Learned lambda (no reference counterpart: the reference always searches
  * lambda, src/pgure.hpp:196-237).  A network proposes each window's lambda
  * from n_in = 256 * Nw + 3 features: log(max(s, 1e-6)) of the sorted

@@ -87,9 +87,10 @@ class Profiler {
   explicit Profiler(hipStream_t s) : s_(s) {}
   ~Profiler() { flush(); }
   // time `launch` (which enqueues kernels on the stream) under `name`
+  bool suspended = false;  // set while a graph is captured: launches only
   template <class F>
   void run(const char* name, double units, F launch) {
-    if (!g_prof_on) {
+    if (!g_prof_on || suspended) {
       launch();
       return;
     }
@@ -651,18 +652,14 @@ struct Pipeline {
     max_jobs = 2 * g.Nw;
     jobs_d.alloc((size_t)NB * max_jobs * 4);
     stats.alloc(1);
-    hipchk(hipMemsetAsync(stats.p, 0, stats.bytes(), s), "memset stats");
+    hipchk(launch_fill32(stats.p, stats.n, 0, s), "clear stats");
   }
 
   // volumes, trajectories, counts and factors of the windows in `ms`
-  void decompose(std::vector<WinMeta>& ms, uint32_t nvar_run) {
+  // metadata and ARPS schedules of the batch to the device (synchronous)
+  void upload_batch(const std::vector<WinMeta>& ms) {
     const uint32_t nb = (uint32_t)ms.size();
     hipchk(hipMemcpyAsync(meta_d.p, ms.data(), nb * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
-    hipchk(launch_prepare(Xd.p, median ? Zd.p : nullptr, meta_d.p, nb, d2.p, sh, nvar, eps2, d2neg, d2pos, u64.p,
-                          median ? w64.p : nullptr, uvar.p, s),
-           "prepare");
-    hipchk(hipMemsetAsync(pos.p, 0, (size_t)nb * sh.nw * sh.p_all * sizeof(int32_t), s), "memset pos");
-    hipchk(launch_fill_ref(meta_d.p, nb, sh, pos.p, s), "fill_ref");
     if (p.motion_estimation) {
       std::vector<int16_t> all((size_t)nb * max_jobs * 4, (int16_t)-1);
       for (uint32_t b = 0; b < nb; ++b) {
@@ -672,7 +669,18 @@ struct Pipeline {
       }
       hipchk(hipMemcpyAsync(jobs_d.p, all.data(), all.size() * sizeof(int16_t), hipMemcpyHostToDevice, s),
              "H2D jobs");
-      hipchk(hipStreamSynchronize(s), "jobs sync");  // `all` is pageable and local
+    }
+    hipchk(hipStreamSynchronize(s), "upload sync");  // the host vectors are local
+  }
+
+  // the decomposition of the uploaded batch: kernels only (graph-capturable)
+  void decompose_kernels(uint32_t nb, uint32_t nvar_run, int svd_mapping = 0) {
+    hipchk(launch_prepare(Xd.p, median ? Zd.p : nullptr, meta_d.p, nb, d2.p, sh, nvar, eps2, d2neg, d2pos, u64.p,
+                          median ? w64.p : nullptr, uvar.p, s),
+           "prepare");
+    hipchk(launch_fill32(pos.p, (size_t)nb * sh.nw * sh.p_all, 0, s), "clear pos");
+    hipchk(launch_fill_ref(meta_d.p, nb, sh, pos.p, s), "fill_ref");
+    if (p.motion_estimation) {
       // searches: one per (window pass, block)
       prof->run("arps", (double)nb * (g.Nw - 1) * sh.p_all, [&] {
         hipchk(launch_arps(median ? w64.p : u64.p, jobs_d.p, max_jobs, nb, sh, (int)p.motion_window, pos.p, s),
@@ -687,9 +695,14 @@ struct Pipeline {
     prof->run("svd_jacobi", (double)nb * nvar_run * sh.p_sel, [&] {
       // rotation threshold |c| > tol sqrt(ab): 1e-6 in FP32, 1e-12 in FP64
       hipchk(launch_svd(uvar.p, pos.p, sel_d.p, nb, nvar_run, sh, factors.p, 30, sh.fp64 ? 1e-12 : 1e-6, stats.p,
-                        s),
+                        s, svd_mapping),
              "svd");
     });
+  }
+
+  void decompose(std::vector<WinMeta>& ms, uint32_t nvar_run) {
+    upload_batch(ms);
+    decompose_kernels((uint32_t)ms.size(), nvar_run);
   }
 
   // PGURE risk (src/pgure.hpp:136) from the GPU sums; noise as the driver
@@ -894,10 +907,26 @@ constexpr uint32_t kFeatureSamples = 256;
 
 // What one call computes.  mode 0: the reference driver (PGURE search or the
 // fixed lambda); 1: lambda proposed by the learned network; 2: only the
-// learned-lambda features of each window (no output frames).
+// learned-lambda features of each window (no output frames); 3: PGURE risk
+// over a grid of lambdas per window, least risk chosen, the whole GPU work of
+// a batch captured in one graph.
+
+// Autotuned lanes-per-matrix mapping of the SVD per (patch, window) shape:
+// the first batch of a shape times both mappings, later ones reuse the
+// faster.  Used by the grid mode only (the reference path keeps the fixed
+// mapping, whose results its parity tests pin).
+struct Autotune {
+  std::mutex mu;
+  std::map<std::pair<uint32_t, uint32_t>, int> svd;
+  static Autotune& get() {
+    static Autotune* a = new Autotune;
+    return *a;
+  }
+};
 struct RunSpec {
   uint32_t tb = 0, te = 0;  // output windows [tb, te)
-  int mode = 0;
+  int mode = 0;             // 3: PGURE over a lambda grid, each batch one graph launch
+  uint32_t grid = 64;       // mode 3: lambda candidates per window
   const pgsvt_lambda_model* model = nullptr;
   float* features = nullptr;  // mode 2: [te - tb][n_in]
 };
@@ -914,10 +943,12 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
     return fail(PGSVT_E_ARG, "learned lambda needs a model");
   if (int rc = check_device()) return rc;
   HostTimer total_timer("host_total_call");
-  const bool learned = rs.mode != 0;
+  const bool learned = rs.mode == 1 || rs.mode == 2;
   const uint32_t count = rs.te - rs.tb;
   pgsvt_params pl = *pp;
   if (learned) pl.optimize_pgure = 0;  // no perturbed decompositions, no search
+  if (rs.mode == 3) pl.optimize_pgure = 1;  // the grid needs the risk's perturbed decompositions
+  if (rs.mode == 3 && (rs.grid < 1 || rs.grid > 1024)) return fail(PGSVT_E_ARG, "lambda grid size must be 1..1024");
   Pipeline P;
   P.tb = rs.tb;
   P.host_noise = learned;
@@ -1032,7 +1063,7 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
   DBuf<uint32_t> sample_d;
   DBuf<float> feat_d, mean_d, istd_d, yout_d;
   const uint32_t nlog = kFeatureSamples * g.Nw;
-  if (learned) {
+  if (learned || rs.mode == 3) {
     const std::vector<uint32_t> smp = feature_sample(P.sh.p_sel, kFeatureSamples);
     sample_d.alloc(smp.size());
     hipchk(hipMemcpyAsync(sample_d.p, smp.data(), smp.size() * 4, hipMemcpyHostToDevice, P.s), "H2D sample");
@@ -1050,12 +1081,144 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
     hipchk(hipStreamSynchronize(P.s), "model sync");  // host vectors above are local
   }
 
+  // grid mode: per-batch buffers and the captured graph
+  const uint32_t G = rs.grid;
+  DBuf<double> lam_g, sums_g, risk_g, noise_g;
+  DBuf<uint32_t> wl_g;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  uint32_t graph_nb = 0;
+  struct GraphOwner {
+    hipGraph_t* g;
+    hipGraphExec_t* e;
+    ~GraphOwner() {
+      if (*e) (void)hipGraphExecDestroy(*e);
+      if (*g) (void)hipGraphDestroy(*g);
+    }
+  } graph_owner{&graph, &gexec};
+  int svd_mapping = 0;
+  if (rs.mode == 3) {
+    lam_g.alloc((size_t)P.NB * G);
+    sums_g.alloc((size_t)P.NB * G * 6);
+    risk_g.alloc((size_t)P.NB * G);
+    wl_g.alloc((size_t)P.NB * G);
+    noise_g.alloc((size_t)P.NB * 3);
+  }
+
   int status = 0;
   std::string warn;
   for (uint32_t t0 = rs.tb; t0 < rs.te; t0 += P.NB) {
     const uint32_t nb = std::min(P.NB, rs.te - t0);
     std::vector<WinMeta> ms(nb);
     for (uint32_t b = 0; b < nb; ++b) ms[b] = P.meta_of(t0 + b);
+    if (rs.mode == 3) {
+      std::vector<WindowPrep> prep(nb);
+      P.upload_batch(ms);
+      std::vector<double> nz((size_t)nb * 3);
+      {
+        HostTimer tw("host_noise_wait");
+        for (uint32_t b = 0; b < nb; ++b) {
+          prep[b] = pool->wait(t0 + b - rs.tb);
+          if (!prep[b].err.empty()) throw std::runtime_error(prep[b].err);
+          nz[3 * b] = prep[b].alpha;
+          nz[3 * b + 1] = prep[b].mu;
+          nz[3 * b + 2] = prep[b].sigma;
+        }
+      }
+      std::vector<uint32_t> wlg((size_t)nb * G), wlf(nb);
+      for (uint32_t q = 0; q < nb * G; ++q) wlg[q] = q / G;
+      for (uint32_t b = 0; b < nb; ++b) wlf[b] = b;
+      hipchk(hipMemcpyAsync(noise_g.p, nz.data(), nz.size() * 8, hipMemcpyHostToDevice, P.s), "H2D noise");
+      hipchk(hipMemcpyAsync(wl_g.p, wlg.data(), wlg.size() * 4, hipMemcpyHostToDevice, P.s), "H2D grid list");
+      hipchk(hipMemcpyAsync(P.wl_d.p, wlf.data(), wlf.size() * 4, hipMemcpyHostToDevice, P.s), "H2D list");
+      hipchk(hipStreamSynchronize(P.s), "grid upload sync");
+      if (!gexec) {  // first batch of the call: the SVD mapping (autotuned once per shape)
+        auto& at = Autotune::get();
+        const auto key = std::make_pair(P.sh.bs, P.sh.nw);
+        bool known;
+        {
+          std::lock_guard<std::mutex> lk(at.mu);
+          known = at.svd.count(key) != 0;
+          if (known) svd_mapping = at.svd[key];
+        }
+        if (!known && !P.sh.fp64) {
+          // time both mappings on this batch's own matrices
+          P.decompose_kernels(nb, P.nvar, 0);
+          float ms_map[2] = {0, 0};
+          for (int m = 0; m < 2; ++m) {
+            hipEvent_t e0, e1;
+            hipchk(hipEventCreate(&e0), "event");
+            hipchk(hipEventCreate(&e1), "event");
+            hipchk(hipEventRecord(e0, P.s), "event record");
+            hipchk(launch_svd(P.uvar.p, P.pos.p, P.sel_d.p, nb, P.nvar, P.sh, P.factors.p, 30, 1e-6, P.stats.p, P.s,
+                              m),
+                   "svd autotune");
+            hipchk(hipEventRecord(e1, P.s), "event record");
+            hipchk(hipEventSynchronize(e1), "event sync");
+            hipchk(hipEventElapsedTime(&ms_map[m], e0, e1), "event time");
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+          }
+          svd_mapping = ms_map[1] < ms_map[0] ? 1 : 0;
+          std::lock_guard<std::mutex> lk(at.mu);
+          at.svd[key] = svd_mapping;
+        }
+      }
+      if (!gexec || graph_nb != nb) {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        // one graph: decomposition, lambda grid, risk of every candidate in
+        // chunks of NB slots, argmin, final frames
+        const uint32_t s_at = g.Nw * ((g.Nw + 3) & ~3u) + g.Nw * P.sh.m;
+        const uint32_t slots = nb * G, C = P.NB;
+        const double OoN = 1.0 / ((double)nr * nc * g.Nw);
+        P.prof->suspended = true;
+        hipchk(hipStreamBeginCapture(P.s, hipStreamCaptureModeThreadLocal), "begin capture");
+        P.decompose_kernels(nb, P.nvar, svd_mapping);
+        hipchk(launch_grid_lambdas(P.factors.p, P.nvar, P.sh, s_at, sample_d.p, kFeatureSamples, nb, G,
+                                   p.exponential_weighting, lam_g.p, P.s),
+               "grid lambdas");
+        for (uint32_t c0 = 0; c0 < slots; c0 += C) {
+          const uint32_t na = std::min(C, slots - c0);
+          hipchk(launch_eval(P.factors.p, P.pos.p, P.sel_d.p, P.meta_d.p, wl_g.p + c0, na, P.nvar, 3, P.sh,
+                             p.exponential_weighting, P.dw.p, P.inv.p, P.B0.p, P.partial.p, P.s, lam_g.p + c0),
+                 "grid eval");
+          hipchk(launch_gather(P.B0.p, P.off.p, P.invw.p, P.u64.p, P.meta_d.p, wl_g.p + c0, na, P.sh, 0, nullptr,
+                               P.partial_b.p, P.s),
+                 "grid gather");
+          hipchk(launch_risk_reduce(P.partial.p, eval_blocks(P.sh, 3), P.partial_b.p, P.gather_blocks, na,
+                                    sums_g.p + (size_t)c0 * 6, P.s),
+                 "grid reduce");
+        }
+        hipchk(launch_grid_select(sums_g.p, noise_g.p, nb, G, OoN, P.eps2, lam_g.p, P.meta_d.p, risk_g.p, P.s),
+               "grid select");
+        hipchk(launch_eval(P.factors.p, P.pos.p, P.sel_d.p, P.meta_d.p, P.wl_d.p, nb, P.nvar, 1, P.sh,
+                           p.exponential_weighting, nullptr, P.inv.p, P.B0.p, nullptr, P.s),
+               "eval final");
+        hipchk(launch_gather(P.B0.p, P.off.p, P.invw.p, P.u64.p, P.meta_d.p, P.wl_d.p, nb, P.sh, 1, P.Yout, nullptr,
+                             P.s),
+               "gather final");
+        const hipError_t ce = hipStreamEndCapture(P.s, &graph);
+        P.prof->suspended = false;
+        hipchk(ce, "end capture");
+        hipchk(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "graph instantiate");
+        graph_nb = nb;
+      }
+      P.prof->run("grid_graph", (double)nb * G, [&] { hipchk(hipGraphLaunch(gexec, P.s), "graph launch"); });
+      hipchk(hipMemcpyAsync(ms.data(), P.meta_d.p, nb * sizeof(WinMeta), hipMemcpyDeviceToHost, P.s), "D2H meta");
+      hipchk(hipStreamSynchronize(P.s), "graph sync");
+      P.prof->flush();
+      for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t i = t0 + b - rs.tb;
+        est[i + (size_t)count * 0] = ms[b].lambda;
+        est[i + (size_t)count * 1] = prep[b].alpha;
+        est[i + (size_t)count * 2] = prep[b].mu;
+        est[i + (size_t)count * 3] = prep[b].sigma;
+      }
+      continue;
+    }
     P.decompose(ms, P.nvar);
     std::vector<WindowPrep> prep(nb);
     if (learned) {
@@ -1327,6 +1490,66 @@ int pgsvt_denoise_learned_device(const void* X_dev, int dtype, uint32_t n_rows, 
     rs.model = model;
     return run_core(X_dev, dtype, n_rows, n_cols, n_frames, p, Y_dev, estimates, true, rs);
   });
+}
+
+int pgsvt_denoise_grid(const void* X, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
+                       const pgsvt_params* p, uint32_t n_lambda, double* Y, double* estimates) {
+  return guarded([&] {
+    RunSpec rs;
+    rs.tb = 0;
+    rs.te = n_frames;
+    rs.mode = 3;
+    rs.grid = n_lambda;
+    return run_core(X, dtype, n_rows, n_cols, n_frames, p, Y, estimates, false, rs);
+  });
+}
+
+int pgsvt_denoise_batch(pgsvt_job* jobs, uint32_t n, uint32_t n_lambda, uint32_t concurrency) {
+  if (!jobs && n) return fail(PGSVT_E_ARG, "null job list");
+  // largest first (patches x window x frames), `concurrency` calls at once,
+  // each on its own stream with its share of the host threads
+  std::vector<uint32_t> order(n);
+  for (uint32_t i = 0; i < n; ++i) order[i] = i;
+  auto cost = [&](uint32_t i) {
+    const pgsvt_job& j = jobs[i];
+    const double pr = (double)j.n_rows - j.params.patch_size + 1;
+    return pr * pr * j.params.patch_size * j.params.patch_size * j.params.trajectory_length * j.n_frames;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
+  const uint32_t K = std::max(1u, std::min(concurrency ? concurrency : 1u, std::max(1u, n)));
+  const uint32_t share = std::max(1u, thread_count(-1) / K);
+  std::atomic<uint32_t> next{0};
+  std::vector<std::thread> th;
+  std::mutex mu;
+  int first_rc = 0;
+  std::string first_msg;
+  for (uint32_t k = 0; k < K; ++k)
+    th.emplace_back([&] {
+      for (;;) {
+        const uint32_t q = next.fetch_add(1);
+        if (q >= n) return;
+        pgsvt_job& j = jobs[order[q]];
+        pgsvt_params pp = j.params;
+        if (pp.n_jobs < 0) pp.n_jobs = share;
+        RunSpec rs;
+        rs.tb = 0;
+        rs.te = j.n_frames;
+        rs.mode = n_lambda ? 3 : 0;
+        rs.grid = n_lambda ? n_lambda : 64;
+        j.status = guarded([&] { return run_core(j.X, j.dtype, j.n_rows, j.n_cols, j.n_frames, &pp, j.Y,
+                                                 j.estimates, false, rs); });
+        if (j.status) {
+          std::lock_guard<std::mutex> lk(mu);
+          if (!first_rc) {
+            first_rc = j.status;
+            first_msg = "job " + std::to_string(order[q]) + ": " + pgsvt_last_error();
+          }
+        }
+      }
+    });
+  for (auto& t : th) t.join();
+  if (first_rc) return fail(first_rc, first_msg);
+  return 0;
 }
 
 void pgsvt_profile_enable(int on) {

@@ -659,16 +659,30 @@ __global__ void k_bucket_inverse(const int32_t* __restrict__ list, uint32_t p_se
   inv[tab * p_sel + list[tab * p_sel + j]] = (int32_t)j;
 }
 
+// 32-bit fill (stands in for memset so the launch sequence can be captured
+// in a graph)
+__global__ void k_fill32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+hipError_t launch_fill32(void* p, uint64_t n, uint32_t v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill32, dim3(blocks), dim3(256), 0, s, (uint32_t*)p, n, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_buckets(const int32_t* pos, const uint32_t* sel, uint32_t nb, const Shape& sh, int32_t* cnt,
                           int32_t* off, int32_t* list, int32_t* inv, hipStream_t s) {
   const uint32_t npos = sh.n1 * sh.n1;
   const size_t tabs = (size_t)nb * sh.nw;
-  hipError_t e = hipMemsetAsync(cnt, 0, tabs * npos * sizeof(int32_t), s);
+  hipError_t e = launch_fill32(cnt, tabs * npos, 0, s);
   if (e != hipSuccess) return e;
   const dim3 g1((sh.p_sel + 255) / 256, sh.nw, nb);
   hipLaunchKernelGGL(k_bucket_count, g1, dim3(256), 0, s, pos, sel, sh.p_sel, sh.p_all, sh.nw, sh.n1, cnt);
   hipLaunchKernelGGL(k_bucket_scan, dim3(sh.nw, nb), dim3(1024), 0, s, cnt, npos, off);
-  e = hipMemsetAsync(cnt, 0, tabs * npos * sizeof(int32_t), s);
+  e = launch_fill32(cnt, tabs * npos, 0, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_bucket_fill, g1, dim3(256), 0, s, pos, sel, sh.p_sel, sh.p_all, sh.nw, sh.n1, off, cnt, list);
   hipLaunchKernelGGL(k_bucket_sort, dim3((npos + 255) / 256, (unsigned)tabs), dim3(256), 0, s, off, npos, sh.p_sel,
@@ -749,7 +763,7 @@ __global__ __launch_bounds__(256) void k_eval(
     const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl, uint32_t nvar_stored, uint32_t nvar_run,
     uint32_t p_sel, uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim,
     uint32_t fstride, int exp_w, const double* __restrict__ dw, const int32_t* __restrict__ inv,
-    T* __restrict__ B0, double* __restrict__ partial) {
+    T* __restrict__ B0, double* __restrict__ partial, const double* __restrict__ slot_lam) {
   constexpr int M = BS * BS, G = 256 / L, SR = (N + 3) & ~3, P3 = G / 3;
   __shared__ T xch[P3][3][M][N];  // reconstructions of the three variants
   __shared__ double red[4];
@@ -770,7 +784,7 @@ __global__ __launch_bounds__(256) void k_eval(
   }
   const bool valid = var >= 0 && p < p_sel;
   const bool row = valid && lane < M;
-  const double lam = meta[b].lambda;
+  const double lam = slot_lam ? slot_lam[slot] : meta[b].lambda;
   const T* F = factors + (((size_t)b * nvar_stored + (valid ? var : 0)) * p_sel + (valid ? p : 0)) * fstride;
   const T* FW = F + N * SR;
   const T* FS = FW + N * M;
@@ -1198,6 +1212,25 @@ __global__ __launch_bounds__(256) void k_risk_reduce(const double* __restrict__ 
   X(2, 3, 16, 4) X(3, 3, 16, 4) X(3, 5, 16, 4) X(3, 7, 16, 4) X(3, 9, 16, 4) \
   X(6, 7, 64, 16) X(6, 15, 64, 16) X(6, 31, 64, 32) X(8, 7, 64, 16) X(8, 15, 64, 32) X(8, 31, 64, 64)
 
+// second lanes-per-matrix mapping of the SVD for each shape, timed against
+// the first by the autotuner of the batched path
+#define PG_SVD_ALT(X) \
+  X(4, 3, 8) X(4, 5, 8) X(4, 7, 8) X(4, 9, 8) X(4, 11, 8) X(4, 13, 8) X(4, 15, 8) \
+  X(2, 3, 2) X(3, 3, 8) X(3, 5, 8) X(3, 7, 8) X(3, 9, 8) \
+  X(6, 7, 32) X(6, 15, 32) X(6, 31, 64) X(8, 7, 32) X(8, 15, 64) X(8, 31, 32)
+
+uint32_t svd_lanes(uint32_t bs, uint32_t nw, int which) {
+#define PG_L1(B, N, LE, L) \
+  if (which == 0 && bs == B && nw == N) return L;
+  PG_SHAPES(PG_L1)
+#undef PG_L1
+#define PG_L2(B, N, L) \
+  if (which == 1 && bs == B && nw == N) return L;
+  PG_SVD_ALT(PG_L2)
+#undef PG_L2
+  return 0;
+}
+
 bool svd_shape_supported(uint32_t bs, uint32_t nw) {
 #define PG_CHK(B, N, L, LS) \
   if (bs == B && nw == N) return true;
@@ -1207,11 +1240,20 @@ bool svd_shape_supported(uint32_t bs, uint32_t nw) {
 }
 
 hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
-                      const Shape& sh, void* factors, int max_sweeps, double tol, uint32_t* stats, hipStream_t s) {
+                      const Shape& sh, void* factors, int max_sweeps, double tol, uint32_t* stats, hipStream_t s,
+                      int mapping) {
   const uint32_t lim = sh.nr - sh.bs;
+  const uint32_t groups = nvar * sh.p_sel;
+#define PG_SVD_F32(B, N, L)                                                                                      \
+  {                                                                                                            \
+    const dim3 grid((groups * L + 255) / 256, nb);                                                             \
+    hipLaunchKernelGGL((k_svd<B, N, L, float>), grid, dim3(256), 0, s, (const float*)uvar, pos, sel, nvar,     \
+                       sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps,            \
+                       (float)(tol * tol), (float*)factors, stats);                                            \
+    return hipGetLastError();                                                                                  \
+  }
 #define PG_SVD(B, N, LE, L)                                                                                      \
-  if (sh.bs == B && sh.nw == N) {                                                                              \
-    const uint32_t groups = nvar * sh.p_sel;                                                                   \
+  if (sh.bs == B && sh.nw == N && (sh.fp64 || mapping == 0)) {                                                 \
     if (sh.fp64) {                                                                                             \
       /* FP64: at least 16 lanes per matrix (fewer rows per lane fit the registers) */                        \
       constexpr int LD = (L < 16) ? 16 : L;                                                                    \
@@ -1219,16 +1261,17 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
       hipLaunchKernelGGL((k_svd<B, N, LD, double>), grid, dim3(256), 0, s, (const double*)uvar, pos, sel, nvar, \
                          sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps, tol * tol, \
                          (double*)factors, stats);                                                             \
-    } else {                                                                                                   \
-      const dim3 grid((groups * L + 255) / 256, nb);                                                           \
-      hipLaunchKernelGGL((k_svd<B, N, L, float>), grid, dim3(256), 0, s, (const float*)uvar, pos, sel, nvar,   \
-                         sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps,          \
-                         (float)(tol * tol), (float*)factors, stats);                                          \
+      return hipGetLastError();                                                                                \
     }                                                                                                          \
-    return hipGetLastError();                                                                                  \
+    PG_SVD_F32(B, N, L)                                                                                        \
   }
   PG_SHAPES(PG_SVD)
 #undef PG_SVD
+#define PG_SVD2(B, N, L)                                                                                         \
+  if (sh.bs == B && sh.nw == N && mapping == 1) PG_SVD_F32(B, N, L)
+  PG_SVD_ALT(PG_SVD2)
+#undef PG_SVD2
+#undef PG_SVD_F32
   return hipErrorInvalidValue;
 }
 
@@ -1240,7 +1283,8 @@ uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run) {
 
 hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
                        const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh,
-                       int exp_w, const double* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s) {
+                       int exp_w, const double* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s,
+                       const double* slot_lam) {
   const uint32_t lim = sh.nr - sh.bs;
   const uint32_t nblk = eval_blocks(sh, nvar_run);
 #define PG_EV(B, N, L, LS)                                                                                       \
@@ -1248,11 +1292,11 @@ hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* 
     if (sh.fp64)                                                                                               \
       hipLaunchKernelGGL((k_eval<B, N, L, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)factors,    \
                          pos, sel, meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw,    \
-                         sh.vol, lim, sh.fstride, exp_w, dw, inv, (double*)B0, partial);                       \
+                         sh.vol, lim, sh.fstride, exp_w, dw, inv, (double*)B0, partial, slot_lam);             \
     else                                                                                                       \
       hipLaunchKernelGGL((k_eval<B, N, L, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors, pos, \
                          sel, meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, \
-                         lim, sh.fstride, exp_w, dw, inv, (float*)B0, partial);                                \
+                         lim, sh.fstride, exp_w, dw, inv, (float*)B0, partial, slot_lam);                      \
     return hipGetLastError();                                                                                  \
   }
   PG_SHAPES(PG_EV)
@@ -1286,6 +1330,96 @@ hipError_t launch_gather(const void* B0, const int32_t* off, const double* invw,
     default: return hipErrorInvalidValue;
   }
 #undef PG_GA
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- lambda grid --
+// Per window b: G thresholds log-spaced over [0.01, 10] times a scale from
+// the median of the sorted singular values of `nsamp` sampled patches (1 /
+// median^2 with exponential weighting, whose threshold acts on s^2, the
+// median itself without).  One block per window; the values are sorted in
+// LDS (bitonic, padded with +inf) and the median is the mean of the middle
+// pair for an even count.
+template <class T>
+__global__ __launch_bounds__(256) void k_grid_lambdas(const T* __restrict__ factors, uint32_t nvar_stored,
+                                                      uint32_t p_sel, uint32_t fstride, uint32_t s_at, uint32_t n,
+                                                      const uint32_t* __restrict__ sample, uint32_t nsamp,
+                                                      uint32_t G, int exp_w, double* __restrict__ lam) {
+  constexpr int CAPV = 8192;
+  __shared__ float v[CAPV];
+  const uint32_t b = blockIdx.x, cnt = min(nsamp * n, (uint32_t)CAPV);
+  uint32_t P2 = 1;
+  while (P2 < cnt) P2 <<= 1;
+  for (uint32_t e = threadIdx.x; e < P2; e += blockDim.x)
+    v[e] = (e < cnt) ? (float)factors[((size_t)b * nvar_stored * p_sel + sample[e / n]) * fstride + s_at + e % n]
+                     : INFINITY;
+  __syncthreads();
+  for (uint32_t k = 2; k <= P2; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t e = threadIdx.x; e < P2; e += blockDim.x) {
+        const uint32_t q = e ^ j;
+        if (q > e) {
+          const bool up = (e & k) == 0;
+          const float a = v[e], c = v[q];
+          if ((a > c) == up) {
+            v[e] = c;
+            v[q] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  const double med = (cnt % 2) ? (double)v[cnt / 2] : 0.5 * ((double)v[cnt / 2 - 1] + (double)v[cnt / 2]);
+  const double m = fmax(med, 1e-12);
+  const double scale = exp_w ? 1.0 / (m * m) : m;
+  for (uint32_t j = threadIdx.x; j < G; j += blockDim.x)
+    lam[(size_t)b * G + j] = scale * pow(10.0, -2.0 + 3.0 * (double)j / (double)(G > 1 ? G - 1 : 1));
+}
+
+// PGURE risk of every (window, candidate) from the GPU sums, as the driver
+// evaluates it (src/pgure.hpp:136, mu and sigma swapped at the constructor
+// call, the delta1 term zero), then the first candidate of least risk
+// becomes the window's lambda.  One thread per window.
+__global__ void k_grid_select(const double* __restrict__ sums, const double* __restrict__ noise, uint32_t nb,
+                              uint32_t G, double OoN, double eps2, const double* __restrict__ lam,
+                              WinMeta* __restrict__ meta, double* __restrict__ risk) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const double alpha = noise[3 * b], mup = noise[3 * b + 2], sigp = noise[3 * b + 1];
+  const double sigmasq = sigp * sigp;
+  uint32_t best = 0;
+  double rbest = 0;
+  for (uint32_t j = 0; j < G; ++j) {
+    const double* S = sums + ((size_t)b * G + j) * 6;
+    const double S1 = S[0] - 2.0 * S[3] + S[2];
+    const double r =
+        OoN * (S1 - (alpha + mup) * S[1] - (2 * sigmasq * alpha / (eps2 * eps2) * S[4]) + (2 * mup * S[5]) + mup) -
+        sigmasq;
+    risk[(size_t)b * G + j] = r;
+    if (j == 0 || r < rbest) {
+      rbest = r;
+      best = j;
+    }
+  }
+  meta[b].lambda = lam[(size_t)b * G + best];
+}
+
+hipError_t launch_grid_lambdas(const void* factors, uint32_t nvar_stored, const Shape& sh, uint32_t s_at,
+                               const uint32_t* sample, uint32_t nsamp, uint32_t nb, uint32_t G, int exp_w,
+                               double* lam, hipStream_t s) {
+  if (sh.fp64)
+    hipLaunchKernelGGL(k_grid_lambdas<double>, dim3(nb), dim3(256), 0, s, (const double*)factors, nvar_stored,
+                       sh.p_sel, sh.fstride, s_at, sh.nw, sample, nsamp, G, exp_w, lam);
+  else
+    hipLaunchKernelGGL(k_grid_lambdas<float>, dim3(nb), dim3(256), 0, s, (const float*)factors, nvar_stored,
+                       sh.p_sel, sh.fstride, s_at, sh.nw, sample, nsamp, G, exp_w, lam);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_select(const double* sums, const double* noise, uint32_t nb, uint32_t G, double OoN,
+                              double eps2, const double* lam, WinMeta* meta, double* risk, hipStream_t s) {
+  hipLaunchKernelGGL(k_grid_select, dim3((nb + 63) / 64), dim3(64), 0, s, sums, noise, nb, G, OoN, eps2, lam, meta,
+                     risk);
   return hipGetLastError();
 }
 

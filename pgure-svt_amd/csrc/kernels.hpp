@@ -37,8 +37,12 @@ hipError_t launch_prepare(const double* X, const double* Z, const WinMeta* meta,
 hipError_t launch_fill_ref(const WinMeta* meta, uint32_t nb, const Shape& sh, int32_t* pos, hipStream_t s);
 hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs, uint32_t nb, const Shape& sh,
                        int mw, int32_t* pos, hipStream_t s);
+// mapping 0: the default lanes per matrix of the shape, 1: the alternative
+// (FP32 only; see svd_lanes)
 hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
-                      const Shape& sh, void* factors, int max_sweeps, double tol, uint32_t* stats, hipStream_t s);
+                      const Shape& sh, void* factors, int max_sweeps, double tol, uint32_t* stats, hipStream_t s,
+                      int mapping = 0);
+uint32_t svd_lanes(uint32_t bs, uint32_t nw, int mapping);
 // position buckets of every (window, frame): cnt and off are [nb][nw][n1*n1
 // (+1)], list [nb][nw][p_sel]
 // inv [nb][nw][p_sel]: bucket-sorted place of every patch
@@ -53,7 +57,8 @@ hipError_t launch_weights(const int32_t* off, const uint8_t* d2, uint32_t nb, co
 uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run);
 hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
                        const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh,
-                       int exp_w, const double* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s);
+                       int exp_w, const double* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s,
+                       const double* slot_lam = nullptr);
 // overlap-add gather: mode 0 risk sums into partial [na][gather_blocks][5]; mode 1
 // final reference frames into out (the output cube); mode 2 the whole
 // window of slot 0 into out
@@ -88,5 +93,16 @@ hipError_t launch_gather_rows(const float* X, const uint32_t* idx, uint32_t rows
 hipError_t launch_features(const float* factors, uint32_t nvar_stored, uint32_t p_sel, uint32_t fstride,
                            uint32_t s_at, uint32_t n, const uint32_t* sample, uint32_t nsamp, uint32_t nb, uint32_t ld,
                            const float* mean, const float* inv_std, float* feat, hipStream_t s);
+
+// fill n 32-bit words with v (graph-capturable, unlike a memset node)
+hipError_t launch_fill32(void* p, uint64_t n, uint32_t v, hipStream_t s);
+// lambda grid: G candidates per window from the sampled patches' median
+// singular value; then risk of every candidate from the sums [nb*G][6] and
+// the argmin into meta[b].lambda (risk [nb*G] kept)
+hipError_t launch_grid_lambdas(const void* factors, uint32_t nvar_stored, const Shape& sh, uint32_t s_at,
+                               const uint32_t* sample, uint32_t nsamp, uint32_t nb, uint32_t G, int exp_w,
+                               double* lam, hipStream_t s);
+hipError_t launch_grid_select(const double* sums, const double* noise, uint32_t nb, uint32_t G, double OoN,
+                              double eps2, const double* lam, WinMeta* meta, double* risk, hipStream_t s);
 
 }  // namespace pgsvt
