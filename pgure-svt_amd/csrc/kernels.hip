@@ -423,8 +423,15 @@ __device__ __forceinline__ float dpp_f(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
 }
 
+// The empty asm statement on entry makes v an opaque, already rounded
+// value.  Without it the compiler contracts the caller's product into the
+// first butterfly add, fma(x_i, x_j, neighbour's rounded product), which
+// rounds differently in the two lanes of a pair: the lanes then hold
+// different sums and apply different rotations (seen as singular values
+// off by 1e-2 of s_max on near-degenerate column pairs).
 template <int L>
 __device__ __forceinline__ float group_sum(float v) {
+  asm volatile("" : "+v"(v));
   if (L >= 2) v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
   if (L >= 4) v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
   if (L >= 8) v += dpp_f<0x141>(v);  // row_half_mirror
@@ -438,6 +445,7 @@ __device__ __forceinline__ float group_sum(float v) {
 // (every lane of the group ends with the bit-identical sum)
 template <int L>
 __device__ __forceinline__ double group_sum(double v) {
+  asm volatile("" : "+v"(v));
 #pragma unroll
   for (int o = 1; o < L; o <<= 1) v += __shfl_xor(v, o);
   return v;

@@ -132,3 +132,50 @@ def test_denoise_pgure_motion_on(gpu, seq64):
     same = lam_rel == 0
     log("  windows with identical lambda: %d / %d" % (same.sum(), len(same)))
     assert lam_rel.max() < 1e-2
+
+
+# Every other instantiated (patch, window) family, at a size the oracle
+# finishes quickly: fixed-lambda denoise of the whole sequence, and the
+# PGURE risk of one window at three thresholds.  Patch 8 with step 2 is the
+# reference's non-grid patch subset (src/svt.hpp:61-92).
+SHAPES = [
+    (2, 1, 3, False),
+    (3, 1, 5, True),
+    (3, 1, 9, True),
+    (4, 2, 15, True),
+    (4, 1, 7, False),
+    (6, 1, 7, True),
+    (6, 2, 15, True),
+    (8, 2, 7, True),
+    (8, 2, 15, True),
+    (8, 1, 31, True),
+]
+
+
+@pytest.fixture(scope="module")
+def seq64x40():
+    clean, noisy = phantom.noisy_sequence(64, 40, drift=1, alpha=1.0, sigma=6.0, seed=12)
+    return noisy
+
+
+@pytest.mark.parametrize("bs,step,T,expw", SHAPES)
+def test_shapes_fixed_lambda_and_risk(gpu, seq64x40, bs, step, T, expw):
+    log("shape", bs, step, T, expw)
+    kw = dict(patch_size=bs, patch_overlap=step, trajectory_length=T, exponential_weighting=expw)
+    for t in (0, 20, 39):  # ARPS trajectories bit-exact at this block size
+        tg = G.window_trajectories(seq64x40, t, **kw)
+        to = O.window_trajectories(seq64x40, t, **kw)
+        log("  trajectories t=%d: %d of %d differ" % (t, int((tg != to).any(-1).sum()), tg.shape[0] * tg.shape[1]))
+        assert np.array_equal(tg, to)
+    lam = 0.5 if expw else 0.05
+    Yg, eg = G.denoise(seq64x40, optimize_pgure=False, lambda1=lam, **kw)
+    Yo, eo = O.denoise(seq64x40, optimize_pgure=0, lambda1=lam, **kw)
+    e = G.rel_l2(Yg, Yo)
+    log("  frames rel L2 =", e)
+    assert e < FRAME_TOL
+    t = 20
+    noise = [0.02, 0.03, 0.04]
+    rg = G.window_risk(seq64x40, t, noise, [lam / 3, lam, 3 * lam], **kw)
+    ro = O.window_risk(seq64x40, t, noise, [lam / 3, lam, 3 * lam], **kw)
+    log("  risk rel =", np.abs(rg - ro) / np.abs(ro))
+    assert np.all(np.abs(rg - ro) <= RISK_TOL * np.abs(ro))
