@@ -137,16 +137,19 @@ class Profiler {
 };
 
 // wall-clock time of a host phase, recorded next to the kernel timings
-void prof_host(const char* name, double ms) {
+// host-side entry: wall time `ms` and `units` of work (a pure count when
+// ms is 0, such as the search's repeated lambdas)
+void prof_host(const char* name, double ms, double units = 0) {
   std::lock_guard<std::mutex> g(g_prof_mu);
   if (!g_prof_on) return;
   for (auto& e : g_prof)
     if (e.name == name) {
       e.launches++;
       e.ms += ms;
+      e.units += units;
       return;
     }
-  g_prof.push_back(ProfEntry{name, 1, ms, 0});
+  g_prof.push_back(ProfEntry{name, 1, ms, units});
 }
 
 struct HostTimer {
@@ -1286,13 +1289,33 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
       };
       HostTimer tr("host_search_rounds");
       std::vector<double> sums;
+      // Risks already evaluated, per window.  The search asks for the very
+      // same lambda again (a reflection clamped at the bound, a contraction
+      // onto a vertex): about a quarter of its requests.  The risk of a
+      // window is a deterministic function of lambda (fixed-order sums), so
+      // a repeat is answered from here with the identical value.
+      std::vector<std::vector<std::pair<double, double>>> seen(nb);
+      auto recall = [&](uint32_t b, double l, double& r) {
+        for (const auto& e : seen[b])
+          if (e.first == l) {
+            r = e.second;
+            return true;
+          }
+        return false;
+      };
+      double repeats = 0;
       for (;;) {
         for (uint32_t b = 0; b < nb; ++b)
           if (!started[b] && pool->ready(t0 + b - rs.tb)) admit(b);
         std::vector<uint32_t> list;
         for (uint32_t b = 0; b < nb; ++b) {
           if (!started[b] || code[b] != Sbplx1D::kNeed) continue;
-          const int rc = opt[b]->run();
+          int rc;
+          double r;
+          while ((rc = opt[b]->run()) == Sbplx1D::kNeed && recall(b, opt[b]->pending(), r)) {
+            opt[b]->supply(r);
+            repeats += 1;
+          }
           if (rc == Sbplx1D::kNeed) {
             ms[b].lambda = opt[b]->pending();
             list.push_back(b);
@@ -1311,8 +1334,14 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
           continue;
         }
         P.eval_risk(ms, list, sums);
-        for (size_t k = 0; k < list.size(); ++k) opt[list[k]]->supply(P.risk(&sums[k * 6], prep[list[k]]));
+        for (size_t k = 0; k < list.size(); ++k) {
+          const uint32_t b = list[k];
+          const double r = P.risk(&sums[k * 6], prep[b]);
+          seen[b].emplace_back(ms[b].lambda, r);
+          opt[b]->supply(r);
+        }
       }
+      prof_host("search_repeats", 0.0, repeats);
       for (uint32_t b = 0; b < nb; ++b) {
         ms[b].lambda = opt[b]->last_x();
         if (code[b] < 0) {

@@ -473,7 +473,7 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
   const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / L;
   const uint32_t var = g / p_sel, p = g % p_sel;
   const bool valid = var < nvar;  // uniform per group
-  const uint32_t it = valid ? sel[p] : 0;
+  const uint32_t it = valid ? (p_sel == p_all ? p : sel[p]) : 0;  // step 1: identity selection
 
   T x[R][N], v[VR][N];
   const T* U = u32 + ((size_t)b * nvar + (valid ? var : 0)) * vol;
@@ -735,6 +735,44 @@ hipError_t launch_weights(const int32_t* off, const uint8_t* d2, uint32_t nb, co
   return hipGetLastError();
 }
 
+// Lane K of each 16-lane row of the wave, in every lane of that row (DPP
+// row_newbcast; FP64 as two 32-bit halves).
+template <int K>
+__device__ __forceinline__ float row_bcast(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + K, 0xF, 0xF, true));
+}
+template <int K>
+__device__ __forceinline__ double row_bcast(double v) {
+  const int2 h = __builtin_bit_cast(int2, v);
+  int2 o;
+  o.x = __builtin_amdgcn_mov_dpp(h.x, 0x150 + K, 0xF, 0xF, true);
+  o.y = __builtin_amdgcn_mov_dpp(h.y, 0x150 + K, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, o);
+}
+
+// bq[k] += y * (element k of v's row), k < N
+template <int K, int N, class T>
+__device__ __forceinline__ void axpy_row(T y, T v, T* bq) {
+  if constexpr (K < N) {
+    bq[K] = fma(y, row_bcast<K>(v), bq[K]);
+    axpy_row<K + 1, N>(y, v, bq);
+  }
+}
+
+// ranks R, R + 1, ... < nkeep of a reconstruction row: y = (g(s_r)/s_r,
+// held by lane r) * w_r[q], then bq += y v_r^T; same operations and order
+// as the general rank loop of k_eval.
+template <int R, int N, class T>
+__device__ __forceinline__ void rank_products(int nkeep, T dl, const T* vr, const T* wr, T* bq) {
+  if constexpr (R < N) {
+    if (R < nkeep) {
+      const T y = row_bcast<R>(dl) * wr[R];
+      axpy_row<0, N>(y, vr[R], bq);
+      rank_products<R + 1, N>(nkeep, dl, vr, wr, bq);
+    }
+  }
+}
+
 // Workgroups of a launch are handed to the 8 XCDs round-robin by linear
 // id, and each XCD has its own L2.  xcd_block maps the linear id so that XCD
 // x runs one contiguous range of logical (x, y) blocks: neighbouring blocks,
@@ -806,7 +844,8 @@ __global__ __launch_bounds__(256) void k_eval(
     sl = FS[lane];
     smax = FS[0];
   }
-  const uint32_t it = pg ? sel[p] : 0;
+  // every patch selected (step 1): the selection is the identity
+  const uint32_t it = pg ? (p_sel == p_all ? p : sel[p]) : 0;
   int32_t ib[KP], pk[KP];
   if (pg) {
     const int32_t* I = inv + (size_t)b * nw * p_sel + p;
@@ -830,6 +869,25 @@ __global__ __launch_bounds__(256) void k_eval(
   T bq[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) bq[k] = T(0);
+  if constexpr (L == 16) {
+    // A group is one 16-lane DPP row.  Lane j loads element j of every
+    // kept V^T row and its own W^T element, all before the first product
+    // (one memory wait for the whole rank loop); the products take V^T
+    // elements and g(s_r)/s_r from their lanes by row broadcasts.
+    // Entries r >= nkeep stay unset and are never read.  Lanes past the
+    // row length read an in-bounds element that is never broadcast (V^T)
+    // or stored (W^T: lanes >= M hold no row of B).
+    T vr[N], wr[N];
+    const int lv = min(lane, SR - 1), lw = min(lane, M - 1);
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      if (r < nkeep) {
+        vr[r] = F[r * SR + lv];
+        wr[r] = FW[r * M + lw];
+      }
+    }
+    rank_products<0, N>(nkeep, dl, vr, wr, bq);
+  } else {
   // rank loop, loads of rank r + 1 issued before the products of rank r
   // (FP32: the V^T row as SR/4 float4 loads; FP64: SR scalar loads)
   constexpr int VW = (sizeof(T) == 4) ? 4 : 1, NV = SR / VW;
@@ -861,6 +919,7 @@ __global__ __launch_bounds__(256) void k_eval(
         if (j < N) bq[j] = fma(y, (T)vc[j], bq[j]);
       }
     }
+  }
   }
   double s4 = 0.0;
   if (nvar_run == 3) {
@@ -1071,29 +1130,54 @@ __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, c
   const int prlo = max(r0 - (BS - 1), 0), prhi = min(r0 + TR - 1, (int)n1 - 1);
   const int pclo = max(c0 - (BS - 1), 0), pchi = min(c0 + TC - 1, (int)n1 - 1);
   const int ncol = pchi - pclo + 1, nrow = prhi - prlo + 2;  // nrow counts the end offset
-  for (int e = threadIdx.x; e < ncol * nrow; e += blockDim.x) {
-    const int lc = e / nrow, i = e % nrow;
-    sO[lc][i] = O[(size_t)n1 * (pclo + lc) + prlo + i];
+  // Both staging copies have a fixed trip count per thread and issue all
+  // their loads before the first LDS write, so a thread waits for memory
+  // once per copy instead of once per element.  Slots past the end read an
+  // in-bounds element (the first of the table) and are not written.
+  {
+    constexpr int NO = (NC * NRW + 255) / 256;
+    int32_t ov[NO];
+#pragma unroll
+    for (int s = 0; s < NO; ++s) {
+      const int e = (int)threadIdx.x + 256 * s;
+      const bool use = e < ncol * nrow;
+      const int lc = use ? e / nrow : 0, i = use ? e % nrow : 0;
+      ov[s] = O[(size_t)n1 * (pclo + lc) + prlo + i];
+    }
+#pragma unroll
+    for (int s = 0; s < NO; ++s) {
+      const int e = (int)threadIdx.x + 256 * s;
+      if (e < ncol * nrow) sO[e / nrow][e % nrow] = ov[s];
+    }
   }
   __syncthreads();
   bool fits = true;
   for (int lc = 0; lc < ncol; ++lc) fits = fits && (sO[lc][nrow - 1] - sO[lc][0] <= CAP);
   if (fits) {
-    constexpr int PER = CAP * (M / W);  // staging elements per patch column
-    for (int e = threadIdx.x; e < ncol * PER; e += blockDim.x) {
-      const int lc = e / PER, rem = e % PER, lr = rem / (M / W), q = (rem % (M / W)) * W;
-      const int32_t j0 = sO[lc][0];
-      if (lr < sO[lc][nrow - 1] - j0) {
-        const int dst = lc * CAP + lr;
-        if constexpr (W > 1) {
-          const VT v = *reinterpret_cast<const VT*>(Bk + (size_t)(j0 + lr) * M + q);
-          const T* vv = reinterpret_cast<const T*>(&v);
+    constexpr int PER = CAP * (M / W);            // staging elements per patch column
+    constexpr int NS = (NC * PER + 255) / 256;    // per thread
+    typedef typename std::conditional<(W > 1), VT, T>::type ST;
+    ST sv[NS];
+    int sd[NS];  // LDS row of the element, -1 where nothing is staged
 #pragma unroll
-          for (int w = 0; w < W; ++w) sB[(q + w) * RS + dst] = vv[w];
-        } else {
-          sB[q * RS + dst] = Bk[(size_t)(j0 + lr) * M + q];
-        }
-      }
+    for (int s = 0; s < NS; ++s) {
+      const int e = (int)threadIdx.x + 256 * s;
+      const int ee = e < ncol * PER ? e : 0;
+      const int lc = ee / PER, rem = ee % PER, lr = rem / (M / W), q = (rem % (M / W)) * W;
+      const int32_t j0 = sO[lc][0];
+      const bool use = e < ncol * PER && lr < sO[lc][nrow - 1] - j0;
+      const size_t src = use ? (size_t)(j0 + lr) * M + q : 0;
+      sv[s] = *reinterpret_cast<const ST*>(Bk + src);
+      sd[s] = use ? lc * CAP + lr : -1;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      if (sd[s] < 0) continue;
+      const int e = (int)threadIdx.x + 256 * s;
+      const int q = ((e % PER) % (M / W)) * W;
+      const T* vv = reinterpret_cast<const T*>(&sv[s]);
+#pragma unroll
+      for (int w = 0; w < W; ++w) sB[(q + w) * RS + sd[s]] = vv[w];
     }
   }
   __syncthreads();
