@@ -514,6 +514,10 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
         for (int h = 1; h < R; ++h) cp = fma(x[h][i], x[h][j], cp);
         const T c = group_sum<L>(cp);
         const bool rot = c * c > tol2 * a[i] * a[j];
+        // Late in the iteration most pairs rotate in none of the wave's
+        // matrices; the update is then the identity (t = 0, cs = 1, sn = 0
+        // reproduce every value exactly) and is skipped for the whole wave.
+        if (!__any(rot)) continue;
         const T zeta = (a[j] - a[i]) * rcp_(c + c);
         const T den = fabs(zeta) + sqrt_(fma(zeta, zeta, T(1)));
         const T t = rot ? copysign(rcp_(den), zeta) : T(0);
@@ -844,18 +848,39 @@ __global__ __launch_bounds__(256) void k_eval(
     sl = FS[lane];
     smax = FS[0];
   }
+  // In a PGURE launch every lane issues these loads, lanes without a row
+  // (and the idle group) at in-bounds addresses of patch 0 / row M - 1:
+  // no lane skips a load, which keeps the memory waits below exact.
+  const bool pgl = nvar_run == 3;
+  const uint32_t pa = valid ? p : 0;
+  const int va = max(var, 0);
   // every patch selected (step 1): the selection is the identity
-  const uint32_t it = pg ? (p_sel == p_all ? p : sel[p]) : 0;
+  const uint32_t it = pgl ? (p_sel == p_all ? pa : sel[pa]) : 0;
   int32_t ib[KP], pk[KP];
-  if (pg) {
-    const int32_t* I = inv + (size_t)b * nw * p_sel + p;
+  if (pgl) {
+    const int32_t* I = inv + (size_t)b * nw * p_sel + pa;
 #pragma unroll
     for (int i = 0; i < KP; ++i) {
-      const int k = min(var + 3 * i, N - 1);
+      const int k = min(va + 3 * i, N - 1);
       ib[i] = I[(size_t)k * p_sel];
       pk[i] = pos[((size_t)b * nw + k) * p_all + it];
     }
   }
+  // delta2 / W at the voxels of those frames (PGURE), also independent of
+  // the reconstruction.  Issued after the factor loads of the rank loop
+  // (loads complete in order, so the products wait for the factors only).
+  double dwv[KP];
+  auto load_dw = [&] {
+    const int lq = min(lane, M - 1);
+    const int rr = lq % BS, cc = lq / BS;
+    const double* DW = dw + (size_t)b * vol;
+#pragma unroll
+    for (int i = 0; i < KP; ++i) {
+      const int k = min(va + 3 * i, N - 1);
+      const uint32_t r0 = min((uint32_t)(pk[i] & 0xffff), lim), c0 = min((uint32_t)(pk[i] >> 16), lim);
+      dwv[i] = DW[(uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc)];
+    }
+  };
   // shrink factor g(s_r)/s_r in lane r < N; s_0 is the largest
   T dl = T(0);
   if (valid && lane < N) {
@@ -886,7 +911,12 @@ __global__ __launch_bounds__(256) void k_eval(
         wr[r] = FW[r * M + lw];
       }
     }
-    rank_products<0, N>(nkeep, dl, vr, wr, bq);
+    if (pgl) {  // uniform: each path has its own exact waits
+      load_dw();
+      rank_products<0, N>(nkeep, dl, vr, wr, bq);
+    } else {
+      rank_products<0, N>(nkeep, dl, vr, wr, bq);
+    }
   } else {
   // rank loop, loads of rank r + 1 issued before the products of rank r
   // (FP32: the V^T row as SR/4 float4 loads; FP64: SR scalar loads)
@@ -901,6 +931,7 @@ __global__ __launch_bounds__(256) void k_eval(
     for (int j = 0; j < NV; ++j) vn[j] = vr[j];
   };
   if (nkeep > 0) fetch(0);
+  if (pgl) load_dw();
   for (int r = 0; r < nkeep; ++r) {
     const T y = __shfl(dl, r, L) * wn;
     VT vc[NV];
@@ -926,16 +957,7 @@ __global__ __launch_bounds__(256) void k_eval(
     // The three groups of a patch share the per-frame work: group `var`
     // takes frames var, var + 3, ...: the U row goes to its bucket-sorted
     // place in B0 and (B1 - 2 B0 + B2) delta2_v / W_v is accumulated.
-    double dwv[KP];
-    const int rr = lane % BS, cc = lane / BS;
     if (pg) {
-      const double* DW = dw + (size_t)b * vol;
-#pragma unroll
-      for (int i = 0; i < KP; ++i) {
-        const int k = min(var + 3 * i, N - 1);
-        const uint32_t r0 = min((uint32_t)(pk[i] & 0xffff), lim), c0 = min((uint32_t)(pk[i] >> 16), lim);
-        dwv[i] = DW[(uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc)];
-      }
 #pragma unroll
       for (int k = 0; k < N; ++k) xch[pl][var][lane][k] = bq[k];
     }
