@@ -990,6 +990,124 @@ __global__ __launch_bounds__(256) void k_eval(
   if (threadIdx.x == 0) partial[(size_t)slot * gridDim.x + bx] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+// PGURE evaluation for patches up to 4 x 4: one 16-lane group (one DPP
+// row) per patch, lane q = row q of B.  The group reconstructs the three
+// variants U, U + eps2 delta2, U - eps2 delta2 in turn, stores the U row of
+// every frame at its bucket-sorted place in B0 and accumulates
+// sum_k (B1 - 2 B0 + B2)_k delta2_v / W_v in FP64 (src/pgure.hpp:136) in
+// registers: the second difference of FP32 values is exact in FP64, so
+// forming it as (-2 B0 + B1) + B2 across the three passes loses nothing
+// and no exchange between groups is needed.  Lane k holds the trajectory
+// position and bucket place of frame k; each frame takes them by row
+// broadcast.
+template <int K>
+__device__ __forceinline__ int32_t row_bcast(int32_t v) {
+  return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xF, 0xF, true);
+}
+
+// f(std::integral_constant<int, K>) for K = K0 .. K1 - 1, in order
+template <int K0, int K1, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (K0 < K1) {
+    f(std::integral_constant<int, K0>());
+    static_for<K0 + 1, K1>(f);
+  }
+}
+
+template <int BS, int N, class T>
+__global__ __launch_bounds__(256) void k_eval_pg(
+    const T* __restrict__ factors, const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel,
+    const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl, uint32_t nvar_stored, uint32_t p_sel,
+    uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim, uint32_t fstride, int exp_w,
+    const double* __restrict__ dw, const int32_t* __restrict__ inv, T* __restrict__ B0,
+    double* __restrict__ partial, const double* __restrict__ slot_lam) {
+  constexpr int M = BS * BS, SR = (N + 3) & ~3;
+  static_assert(M <= 16 && SR <= 16, "one 16-lane row per patch");
+  __shared__ double red[4];
+  uint32_t bx, slot;
+  xcd_block(bx, slot);
+  const uint32_t b = wl[slot];
+  const int lane = threadIdx.x % 16;
+  const uint32_t p = bx * 16 + threadIdx.x / 16;
+  const bool valid = p < p_sel;
+  const bool row = valid && lane < M;
+  const uint32_t pa = valid ? p : 0;  // addresses of groups past the end: patch 0
+  const double lam = slot_lam ? slot_lam[slot] : meta[b].lambda;
+  const size_t vstep = (size_t)p_sel * fstride;  // one variant's factors to the next
+  const T* F = factors + ((size_t)b * nvar_stored * p_sel + pa) * fstride;
+  // singular values of the three variants (lane r: s_r), then frame
+  // lk = min(lane, N - 1)'s trajectory position and bucket place
+  const int lk = min(lane, N - 1);
+  T sl[3], smax[3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    sl[v] = F[v * vstep + N * SR + N * M + lk];
+    smax[v] = F[v * vstep + N * SR + N * M];
+  }
+  const uint32_t it = p_sel == p_all ? pa : sel[pa];
+  const int32_t pk_l = pos[((size_t)b * nw + lk) * p_all + it];
+  const int32_t ib_l = inv[((size_t)b * nw + lk) * p_sel + pa];
+  const int lv = min(lane, SR - 1), lw = min(lane, M - 1);
+  const int rr = lw % BS, cc = lw / BS;
+  const double* DW = dw + (size_t)b * vol;
+  double t2[N], dwv[N], s4 = 0.0;
+  static_for<0, 3>([&](auto vc) {
+    constexpr int v = decltype(vc)::value;
+    const T* Fv = F + v * vstep;
+    // shrink factor g(s_r)/s_r in lane r < N, kept ranks of the group
+    T dl = T(0);
+    if (valid && lane < N) {
+      const T w = exp_w ? fabs(smax[v] * exp((T)(-0.5 * lam) * (sl[v] * sl[v]))) : (T)lam;
+      dl = sl[v] > T(0) ? fmax(sl[v] - w, T(0)) / sl[v] : T(0);
+    }
+    const uint64_t bal = __ballot(dl > T(0));
+    const uint32_t gbits = (uint32_t)((bal >> ((threadIdx.x % 64) & ~15u)) & 0xffffu);
+    const int nkeep = gbits ? 32 - __clz(gbits) : 0;
+    // kept ranks of V^T (lane j: element j) and W^T (lane q: row q), read
+    // before the first product; entries past nkeep stay unset and unread
+    T vr[N], wr[N];
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      if (r < nkeep) {
+        vr[r] = Fv[r * SR + lv];
+        wr[r] = Fv[N * SR + r * M + lw];
+      }
+    }
+    if constexpr (v == 1) {
+      // delta2 / W at the patch's voxels of every frame, for the last pass
+      static_for<0, N>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const int32_t pkk = row_bcast<k>(pk_l);
+        const uint32_t r0 = min((uint32_t)(pkk & 0xffff), lim), c0 = min((uint32_t)(pkk >> 16), lim);
+        dwv[k] = DW[(uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc)];
+      });
+    }
+    T bq[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) bq[k] = T(0);
+    rank_products<0, N>(nkeep, dl, vr, wr, bq);
+    if constexpr (v == 0) {
+      static_for<0, N>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const int32_t ibk = row_bcast<k>(ib_l);
+        if (row) B0[(((size_t)slot * nw + k) * p_sel + ibk) * M + lane] = bq[k];
+        t2[k] = -2.0 * (double)bq[k];
+      });
+    } else if constexpr (v == 1) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) t2[k] += (double)bq[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < N; ++k)
+        if (row) s4 = fma(t2[k] + (double)bq[k], dwv[k], s4);
+    }
+  });
+  for (int o = 32; o > 0; o >>= 1) s4 += __shfl_xor(s4, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = s4;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[(size_t)slot * gridDim.x + bx] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
 // ----------------------------------------------------------- gather --
 // Uhat_v = (sum of the contributions covering v) / W_v, contributions taken
 // position by position (patch column cb, then row ra) and bucket by bucket
@@ -1391,7 +1509,8 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
 
 uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run) {
   const uint32_t G = (sh.m <= 16) ? 16 : (sh.m <= 32 ? 8 : 4);
-  const uint32_t per = (nvar_run == 3) ? G / 3 : G;
+  // PGURE: one patch per 16-lane group (k_eval_pg), else three groups
+  const uint32_t per = (nvar_run == 3 && sh.m > 16) ? G / 3 : G;
   return (sh.p_sel + per - 1) / per;
 }
 
@@ -1403,6 +1522,20 @@ hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* 
   const uint32_t nblk = eval_blocks(sh, nvar_run);
 #define PG_EV(B, N, L, LS)                                                                                       \
   if (sh.bs == B && sh.nw == N) {                                                                              \
+    if constexpr (L == 16) {                                                                                   \
+      if (nvar_run == 3) {                                                                                     \
+        if (sh.fp64)                                                                                           \
+          hipLaunchKernelGGL((k_eval_pg<B, N, double>), dim3(nblk, na), dim3(256), 0, s,                       \
+                             (const double*)factors, pos, sel, meta, wl, nvar_stored, sh.p_sel, sh.p_all,       \
+                             sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, exp_w, dw, inv, (double*)B0,        \
+                             partial, slot_lam);                                                               \
+        else                                                                                                   \
+          hipLaunchKernelGGL((k_eval_pg<B, N, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors, \
+                             pos, sel, meta, wl, nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, \
+                             lim, sh.fstride, exp_w, dw, inv, (float*)B0, partial, slot_lam);                  \
+        return hipGetLastError();                                                                              \
+      }                                                                                                        \
+    }                                                                                                          \
     if (sh.fp64)                                                                                               \
       hipLaunchKernelGGL((k_eval<B, N, L, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)factors,    \
                          pos, sel, meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw,    \
