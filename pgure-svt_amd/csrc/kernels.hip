@@ -1247,8 +1247,15 @@ __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, c
   constexpr int VW = 16 / (int)sizeof(T);      // values per 16-byte load
   constexpr int W = (M % VW == 0) ? VW : 1;    // values per staging load
   typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type VT;
+  typedef typename std::conditional<(W > 1), VT, T>::type ST;
+  // sB: staged patch rows, one plane per patch row q (plane q, LDS row j);
+  // the pad element of each plane is read (and ignored) by empty buckets.
+  // sT: bucket offsets of the tile's halo, cell (lc, i) = position (row
+  // r0 - (BS-1) + i clamped to [0, n1], column c0 - (BS-1) + lc), all 0 for
+  // columns outside the frame (empty buckets).  A voxel's bounds are then
+  // cells at fixed offsets from one per-thread base.
   __shared__ T sB[M * RS];
-  __shared__ int32_t sO[NC][NRW];  // bucket offsets of the tile's positions (+ one past the last row)
+  __shared__ int32_t sT[NC * NRW];
   __shared__ double sm[5][4];
   uint32_t bx, slot;
   xcd_block(bx, slot);
@@ -1266,103 +1273,96 @@ __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, c
   const uint32_t pix = (uint32_t)min(r, (int)nr - 1) + nr * (uint32_t)min(c, (int)nr - 1), v = k * fs + pix;
   const double iw = invw[(size_t)b * vol + v];
   const double U = (mode == 0) ? u64[(size_t)b * vol + v] : 0.0;
-  // positions that can cover the tile: rows prlo..prhi, columns pclo..pchi
-  const int prlo = max(r0 - (BS - 1), 0), prhi = min(r0 + TR - 1, (int)n1 - 1);
-  const int pclo = max(c0 - (BS - 1), 0), pchi = min(c0 + TC - 1, (int)n1 - 1);
-  const int ncol = pchi - pclo + 1, nrow = prhi - prlo + 2;  // nrow counts the end offset
   // Both staging copies have a fixed trip count per thread and issue all
-  // their loads before the first LDS write, so a thread waits for memory
-  // once per copy instead of once per element.  Slots past the end read an
-  // in-bounds element (the first of the table) and are not written.
+  // their loads before the first LDS write (one memory wait per copy).
   {
     constexpr int NO = (NC * NRW + 255) / 256;
     int32_t ov[NO];
 #pragma unroll
     for (int s = 0; s < NO; ++s) {
-      const int e = (int)threadIdx.x + 256 * s;
-      const bool use = e < ncol * nrow;
-      const int lc = use ? e / nrow : 0, i = use ? e % nrow : 0;
-      ov[s] = O[(size_t)n1 * (pclo + lc) + prlo + i];
+      const int e = min((int)threadIdx.x + 256 * s, NC * NRW - 1);
+      const int col = c0 - (BS - 1) + e / NRW, row = min(max(r0 - (BS - 1) + e % NRW, 0), (int)n1);
+      const int32_t o = O[(size_t)n1 * min(max(col, 0), (int)n1 - 1) + row];
+      ov[s] = (col >= 0 && col < (int)n1) ? o : 0;
     }
 #pragma unroll
     for (int s = 0; s < NO; ++s) {
       const int e = (int)threadIdx.x + 256 * s;
-      if (e < ncol * nrow) sO[e / nrow][e % nrow] = ov[s];
+      if (e < NC * NRW) sT[e] = ov[s];
     }
   }
   __syncthreads();
   bool fits = true;
-  for (int lc = 0; lc < ncol; ++lc) fits = fits && (sO[lc][nrow - 1] - sO[lc][0] <= CAP);
-  if (fits) {
-    constexpr int PER = CAP * (M / W);            // staging elements per patch column
-    constexpr int NS = (NC * PER + 255) / 256;    // per thread
-    typedef typename std::conditional<(W > 1), VT, T>::type ST;
-    ST sv[NS];
-    int sd[NS];  // LDS row of the element, -1 where nothing is staged
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int e = (int)threadIdx.x + 256 * s;
-      const int ee = e < ncol * PER ? e : 0;
-      const int lc = ee / PER, rem = ee % PER, lr = rem / (M / W), q = (rem % (M / W)) * W;
-      const int32_t j0 = sO[lc][0];
-      const bool use = e < ncol * PER && lr < sO[lc][nrow - 1] - j0;
-      const size_t src = use ? (size_t)(j0 + lr) * M + q : 0;
-      sv[s] = *reinterpret_cast<const ST*>(Bk + src);
-      sd[s] = use ? lc * CAP + lr : -1;
+  for (int lc = 0; lc < NC; ++lc) fits = fits && (sT[lc * NRW + NRW - 1] - sT[lc * NRW] <= CAP);
+  if (fits) {
+    // patch column lc: thread -> (row lr, values q .. q + W - 1) of the
+    // column's bucket range, RPI rows per pass
+    constexpr int QW = M / W, RPI = 256 / QW, NH = (CAP + RPI - 1) / RPI;
+    const int lr0 = (int)threadIdx.x / QW, q = ((int)threadIdx.x % QW) * W;
+    ST sv[NC][NH];
+#pragma unroll
+    for (int lc = 0; lc < NC; ++lc) {
+      const int32_t j0 = sT[lc * NRW], cnt = sT[lc * NRW + NRW - 1] - j0;
+#pragma unroll
+      for (int h = 0; h < NH; ++h) {
+        const int lr = lr0 + RPI * h;
+        const bool use = lr0 < RPI && lr < cnt;
+        sv[lc][h] = *reinterpret_cast<const ST*>(Bk + (use ? (size_t)(j0 + lr) * M + q : 0));
+      }
     }
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      if (sd[s] < 0) continue;
-      const int e = (int)threadIdx.x + 256 * s;
-      const int q = ((e % PER) % (M / W)) * W;
-      const T* vv = reinterpret_cast<const T*>(&sv[s]);
+    for (int lc = 0; lc < NC; ++lc) {
+      const int32_t cnt = sT[lc * NRW + NRW - 1] - sT[lc * NRW];
 #pragma unroll
-      for (int w = 0; w < W; ++w) sB[(q + w) * RS + sd[s]] = vv[w];
+      for (int h = 0; h < NH; ++h) {
+        const int lr = lr0 + RPI * h;
+        if (lr0 < RPI && lr < cnt) {
+          const T* vv = reinterpret_cast<const T*>(&sv[lc][h]);
+#pragma unroll
+          for (int w = 0; w < W; ++w) sB[(q + w) * RS + lc * CAP + lr] = vv[w];
+        }
+      }
     }
   }
   __syncthreads();
   double a_q = 0, a_h = 0, a_hu = 0, a_u = 0, a_u2 = 0;
   if (live) {
     double acc = 0.0;
-    // bucket bounds bnd[cb][e] of positions (r - BS + 1 + e, c - cb),
-    // clamped to the frame; e = BS is the end of row r
+    // bucket bounds bnd[cb][e] of positions (r - BS + 1 + e, c - cb); e = BS
+    // is the end of row r.  cb0: the voxel's column base cell.
+    const int tb = ((c - c0) + BS - 1) * NRW + (r - r0);
     int32_t bnd[BS][BS + 1];
-    int lcs[BS];
+    int base[BS];  // bucket index j -> LDS row j + base[cb]
 #pragma unroll
     for (int cb = 0; cb < BS; ++cb) {
-      const int pc = c - cb;
-      const bool in = pc >= 0 && pc < (int)n1;
-      lcs[cb] = in ? pc - pclo : 0;
 #pragma unroll
-      for (int e = 0; e <= BS; ++e) {
-        const int prc = min(max(r - (BS - 1) + e, 0), (int)n1);
-        bnd[cb][e] = in ? sO[lcs[cb]][prc - prlo] : 0;
-      }
+      for (int e = 0; e <= BS; ++e) bnd[cb][e] = sT[tb - cb * NRW + e];
+      base[cb] = ((c - c0) + BS - 1 - cb) * CAP - sT[tb - (r - r0) - cb * NRW];
     }
     if (fits) {
       bool multi = false;
 #pragma unroll
       for (int cb = 0; cb < BS; ++cb) {
-        const int base = lcs[cb] * CAP - sO[lcs[cb]][0];  // bucket index j -> LDS row j + base
 #pragma unroll
         for (int ra = 0; ra < BS; ++ra) {
           const int e = BS - 1 - ra;  // position r - ra
           const int32_t j0 = bnd[cb][e], n = bnd[cb][e + 1] - j0;
-          const int row = min(max(j0 + base, 0), NC * CAP - 1);  // in bounds even for empty buckets
-          const T f = sB[(ra + BS * cb) * RS + row];
-          acc += (n > 0) ? (double)f : 0.0;
+          // an empty bucket reads the first row of the next bucket or the
+          // plane's pad element: in bounds, and not added
+          const T f = sB[(ra + BS * cb) * RS + j0 + base[cb]];
+          acc += (double)((n > 0) ? f : T(0));
           multi = multi || n > 1;
         }
       }
       if (__any(multi)) {
 #pragma unroll
         for (int cb = 0; cb < BS; ++cb) {
-          const int base = lcs[cb] * CAP - sO[lcs[cb]][0];
 #pragma unroll
           for (int ra = 0; ra < BS; ++ra) {
             const int e = BS - 1 - ra;
             for (int32_t j = bnd[cb][e] + 1; j < bnd[cb][e + 1]; ++j)
-              acc += (double)sB[(ra + BS * cb) * RS + j + base];
+              acc += (double)sB[(ra + BS * cb) * RS + j + base[cb]];
           }
         }
       }
