@@ -1224,14 +1224,15 @@ __global__ __launch_bounds__(256) void k_gather_direct(const T* __restrict__ B0,
 // Tiled gather for patches up to 4 x 4.  A block covers a tile of TR x TC
 // voxels of one frame; the patches that can cover the tile sit in one
 // contiguous bucket range per patch column, which the block copies into LDS
-// with coalesced float4 loads: patch column lc gets CAP rows, stored as one
-// plane per patch row (lanes on neighbouring voxels read neighbouring
-// words).  Each thread then sums its voxel: first the leading patch of each
-// of its bs x bs buckets (one LDS read and a select, no branch), then, only
-// in waves where a bucket holds several patches, the remaining ones.  The
-// order is fixed, so the sums are reproducible.  A tile with a column range
-// longer than CAP patches (trajectories piling up) reads global memory in
-// the order of k_gather_direct instead.
+// with coalesced float4 loads, the column ranges packed one after another
+// (TOT rows per tile), stored as one plane per patch row (lanes on
+// neighbouring voxels read neighbouring words).  Each thread then sums its
+// voxel: first the leading patch of each of its bs x bs buckets (one LDS
+// read and a select, no branch), then, only in waves where a bucket holds
+// several patches, the remaining ones.  The order is fixed, so the sums are
+// reproducible.  A tile with more than TOT patch rows, or a column range
+// longer than COLCAP (trajectories piling up), reads global memory in the
+// order of k_gather_direct instead.
 template <int BS, class T>
 __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, const int32_t* __restrict__ off,
                                                      const double* __restrict__ invw,
@@ -1241,21 +1242,24 @@ __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, c
                                                      uint32_t nw, uint32_t vol, uint32_t n1, uint32_t p_sel,
                                                      int mode, double* __restrict__ out,
                                                      double* __restrict__ partial) {
-  // CAP: staged patches per patch column (FP64 keeps the LDS below 64 KB)
-  constexpr int M = BS * BS, TR = 32, TC = 8, NC = TC + BS - 1, NRW = TR + BS, CAP = sizeof(T) == 4 ? 48 : 40;
-  constexpr int RS = NC * CAP + 1;             // plane stride (odd: spreads the planes over banks)
+  // TOT: staged patch rows per tile, the bucket ranges of its NC patch
+  // columns packed one after another; COLCAP: rows per patch column
+  constexpr int M = BS * BS, TR = 32, TC = 8, NC = TC + BS - 1, NRW = TR + BS, TOT = 448, COLCAP = 64;
+  constexpr int RS = TOT + 1;                  // plane stride (odd: spreads the planes over banks)
   constexpr int VW = 16 / (int)sizeof(T);      // values per 16-byte load
   constexpr int W = (M % VW == 0) ? VW : 1;    // values per staging load
   typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type VT;
   typedef typename std::conditional<(W > 1), VT, T>::type ST;
   // sB: staged patch rows, one plane per patch row q (plane q, LDS row j);
   // the pad element of each plane is read (and ignored) by empty buckets.
+  // sP: first LDS row of each patch column.
   // sT: bucket offsets of the tile's halo, cell (lc, i) = position (row
   // r0 - (BS-1) + i clamped to [0, n1], column c0 - (BS-1) + lc), all 0 for
   // columns outside the frame (empty buckets).  A voxel's bounds are then
   // cells at fixed offsets from one per-thread base.
   __shared__ T sB[M * RS];
   __shared__ int32_t sT[NC * NRW];
+  __shared__ int32_t sP[NC];
   __shared__ double sm[5][4];
   uint32_t bx, slot;
   xcd_block(bx, slot);
@@ -1292,13 +1296,24 @@ __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, c
     }
   }
   __syncthreads();
+  // column ranges, packed: column lc's rows start at LDS row pre[lc]
+  int32_t pre[NC + 1];
   bool fits = true;
+  pre[0] = 0;
 #pragma unroll
-  for (int lc = 0; lc < NC; ++lc) fits = fits && (sT[lc * NRW + NRW - 1] - sT[lc * NRW] <= CAP);
+  for (int lc = 0; lc < NC; ++lc) {
+    const int32_t cnt = sT[lc * NRW + NRW - 1] - sT[lc * NRW];
+    fits = fits && cnt <= COLCAP;
+    pre[lc + 1] = pre[lc] + cnt;
+  }
+  fits = fits && pre[NC] <= TOT;
+#pragma unroll
+  for (int lc = 0; lc < NC; ++lc)
+    if ((int)threadIdx.x == lc) sP[lc] = pre[lc];
   if (fits) {
     // patch column lc: thread -> (row lr, values q .. q + W - 1) of the
     // column's bucket range, RPI rows per pass
-    constexpr int QW = M / W, RPI = 256 / QW, NH = (CAP + RPI - 1) / RPI;
+    constexpr int QW = M / W, RPI = 256 / QW, NH = (COLCAP + RPI - 1) / RPI;
     const int lr0 = (int)threadIdx.x / QW, q = ((int)threadIdx.x % QW) * W;
     ST sv[NC][NH];
 #pragma unroll
@@ -1320,7 +1335,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, c
         if (lr0 < RPI && lr < cnt) {
           const T* vv = reinterpret_cast<const T*>(&sv[lc][h]);
 #pragma unroll
-          for (int w = 0; w < W; ++w) sB[(q + w) * RS + lc * CAP + lr] = vv[w];
+          for (int w = 0; w < W; ++w) sB[(q + w) * RS + pre[lc] + lr] = vv[w];
         }
       }
     }
@@ -1338,7 +1353,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, c
     for (int cb = 0; cb < BS; ++cb) {
 #pragma unroll
       for (int e = 0; e <= BS; ++e) bnd[cb][e] = sT[tb - cb * NRW + e];
-      base[cb] = ((c - c0) + BS - 1 - cb) * CAP - sT[tb - (r - r0) - cb * NRW];
+      base[cb] = sP[(c - c0) + BS - 1 - cb] - sT[tb - (r - r0) - cb * NRW];
     }
     if (fits) {
       bool multi = false;
