@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <limits>
 #include <type_traits>
 
 #include "common.hpp"
@@ -410,14 +411,18 @@ hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs
 // (arma::svd_econ at src/svt.hpp:111).  Output per patch, sorted by
 // descending singular value: V^T (n x n), W^T = (X V)^T (n x m) and s (n).
 // A group of L lanes holds one matrix; lane l owns rows l, l + L, l + 2L ...
-// of X and of V (R and VR rows), so the rotation parameters, which every
-// lane of the group computes, are shared by R rows.  Column dot products
+// of X (R rows), so the rotation parameters, which every lane of the group
+// computes, are shared by R rows.  Column dot products
 // are a sum over the lane's rows, then a butterfly across the group (DPP
 // inside a 16-lane row, lane swaps for 32/64-lane groups).  Every lane ends
 // the butterfly with the bit-identical sum, so all lanes apply the same
 // rotation.  Column norms are carried through the sweep with the exact
 // Jacobi update (a -= t c, b += t c) and refreshed at the start of every
-// sweep, so a pair costs one butterfly.
+// sweep, so a pair costs one butterfly.  V is not accumulated: after
+// convergence V = X^T W S^-2 (X reloaded), row by row.  Its error in a
+// reconstruction d_r w_r v_r^T stays at eps * s_max (that of v_r is
+// eps * s_max / s_r, times |w_r| = s_r); ranks with s_r <= eps * s_max get
+// v_r = 0, below that level too.
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
@@ -466,8 +471,7 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
                                              uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol,
                                              uint32_t lim, uint32_t fstride, int max_sweeps, T tol2,
                                              T* __restrict__ factors, uint32_t* __restrict__ stats) {
-  constexpr int M = BS * BS, R = (M + L - 1) / L, VR = (N + L - 1) / L, SR = (N + 3) & ~3;
-  static_assert(L * VR >= SR, "V^T padding not covered");
+  constexpr int M = BS * BS, R = (M + L - 1) / L, SR = (N + 3) & ~3;
   const uint32_t b = blockIdx.y;
   const int lane = threadIdx.x % L;
   const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / L;
@@ -475,20 +479,22 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
   const bool valid = var < nvar;  // uniform per group
   const uint32_t it = valid ? (p_sel == p_all ? p : sel[p]) : 0;  // step 1: identity selection
 
-  T x[R][N], v[VR][N];
   const T* U = u32 + ((size_t)b * nvar + (valid ? var : 0)) * vol;
+  // the lane's rows of the Casorati matrix X (column k: frame k's patch)
+  auto load_x = [&](T (&y)[R][N]) {
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    const int32_t pk = valid ? pos[((size_t)b * nw + k) * p_all + it] : 0;
-    const uint32_t row = min((uint32_t)(pk & 0xffff), lim), col = min((uint32_t)(pk >> 16), lim);
+    for (int k = 0; k < N; ++k) {
+      const int32_t pk = valid ? pos[((size_t)b * nw + k) * p_all + it] : 0;
+      const uint32_t row = min((uint32_t)(pk & 0xffff), lim), col = min((uint32_t)(pk >> 16), lim);
 #pragma unroll
-    for (int h = 0; h < R; ++h) {
-      const int q = lane + L * h;
-      x[h][k] = (valid && q < M) ? U[(size_t)k * fs + (row + q % BS) + (size_t)nr * (col + q / BS)] : T(0);
+      for (int h = 0; h < R; ++h) {
+        const int q = lane + L * h;
+        y[h][k] = (valid && q < M) ? U[(size_t)k * fs + (row + q % BS) + (size_t)nr * (col + q / BS)] : T(0);
+      }
     }
-#pragma unroll
-    for (int h = 0; h < VR; ++h) v[h][k] = (lane + L * h == k) ? T(1) : T(0);
-  }
+  };
+  T x[R][N];
+  load_x(x);
 
   T a[N];
   int sweep = 0;
@@ -529,18 +535,6 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
           x[h][i] = cs * xi - sn * xj;
           x[h][j] = sn * xi + cs * xj;
         }
-#pragma unroll
-        for (int h = 0; h < VR; ++h) {
-          const T vi = v[h][i], vj = v[h][j];
-          v[h][i] = cs * vi - sn * vj;
-          v[h][j] = sn * vi + cs * vj;
-          // Pin the V update next to its rotation.  V only feeds the next
-          // sweep, so without this the compiler sinks every V update to the
-          // end of the sweep and keeps (cs, sn) of all rotations live at
-          // once.  The empty statement only names the two registers; it has
-          // no instruction and no memory access.
-          asm volatile("" : "+v"(v[h][i]), "+v"(v[h][j]));
-        }
         a[i] = fma(-t, c, a[i]);
         a[j] = fma(t, c, a[j]);
         tsum = fma(t, t, tsum);
@@ -566,23 +560,51 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
   T* F = factors + (((size_t)b * nvar + var) * p_sel + p) * fstride;
   T* FW = F + N * SR;
   T* FS = FW + N * M;
+  int rank[N];
+  T inv[N];
+  T amax = a[0];
+#pragma unroll
+  for (int k = 1; k < N; ++k) amax = fmax(amax, a[k]);
+  constexpr T eps = std::numeric_limits<T>::epsilon();
 #pragma unroll
   for (int l = 0; l < N; ++l) {
-    int rank = 0;
+    rank[l] = 0;
 #pragma unroll
-    for (int m = 0; m < N; ++m) rank += (a[m] > a[l] || (a[m] == a[l] && m < l)) ? 1 : 0;
-#pragma unroll
-    for (int h = 0; h < VR; ++h) {
-      const int vr = lane + L * h;
-      if (vr < N) F[rank * SR + vr] = v[h][l];
-      else if (vr < SR) F[rank * SR + vr] = T(0);
-    }
+    for (int m = 0; m < N; ++m) rank[l] += (a[m] > a[l] || (a[m] == a[l] && m < l)) ? 1 : 0;
+    inv[l] = a[l] > eps * eps * amax ? T(1) / a[l] : T(0);
 #pragma unroll
     for (int h = 0; h < R; ++h) {
       const int q = lane + L * h;
-      if (q < M) FW[rank * M + q] = x[h][l];
+      if (q < M) FW[rank[l] * M + q] = x[h][l];
     }
-    if (lane == 0) FS[rank] = sqrt(a[l]);
+    if (lane == 0) FS[rank[l]] = sqrt(a[l]);
+  }
+  // V^T, column by column: element k of v_l = X^T w_l / s_l^2 from column
+  // k of X (reloaded), stored by lane k % L
+#pragma unroll
+  for (int k = 0; k < SR; ++k) {
+    if (k < N) {
+      const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
+      const uint32_t row = min((uint32_t)(pk & 0xffff), lim), col = min((uint32_t)(pk >> 16), lim);
+      T xk[R];
+#pragma unroll
+      for (int h = 0; h < R; ++h) {
+        const int q = lane + L * h;
+        xk[h] = q < M ? U[(size_t)k * fs + (row + q % BS) + (size_t)nr * (col + q / BS)] : T(0);
+      }
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        T pv = xk[0] * x[0][l];
+#pragma unroll
+        for (int h = 1; h < R; ++h) pv = fma(xk[h], x[h][l], pv);
+        const T vk = group_sum<L>(pv) * inv[l];
+        if (lane == k % L) F[rank[l] * SR + k] = vk;
+      }
+    } else {
+#pragma unroll
+      for (int l = 0; l < N; ++l)
+        if (lane == k % L) F[rank[l] * SR + k] = T(0);
+    }
   }
 }
 
