@@ -85,7 +85,10 @@ struct ProfPending {
 class Profiler {
  public:
   explicit Profiler(hipStream_t s) : s_(s) {}
-  ~Profiler() { flush(); }
+  ~Profiler() {
+    flush();
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+  }
   // time `launch` (which enqueues kernels on the stream) under `name`
   bool suspended = false;  // set while a graph is captured: launches only
   template <class F>
@@ -94,9 +97,7 @@ class Profiler {
       launch();
       return;
     }
-    hipEvent_t a, b;
-    hipchk(hipEventCreate(&a), "event");
-    hipchk(hipEventCreate(&b), "event");
+    const hipEvent_t a = event(), b = event();
     hipchk(hipEventRecord(a, s_), "event record");
     launch();
     hipchk(hipEventRecord(b, s_), "event record");
@@ -125,8 +126,8 @@ class Profiler {
         g_prof[p.idx].ms += ms;
         g_prof[p.idx].units += p.units;
       }
-      (void)hipEventDestroy(p.a);
-      (void)hipEventDestroy(p.b);
+      pool_.push_back(p.a);
+      pool_.push_back(p.b);
     }
     pend_.clear();
   }
@@ -134,6 +135,17 @@ class Profiler {
  private:
   hipStream_t s_;
   std::vector<ProfPending> pend_;
+  std::vector<hipEvent_t> pool_;  // events of flushed launches, reused
+  hipEvent_t event() {
+    if (!pool_.empty()) {
+      const hipEvent_t e = pool_.back();
+      pool_.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    hipchk(hipEventCreate(&e), "event");
+    return e;
+  }
 };
 
 // wall-clock time of a host phase, recorded next to the kernel timings
@@ -257,6 +269,28 @@ struct DBuf {
     n = 0;
   }
   size_t bytes() const { return n * sizeof(T); }
+};
+
+// Pinned host staging for the small per-round copies (window metadata and
+// list up, risk sums down), which then need no staging copy by the runtime.
+template <class T>
+struct HBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  HBuf() {}
+  HBuf(const HBuf&) = delete;
+  HBuf& operator=(const HBuf&) = delete;
+  ~HBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void alloc(size_t count) {
+    if (count <= n) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    hipchk(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+    n = count;
+  }
 };
 
 struct Geometry {
@@ -500,6 +534,9 @@ struct Pipeline {
   uint32_t NB = 0;
   DBuf<WinMeta> meta_d;
   DBuf<uint32_t> wl_d;
+  HBuf<WinMeta> meta_h;
+  HBuf<uint32_t> wl_h;
+  HBuf<double> sums_h;
   DBuf<double> u64, w64, invw, dw, partial, partial_b, sums;
   // SVD input variants, factors and U reconstructions: FP32, or FP64 for
   // the precision study (sh.fp64); sized in bytes
@@ -634,6 +671,9 @@ struct Pipeline {
     if (const char* e = std::getenv("PGSVT_BATCH")) NB = std::max(1u, std::min<uint32_t>(NB, (uint32_t)atoi(e)));
     meta_d.alloc(NB);
     wl_d.alloc(NB);
+    meta_h.alloc(NB);
+    wl_h.alloc(NB);
+    sums_h.alloc((size_t)NB * 6);
     u64.alloc((size_t)NB * sh.vol);
     if (median) w64.alloc((size_t)NB * sh.vol);
     const size_t es = sh.fp64 ? 8 : 4;
@@ -726,8 +766,11 @@ struct Pipeline {
   // evaluate the risk of the listed batch slots at their meta.lambda
   void eval_risk(std::vector<WinMeta>& ms, const std::vector<uint32_t>& list_, std::vector<double>& out) {
     const uint32_t na = (uint32_t)list_.size();
-    hipchk(hipMemcpyAsync(meta_d.p, ms.data(), ms.size() * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
-    hipchk(hipMemcpyAsync(wl_d.p, list_.data(), na * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
+    // the staging buffers are free: the previous round was synchronised
+    std::memcpy(meta_h.p, ms.data(), ms.size() * sizeof(WinMeta));
+    std::memcpy(wl_h.p, list_.data(), na * sizeof(uint32_t));
+    hipchk(hipMemcpyAsync(meta_d.p, meta_h.p, ms.size() * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
+    hipchk(hipMemcpyAsync(wl_d.p, wl_h.p, na * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
     // units: thresholded patch reconstructions (patch x variant)
     prof->run("eval_reconstruct", (double)na * 3 * sh.p_sel, [&] {
       hipchk(launch_eval(factors.p, pos.p, sel_d.p, meta_d.p, wl_d.p, na, nvar, 3, sh,
@@ -741,8 +784,9 @@ struct Pipeline {
       hipchk(launch_risk_reduce(partial.p, eval_blocks(sh, 3), partial_b.p, gather_blocks, na, sums.p, s), "reduce");
     });
     out.resize((size_t)na * 6);
-    hipchk(hipMemcpyAsync(out.data(), sums.p, (size_t)na * 6 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H sums");
+    hipchk(hipMemcpyAsync(sums_h.p, sums.p, (size_t)na * 6 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H sums");
     hipchk(hipStreamSynchronize(s), "risk sync");
+    std::memcpy(out.data(), sums_h.p, (size_t)na * 6 * sizeof(double));
     prof->flush();
   }
 
