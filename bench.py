@@ -46,12 +46,19 @@ P_SEL = (N - PATCH + 1) ** 2  # patches per window (step 1)
 # them in its profile):
 #   svd_jacobi        matrix            thin SVD of one m x n Casorati matrix with V and U S:
 #                                       14 m n^2 + 8 n^3 flops (Golub-Van Loan count)
-#   eval_reconstruct  patch x variant   U diag(g(s)) V^T of one patch: 2 m n^2 flops
+#   eval_reconstruct  patch x variant   every datum moved once (fp32 factors and rows): the U
+#                                       reconstruction of every frame written (4 m n / 3: one
+#                                       of three variants), the singular values (4 n), frame
+#                                       positions and bucket places (8 n / 3), delta2/W of the
+#                                       window's voxels (8 NVOX / (3 P)); plus, per kept rank
+#                                       (counted by the library), its V^T and W^T rows (4 (n + m))
 #   gather_risk       voxel             every patch row of the U reconstruction read once (fp32),
 #                                       U (fp64) and 1/W (fp64) of the voxel read once
+EVAL_FIXED = 4.0 * M * NW / 3 + 4.0 * NW + 8.0 * NW / 3 + 8.0 * NVOX / (3 * P_SEL)
+EVAL_PER_RANK = 4.0 * (NW + M)
 ALGO = {
     "svd_jacobi": ("mfma", 14 * M * NW**2 + 8 * NW**3),
-    "eval_reconstruct": ("mfma", 2 * M * NW**2),
+    "eval_reconstruct": ("hbm", EVAL_FIXED),
     "gather_risk": ("hbm", 4.0 * P_SEL * M * NW / NVOX + 16),
 }
 
@@ -195,6 +202,9 @@ def roofline(prof):
     avg_s = e["ms"] / e["launches"] / 1e3
     units_per_launch = e["units"] / e["launches"]
     work = per_unit * units_per_launch
+    kept = prof.get("eval_kept_ranks")
+    if name == "eval_reconstruct" and kept:
+        work += EVAL_PER_RANK * kept["units"] / e["launches"]
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
@@ -219,7 +229,7 @@ def roofline(prof):
 # kernels of each profiled stage, as rocprofv3 names them
 STAGE_KERNELS = {
     "svd_jacobi": "pgsvt::k_svd<",
-    "eval_reconstruct": "pgsvt::k_eval<",
+    "eval_reconstruct": "pgsvt::k_eval",
     "gather_risk": "pgsvt::k_gather_",
 }
 

@@ -536,8 +536,10 @@ struct Pipeline {
   DBuf<uint32_t> wl_d;
   HBuf<WinMeta> meta_h;
   HBuf<uint32_t> wl_h;
-  HBuf<double> sums_h;
-  DBuf<double> u64, w64, invw, dw, partial, partial_b, sums;
+  HBuf<double> sums_h, kept_h;
+  DBuf<double> partial_k, kept_d;  // kept ranks of each eval block / window (profiling)
+  DBuf<double> u64, w64, invw, partial, partial_b, sums;
+  DBuf<char> dw;  // delta2 / W per voxel, in the reconstruction's type
   // SVD input variants, factors and U reconstructions: FP32, or FP64 for
   // the precision study (sh.fp64); sized in bytes
   DBuf<char> uvar, factors, B0;
@@ -674,6 +676,7 @@ struct Pipeline {
     meta_h.alloc(NB);
     wl_h.alloc(NB);
     sums_h.alloc((size_t)NB * 6);
+    kept_h.alloc(NB);
     u64.alloc((size_t)NB * sh.vol);
     if (median) w64.alloc((size_t)NB * sh.vol);
     const size_t es = sh.fp64 ? 8 : 4;
@@ -685,13 +688,15 @@ struct Pipeline {
     list.alloc((size_t)NB * sh.nw * sh.p_sel);
     inv.alloc((size_t)NB * sh.nw * sh.p_sel);
     invw.alloc((size_t)NB * sh.vol);
-    if (optimize) dw.alloc((size_t)NB * sh.vol);
+    if (optimize) dw.alloc((size_t)NB * sh.vol * (sh.fp64 ? sizeof(double) : sizeof(float)));
     B0.alloc((size_t)NB * sh.nw * sh.p_sel * sh.m * es);
     factors.alloc((size_t)NB * nvar * sh.p_sel * sh.fstride * es);
     gather_blocks = pgsvt::gather_blocks(sh, 0);
     partial.alloc((size_t)NB * eval_blocks(sh, 3));
     partial_b.alloc((size_t)NB * gather_blocks * 5);
     sums.alloc((size_t)NB * 6);
+    partial_k.alloc((size_t)NB * eval_blocks(sh, 3));
+    kept_d.alloc(NB);
     max_jobs = 2 * g.Nw;
     jobs_d.alloc((size_t)NB * max_jobs * 4);
     stats.alloc(1);
@@ -772,21 +777,34 @@ struct Pipeline {
     hipchk(hipMemcpyAsync(meta_d.p, meta_h.p, ms.size() * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
     hipchk(hipMemcpyAsync(wl_d.p, wl_h.p, na * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
     // units: thresholded patch reconstructions (patch x variant)
+    // while profiling, the evaluation also counts the ranks it keeps (the
+    // factor rows it reads)
+    const bool count = g_prof_on && !prof->suspended;
     prof->run("eval_reconstruct", (double)na * 3 * sh.p_sel, [&] {
       hipchk(launch_eval(factors.p, pos.p, sel_d.p, meta_d.p, wl_d.p, na, nvar, 3, sh,
-                         p.exponential_weighting, dw.p, inv.p, B0.p, partial.p, s),
+                         p.exponential_weighting, dw.p, inv.p, B0.p, partial.p, s, nullptr,
+                         count ? partial_k.p : nullptr),
              "eval");
     });
     // units: voxels
     prof->run("gather_risk", (double)na * sh.vol, [&] {
       hipchk(launch_gather(B0.p, off.p, invw.p, u64.p, meta_d.p, wl_d.p, na, sh, 0, nullptr, partial_b.p, s),
              "gather");
-      hipchk(launch_risk_reduce(partial.p, eval_blocks(sh, 3), partial_b.p, gather_blocks, na, sums.p, s), "reduce");
+      hipchk(launch_risk_reduce(partial.p, eval_blocks(sh, 3), partial_b.p, gather_blocks, na, sums.p, s,
+                                count ? partial_k.p : nullptr, count ? kept_d.p : nullptr),
+             "reduce");
     });
     out.resize((size_t)na * 6);
     hipchk(hipMemcpyAsync(sums_h.p, sums.p, (size_t)na * 6 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H sums");
+    if (count)
+      hipchk(hipMemcpyAsync(kept_h.p, kept_d.p, (size_t)na * sizeof(double), hipMemcpyDeviceToHost, s), "D2H kept");
     hipchk(hipStreamSynchronize(s), "risk sync");
     std::memcpy(out.data(), sums_h.p, (size_t)na * 6 * sizeof(double));
+    if (count) {
+      double k = 0;
+      for (uint32_t i = 0; i < na; ++i) k += kept_h.p[i];
+      prof_host("eval_kept_ranks", 0.0, k);
+    }
     prof->flush();
   }
 

@@ -727,11 +727,14 @@ hipError_t launch_buckets(const int32_t* pos, const uint32_t* sel, uint32_t nb, 
 }
 
 // weights of src/svt.hpp:157-163: W = number of patches covering a voxel
-// (from the bucket sizes), stored as 1/W (0 where nothing covers it) and
-// as delta2/W for the divergence sum.
+// (from the bucket sizes), stored as 1/W (FP64, 0 where nothing covers it)
+// and as delta2/W for the divergence sum, in the reconstruction's type
+// (FP32: half the bytes the evaluation reads; its rounding, 6e-8 of each
+// term, is far below that of the FP32 reconstructions it multiplies).
+template <class D>
 __global__ void k_weights(const int32_t* __restrict__ off, const uint8_t* __restrict__ d2, uint32_t nr, uint32_t fs,
                           uint32_t vol, uint32_t n1, uint32_t bs, double d2neg, double d2pos,
-                          double* __restrict__ invw, double* __restrict__ dw) {
+                          double* __restrict__ invw, D* __restrict__ dw) {
   const uint32_t b = blockIdx.y;
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= vol) return;
@@ -751,13 +754,18 @@ __global__ void k_weights(const int32_t* __restrict__ off, const uint8_t* __rest
   }
   const double iw = w > 0 ? 1.0 / (double)w : 0.0;
   invw[(size_t)b * vol + v] = iw;
-  if (dw) dw[(size_t)b * vol + v] = (d2[v] ? d2neg : d2pos) * iw;
+  if (dw) dw[(size_t)b * vol + v] = (D)((d2[v] ? d2neg : d2pos) * iw);
 }
 
 hipError_t launch_weights(const int32_t* off, const uint8_t* d2, uint32_t nb, const Shape& sh, double d2neg,
-                          double d2pos, double* invw, double* dw, hipStream_t s) {
-  hipLaunchKernelGGL(k_weights, dim3((sh.vol + 255) / 256, nb), dim3(256), 0, s, off, d2, sh.nr, sh.fs, sh.vol, sh.n1,
-                     sh.bs, d2neg, d2pos, invw, dw);
+                          double d2pos, double* invw, void* dw, hipStream_t s) {
+  const dim3 grid((sh.vol + 255) / 256, nb);
+  if (sh.fp64)
+    hipLaunchKernelGGL(k_weights<double>, grid, dim3(256), 0, s, off, d2, sh.nr, sh.fs, sh.vol, sh.n1, sh.bs, d2neg,
+                       d2pos, invw, (double*)dw);
+  else
+    hipLaunchKernelGGL(k_weights<float>, grid, dim3(256), 0, s, off, d2, sh.nr, sh.fs, sh.vol, sh.n1, sh.bs, d2neg,
+                       d2pos, invw, (float*)dw);
   return hipGetLastError();
 }
 
@@ -814,6 +822,17 @@ __device__ __forceinline__ void xcd_block(uint32_t& bx, uint32_t& by) {
   by = logical / gridDim.x;
 }
 
+// Sum of a per-thread count over the block into *out (a profiling aid: the
+// kept ranks of the evaluation); every thread of the block must call it.
+__device__ __forceinline__ void block_count(double* out, int v) {
+  __shared__ int wsum[4];
+  int w = v;
+  for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x / 64] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = (double)(((wsum[0] + wsum[1]) + wsum[2]) + wsum[3]);
+}
+
 // --------------------------------------------- threshold + reconstruct --
 // One lambda evaluation of the thresholded reconstruction of every patch
 // (src/svt.hpp:121-146 with src/utils.hpp:96-106):
@@ -834,8 +853,9 @@ __global__ __launch_bounds__(256) void k_eval(
     const T* __restrict__ factors, const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel,
     const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl, uint32_t nvar_stored, uint32_t nvar_run,
     uint32_t p_sel, uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim,
-    uint32_t fstride, int exp_w, const double* __restrict__ dw, const int32_t* __restrict__ inv,
-    T* __restrict__ B0, double* __restrict__ partial, const double* __restrict__ slot_lam) {
+    uint32_t fstride, int exp_w, const T* __restrict__ dw, const int32_t* __restrict__ inv,
+    T* __restrict__ B0, double* __restrict__ partial, const double* __restrict__ slot_lam,
+    double* __restrict__ kept) {
   constexpr int M = BS * BS, G = 256 / L, SR = (N + 3) & ~3, P3 = G / 3;
   __shared__ T xch[P3][3][M][N];  // reconstructions of the three variants
   __shared__ double red[4];
@@ -895,7 +915,7 @@ __global__ __launch_bounds__(256) void k_eval(
   auto load_dw = [&] {
     const int lq = min(lane, M - 1);
     const int rr = lq % BS, cc = lq / BS;
-    const double* DW = dw + (size_t)b * vol;
+    const T* DW = dw + (size_t)b * vol;
 #pragma unroll
     for (int i = 0; i < KP; ++i) {
       const int k = min(va + 3 * i, N - 1);
@@ -1010,6 +1030,7 @@ __global__ __launch_bounds__(256) void k_eval(
   if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = s4;
   __syncthreads();
   if (threadIdx.x == 0) partial[(size_t)slot * gridDim.x + bx] = ((red[0] + red[1]) + red[2]) + red[3];
+  if (kept) block_count(kept + (size_t)slot * gridDim.x + bx, (lane == 0 && valid) ? nkeep : 0);
 }
 
 // PGURE evaluation for patches up to 4 x 4: one 16-lane group (one DPP
@@ -1041,8 +1062,8 @@ __global__ __launch_bounds__(256) void k_eval_pg(
     const T* __restrict__ factors, const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel,
     const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl, uint32_t nvar_stored, uint32_t p_sel,
     uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim, uint32_t fstride, int exp_w,
-    const double* __restrict__ dw, const int32_t* __restrict__ inv, T* __restrict__ B0,
-    double* __restrict__ partial, const double* __restrict__ slot_lam) {
+    const T* __restrict__ dw, const int32_t* __restrict__ inv, T* __restrict__ B0,
+    double* __restrict__ partial, const double* __restrict__ slot_lam, double* __restrict__ kept) {
   constexpr int M = BS * BS, SR = (N + 3) & ~3;
   static_assert(M <= 16 && SR <= 16, "one 16-lane row per patch");
   __shared__ double red[4];
@@ -1071,8 +1092,9 @@ __global__ __launch_bounds__(256) void k_eval_pg(
   const int32_t ib_l = inv[((size_t)b * nw + lk) * p_sel + pa];
   const int lv = min(lane, SR - 1), lw = min(lane, M - 1);
   const int rr = lw % BS, cc = lw / BS;
-  const double* DW = dw + (size_t)b * vol;
+  const T* DW = dw + (size_t)b * vol;
   double t2[N], dwv[N], s4 = 0.0;
+  int kr = 0;  // kept ranks of the three variants (lane 0 of the group)
   static_for<0, 3>([&](auto vc) {
     constexpr int v = decltype(vc)::value;
     const T* Fv = F + v * vstep;
@@ -1085,6 +1107,7 @@ __global__ __launch_bounds__(256) void k_eval_pg(
     const uint64_t bal = __ballot(dl > T(0));
     const uint32_t gbits = (uint32_t)((bal >> ((threadIdx.x % 64) & ~15u)) & 0xffffu);
     const int nkeep = gbits ? 32 - __clz(gbits) : 0;
+    if (lane == 0 && valid) kr += nkeep;
     // kept ranks of V^T (lane j: element j) and W^T (lane q: row q), read
     // before the first product; entries past nkeep stay unset and unread
     T vr[N], wr[N];
@@ -1128,6 +1151,7 @@ __global__ __launch_bounds__(256) void k_eval_pg(
   if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = s4;
   __syncthreads();
   if (threadIdx.x == 0) partial[(size_t)slot * gridDim.x + bx] = ((red[0] + red[1]) + red[2]) + red[3];
+  if (kept) block_count(kept + (size_t)slot * gridDim.x + bx, kr);
 }
 
 // ----------------------------------------------------------- gather --
@@ -1450,9 +1474,22 @@ __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, c
 // per slot: (sum Uhat^2, sum U, sum U^2, sum Uhat U, sum d2 (U2p - 2 Uhat + U2m), sum Uhat)
 __global__ __launch_bounds__(256) void k_risk_reduce(const double* __restrict__ pa, uint32_t na,
                                                      const double* __restrict__ pb, uint32_t nb_,
-                                                     double* __restrict__ sums) {
+                                                     double* __restrict__ sums, const double* __restrict__ pk,
+                                                     double* __restrict__ kept) {
   __shared__ double sm[6][256];
   const uint32_t slot = blockIdx.x;
+  if (pk) {  // kept ranks of the slot (profiling): the blocks' counts
+    double c = 0.0;
+    for (uint32_t k = threadIdx.x; k < na; k += blockDim.x) c += pk[(size_t)slot * na + k];
+    sm[0][threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) kept[slot] = sm[0][0];
+    __syncthreads();
+  }
   double s[6] = {0, 0, 0, 0, 0, 0};
   for (uint32_t k = threadIdx.x; k < nb_; k += blockDim.x) {
     const double* P = pb + ((size_t)slot * nb_ + k) * 5;
@@ -1553,8 +1590,8 @@ uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run) {
 
 hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
                        const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh,
-                       int exp_w, const double* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s,
-                       const double* slot_lam) {
+                       int exp_w, const void* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s,
+                       const double* slot_lam, double* kept) {
   const uint32_t lim = sh.nr - sh.bs;
   const uint32_t nblk = eval_blocks(sh, nvar_run);
 #define PG_EV(B, N, L, LS)                                                                                       \
@@ -1564,23 +1601,23 @@ hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* 
         if (sh.fp64)                                                                                           \
           hipLaunchKernelGGL((k_eval_pg<B, N, double>), dim3(nblk, na), dim3(256), 0, s,                       \
                              (const double*)factors, pos, sel, meta, wl, nvar_stored, sh.p_sel, sh.p_all,       \
-                             sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, exp_w, dw, inv, (double*)B0,        \
-                             partial, slot_lam);                                                               \
+                             sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, exp_w, (const double*)dw, inv, (double*)B0,        \
+                             partial, slot_lam, kept);                                                         \
         else                                                                                                   \
           hipLaunchKernelGGL((k_eval_pg<B, N, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors, \
                              pos, sel, meta, wl, nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, \
-                             lim, sh.fstride, exp_w, dw, inv, (float*)B0, partial, slot_lam);                  \
+                             lim, sh.fstride, exp_w, (const float*)dw, inv, (float*)B0, partial, slot_lam, kept);            \
         return hipGetLastError();                                                                              \
       }                                                                                                        \
     }                                                                                                          \
     if (sh.fp64)                                                                                               \
       hipLaunchKernelGGL((k_eval<B, N, L, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)factors,    \
                          pos, sel, meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw,    \
-                         sh.vol, lim, sh.fstride, exp_w, dw, inv, (double*)B0, partial, slot_lam);             \
+                         sh.vol, lim, sh.fstride, exp_w, (const double*)dw, inv, (double*)B0, partial, slot_lam, kept);       \
     else                                                                                                       \
       hipLaunchKernelGGL((k_eval<B, N, L, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors, pos, \
                          sel, meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, \
-                         lim, sh.fstride, exp_w, dw, inv, (float*)B0, partial, slot_lam);                      \
+                         lim, sh.fstride, exp_w, (const float*)dw, inv, (float*)B0, partial, slot_lam, kept);                \
     return hipGetLastError();                                                                                  \
   }
   PG_SHAPES(PG_EV)
@@ -1708,8 +1745,8 @@ hipError_t launch_grid_select(const double* sums, const double* noise, uint32_t 
 }
 
 hipError_t launch_risk_reduce(const double* pa, uint32_t na_blk, const double* pb, uint32_t nb_blk, uint32_t na,
-                              double* sums, hipStream_t s) {
-  hipLaunchKernelGGL(k_risk_reduce, dim3(na), dim3(256), 0, s, pa, na_blk, pb, nb_blk, sums);
+                              double* sums, hipStream_t s, const double* pk, double* kept) {
+  hipLaunchKernelGGL(k_risk_reduce, dim3(na), dim3(256), 0, s, pa, na_blk, pb, nb_blk, sums, pk, kept);
   return hipGetLastError();
 }
 

@@ -50,15 +50,16 @@ hipError_t launch_buckets(const int32_t* pos, const uint32_t* sel, uint32_t nb, 
                           int32_t* off, int32_t* list, int32_t* inv, hipStream_t s);
 // 1/W and delta2/W per voxel, [nb][vol] (dw may be null)
 hipError_t launch_weights(const int32_t* off, const uint8_t* d2, uint32_t nb, const Shape& sh, double d2neg,
-                          double d2pos, double* invw, double* dw, hipStream_t s);
+                          double d2pos, double* invw, void* dw, hipStream_t s);
 // one lambda evaluation of the listed slots; B0 [na][nw][p_sel][m] (rows in
 // bucket order); partial
-// [na][eval_blocks] (divergence sums, only with nvar_run == 3)
+// [na][eval_blocks] (divergence sums, only with nvar_run == 3); kept
+// (optional, nvar_run == 3): [na][eval_blocks] kept ranks of each block
 uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run);
 hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
                        const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh,
-                       int exp_w, const double* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s,
-                       const double* slot_lam = nullptr);
+                       int exp_w, const void* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s,
+                       const double* slot_lam = nullptr, double* kept = nullptr);
 // overlap-add gather: mode 0 risk sums into partial [na][gather_blocks][5]; mode 1
 // final reference frames into out (the output cube); mode 2 the whole
 // window of slot 0 into out
@@ -67,9 +68,10 @@ hipError_t launch_gather(const void* B0, const int32_t* off, const double* invw,
                          const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int mode, double* out,
                          double* partial, hipStream_t s);
 // sums [na][6] = (sum Uhat^2, sum U, sum U^2, sum Uhat U,
-// sum d2 (U2p - 2 Uhat + U2m), sum Uhat)
+// sum d2 (U2p - 2 Uhat + U2m), sum Uhat); with pk (the eval's kept-rank
+// counts), kept [na] = their sum per window
 hipError_t launch_risk_reduce(const double* pa, uint32_t na_blk, const double* pb, uint32_t nb_blk, uint32_t na,
-                              double* sums, hipStream_t s);
+                              double* sums, hipStream_t s, const double* pk = nullptr, double* kept = nullptr);
 
 // hot-pixel filter of uint16 frames in place (one workgroup per frame);
 // scratch: nf * nc uint32
