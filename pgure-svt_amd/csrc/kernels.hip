@@ -624,31 +624,49 @@ __global__ void k_bucket_count(const int32_t* __restrict__ pos, const uint32_t* 
   atomicAdd(cnt + ((size_t)b * nw + k) * n1 * n1 + r + (size_t)n1 * c, 1);
 }
 
-// exclusive scan of one (window, frame) count table into offsets
+// exclusive scan of one (window, frame) count table into offsets.  Wave w
+// owns a contiguous chunk: first every wave sums its chunk (64 lanes on 64
+// consecutive counts per step, coalesced), then each wave scans its chunk
+// 64 counts per step, starting from the sum of the chunks before it.
+template <int W>
+__device__ __forceinline__ int32_t wave_incl_scan(int32_t x, uint32_t lane) {
+#pragma unroll
+  for (int o = 1; o < W; o <<= 1) {
+    const int32_t y = __shfl_up(x, o, W);
+    x += lane >= (uint32_t)o ? y : 0;
+  }
+  return x;
+}
+
 __global__ __launch_bounds__(1024) void k_bucket_scan(const int32_t* __restrict__ cnt, uint32_t npos,
                                                       int32_t* __restrict__ off) {
-  __shared__ int32_t part[1024];
+  __shared__ int32_t wt[16];
   const size_t tab = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
   const int32_t* C = cnt + tab * npos;
   int32_t* O = off + tab * (npos + 1);
-  const uint32_t chunk = (npos + 1023) / 1024;
-  const uint32_t lo = min(threadIdx.x * chunk, npos), hi = min(lo + chunk, npos);
+  const uint32_t w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const uint32_t chunk = (npos + 15) / 16;
+  const uint32_t lo = min(w * chunk, npos), hi = min(lo + chunk, npos);
   int32_t sum = 0;
-  for (uint32_t i = lo; i < hi; ++i) sum += C[i];
-  part[threadIdx.x] = sum;
+  for (uint32_t i = lo + lane; i < hi; i += 64) sum += C[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+  if (lane == 0) wt[w] = sum;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
-    const int32_t v = threadIdx.x >= (uint32_t)o ? part[threadIdx.x - o] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
+  int32_t carry = 0;
+  for (uint32_t k = 0; k < w; ++k) carry += wt[k];
+  for (uint32_t base = lo; base < hi; base += 64) {
+    const uint32_t i = base + lane;
+    const int32_t x = i < hi ? C[i] : 0;
+    const int32_t incl = wave_incl_scan<64>(x, lane);
+    if (i < hi) O[i] = carry + incl - x;
+    carry += __shfl(incl, 63);
   }
-  int32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
-  for (uint32_t i = lo; i < hi; ++i) {
-    O[i] = run;
-    run += C[i];
+  if (threadIdx.x == 0) {
+    int32_t t = 0;
+    for (int k = 0; k < 16; ++k) t += wt[k];
+    O[npos] = t;
   }
-  if (threadIdx.x == 1023) O[npos] = part[1023];
 }
 
 __global__ void k_bucket_fill(const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel, uint32_t p_sel,
