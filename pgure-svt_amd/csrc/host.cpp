@@ -671,6 +671,23 @@ struct Pipeline {
     const size_t per = bytes_per_window();
     NB = (uint32_t)std::max<size_t>(1, std::min<size_t>(want, budget / per));
     if (const char* e = std::getenv("PGSVT_BATCH")) NB = std::max(1u, std::min<uint32_t>(NB, (uint32_t)atoi(e)));
+    // The budget is an estimate: calls running concurrently on the device
+    // size their batches from the same free memory.  A batch that does not
+    // fit is halved (buffers of the failed attempt go back to the cache,
+    // which a failing hipMalloc empties).
+    for (;;) {
+      try {
+        alloc_batch_buffers();
+        return;
+      } catch (const HipError& e) {
+        if (e.e != hipErrorOutOfMemory || NB == 1) throw;
+        (void)hipGetLastError();
+        NB = std::max(1u, NB / 2);
+      }
+    }
+  }
+
+  void alloc_batch_buffers() {
     meta_d.alloc(NB);
     wl_d.alloc(NB);
     meta_h.alloc(NB);
