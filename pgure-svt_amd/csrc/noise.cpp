@@ -108,7 +108,11 @@ double robust_mean(const std::vector<double>& A) {
     auto step = [&](size_t q) {
       const double r = A[q] - m;
       const double a = std::fabs(r * inv);
-      w[q] = (a < 0.75) ? 1. : 0.75 / a;
+      // divide for outliers only (inliers keep weight 1)
+      if (__builtin_expect(a < 0.75, 1))
+        w[q] = 1.;
+      else
+        w[q] = 0.75 / a;
       return std::fabs(r);
     };
     for (k = 0; k + 1 < N; k += 2) {
@@ -141,18 +145,19 @@ bool split_block(const double* F, uint32_t n, uint32_t i0, uint32_t j0, uint32_t
   if (N <= 8) return false;
   auto A = [&](uint32_t y, uint32_t x) { return F[(i0 + y) + (size_t)n * (j0 + x)]; };
   const double sq = std::sqrt((double)(5 * 5 + 5));
-  auto res = [&](size_t k) {
-    const uint32_t y = (uint32_t)(k % N), x = (uint32_t)(k / N);
-    const uint32_t xp = (x + 1 == N) ? 1 : x + 1, yp = (y + 1 == N) ? 1 : y + 1;
-    const uint32_t xm = (x == 0) ? N - 2 : x - 1, ym = (y == 0) ? N - 2 : y - 1;
-    return (5u * A(y, x) - (A(yp, x) + A(ym, x) + A(y, xm) + A(y, xp))) / sq;
-  };
   const uint32_t R = N * N;
   const double OoR = 1.0 / R, OoRm1 = 1.0 / (R - 1);
   std::vector<double> a(R), rs(R);
-  for (size_t k = 0; k < R; ++k) {
-    a[k] = A((uint32_t)(k % N), (uint32_t)(k / N));
-    rs[k] = res(k);
+  // element k = y + N x (column-major), residual of the 5-point stencil
+  // with the reference's wrap-around neighbours
+  for (uint32_t x = 0; x < N; ++x) {
+    const uint32_t xp = (x + 1 == N) ? 1 : x + 1, xm = (x == 0) ? N - 2 : x - 1;
+    for (uint32_t y = 0; y < N; ++y) {
+      const uint32_t yp = (y + 1 == N) ? 1 : y + 1, ym = (y == 0) ? N - 2 : y - 1;
+      const size_t k = y + (size_t)N * x;
+      a[k] = A(y, x);
+      rs[k] = (5u * A(y, x) - (A(yp, x) + A(ym, x) + A(y, xm) + A(y, xp))) / sq;
+    }
   }
   // four sums, each with its own even/odd accumulator pair (as accu)
   double z1 = 0, z2 = 0, r1 = 0, r2 = 0;
