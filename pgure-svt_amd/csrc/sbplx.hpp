@@ -14,13 +14,15 @@
 //
 // Resumable: `run()` replays the search from the start against the values
 // recorded so far and stops with `kNeed` at the first evaluation it has no
-// value for.  The caller evaluates that lambda (on the GPU, batched over
-// many windows) and calls `run()` again.  Replay is deterministic, so the
-// i-th evaluation of every replay is the same lambda.
+// value for (every call site returns at once, before any bookkeeping).  The
+// caller evaluates that lambda (on the GPU, batched over many windows) and
+// calls `run()` again.  Replay is deterministic, so the i-th evaluation of
+// every replay is the same lambda.
 #pragma once
 
 #include <cmath>
 #include <cstdint>
+#include <limits>
 #include <vector>
 
 namespace pgsvt {
@@ -35,11 +37,9 @@ class Sbplx1D {
   // Continue the search.  Returns kNeed with pending() set, or a final code.
   int run() {
     cursor_ = 0;
-    try {
-      return minimize();
-    } catch (const Need&) {
-      return kNeed;
-    }
+    need_ = false;
+    const int rc = minimize();
+    return need_ ? kNeed : rc;
   }
   double pending() const { return pending_; }
   void supply(double f) { values_.push_back(f); }
@@ -49,8 +49,8 @@ class Sbplx1D {
   double best_f() const { return best_f_; }
 
  private:
-  struct Need {};
   double start_, lb_, ub_, step_, ftol_, xtol_;
+  bool need_ = false;  // the replay reached an evaluation with no value yet
   int maxeval_;
   std::vector<double> values_, xs_;
   size_t cursor_ = 0;
@@ -64,7 +64,8 @@ class Sbplx1D {
     }
     if (xs_.size() == values_.size()) xs_.push_back(x);
     pending_ = x;
-    throw Need{};
+    need_ = true;
+    return std::numeric_limits<double>::quiet_NaN();
   }
 
   static bool relstop(double vold, double vnew, double reltol, double abstol) {
@@ -110,6 +111,7 @@ class Sbplx1D {
     }
     if (close_(px[1], x)) return kFailure;
     pf[1] = f(px[1]);
+    if (need_) return kNeed;
     if (count(px[1], pf[1], x, minf)) return kMaxEval;
     for (;;) {
       const int lo = (pf[1] < pf[0]) ? 1 : 0, hi = 1 - lo;
@@ -122,6 +124,7 @@ class Sbplx1D {
       double xr;
       if (!reflect(xr, c, 1.0, xh)) return kXtol;
       const double fr = f(xr);
+      if (need_) return kNeed;
       if (count(xr, fr, x, minf)) return kMaxEval;
       if (fr < fl) {
         double xe;
@@ -129,6 +132,7 @@ class Sbplx1D {
         px[hi] = xe;
         if (!ok) return kXtol;
         fh = f(xe);
+        if (need_) return kNeed;
         if (count(xe, fh, x, minf)) return kMaxEval;
         if (fh >= fr) {
           fh = fr;
@@ -138,6 +142,7 @@ class Sbplx1D {
         double xc;
         if (!reflect(xc, c, fh <= fr ? -0.5 : 0.5, xh)) return kXtol;
         const double fc = f(xc);
+        if (need_) return kNeed;
         if (count(xc, fc, x, minf)) return kMaxEval;
         if (fc < fr && fc < fh) {
           px[hi] = xc;
@@ -145,6 +150,7 @@ class Sbplx1D {
         } else {  // shrink towards the best point and restart
           px[hi] = xl + 0.5 * (px[hi] - xl);
           pf[hi] = f(px[hi]);
+          if (need_) return kNeed;
           if (count(px[hi], pf[hi], x, minf)) return kMaxEval;
           continue;
         }
@@ -159,6 +165,7 @@ class Sbplx1D {
     double x = start_, minf = 0;
     if (lb_ > ub_ || x < lb_ || x > ub_ || step_ == 0) return kInvalid;
     minf = f(x);
+    if (need_) return kNeed;
     nevals_++;
     best_x_ = x;
     best_f_ = minf;
@@ -167,6 +174,7 @@ class Sbplx1D {
     for (;;) {
       const double xprev = x, fprev = minf;
       int ret = nelder_mead(x, minf, xstep, psi);
+      if (ret == kNeed) return kNeed;
       best_x_ = x;
       best_f_ = minf;
       if (ret == kFailure) return kXtol;
