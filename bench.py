@@ -44,20 +44,28 @@ P_SEL = (N - PATCH + 1) ** 2  # patches per window (step 1)
 
 # Algorithmic work per unit of each timed stage (units as the library counts
 # them in its profile):
-#   svd_jacobi        matrix            thin SVD of one m x n Casorati matrix with V and U S:
-#                                       14 m n^2 + 8 n^3 flops (Golub-Van Loan count)
-#   eval_reconstruct  patch x variant   every datum moved once (fp32 factors and rows): the U
-#                                       reconstruction of every frame written (4 m n / 3: one
-#                                       of three variants), the singular values (4 n), frame
-#                                       positions and bucket places (8 n / 3), delta2/W of the
-#                                       window's voxels (8 NVOX / (3 P)); plus, per kept rank
-#                                       (counted by the library), its V^T and W^T rows (4 (n + m))
-#   gather_risk       voxel             every patch row of the U reconstruction read once (fp32),
-#                                       U (fp64) and 1/W (fp64) of the voxel read once
-EVAL_FIXED = 4.0 * M * NW / 3 + 4.0 * NW + 8.0 * NW / 3 + 8.0 * NVOX / (3 * P_SEL)
-EVAL_PER_RANK = 4.0 * (NW + M)
+#   svd_jacobi          matrix            thin SVD of one m x n Casorati matrix with V and U S:
+#                                         14 m n^2 + 8 n^3 flops (Golub-Van Loan count)
+#   divergence_factors  matrix            every datum moved once (fp32): all V^T and W^T rows
+#                                         read (4 n (n + m)), the divergence factors written
+#                                         (4 n), frame positions (4 n / 3) and delta2/W of the
+#                                         window's voxels (4 NVOX / (3 P)) shared by the three
+#                                         variants of a patch
+#   eval_reconstruct    patch x variant   every datum moved once (fp32 factors and rows): the U
+#                                         reconstruction of every frame written (4 m n / 3: one
+#                                         of three variants), the singular values (4 n), bucket
+#                                         places (4 n / 3); plus, per kept rank (counted by the
+#                                         library over the three variants, which keep the same
+#                                         ranks but for a few patches), its divergence factor
+#                                         (4) and, for the U variant, its V^T and W^T rows
+#                                         (4 (n + m) / 3 per counted rank)
+#   gather_risk         voxel             every patch row of the U reconstruction read once (fp32),
+#                                         U (fp64) and 1/W (fp64) of the voxel read once
+EVAL_FIXED = 4.0 * M * NW / 3 + 4.0 * NW + 4.0 * NW / 3
+EVAL_PER_RANK = 4.0 * (NW + M) / 3 + 4.0
 ALGO = {
     "svd_jacobi": ("mfma", 14 * M * NW**2 + 8 * NW**3),
+    "divergence_factors": ("hbm", 4.0 * NW * (NW + M) + 4.0 * NW + 4.0 * NW / 3 + 4.0 * NVOX / (3 * P_SEL)),
     "eval_reconstruct": ("hbm", EVAL_FIXED),
     "gather_risk": ("hbm", 4.0 * P_SEL * M * NW / NVOX + 16),
 }
@@ -229,6 +237,7 @@ def roofline(prof):
 # kernels of each profiled stage, as rocprofv3 names them
 STAGE_KERNELS = {
     "svd_jacobi": "pgsvt::k_svd<",
+    "divergence_factors": "pgsvt::k_divq<",
     "eval_reconstruct": "pgsvt::k_eval",
     "gather_risk": "pgsvt::k_gather_",
 }

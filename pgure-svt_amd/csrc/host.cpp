@@ -540,6 +540,7 @@ struct Pipeline {
   DBuf<double> partial_k, kept_d;  // kept ranks of each eval block / window (profiling)
   DBuf<double> u64, w64, invw, partial, partial_b, sums;
   DBuf<char> dw;  // delta2 / W per voxel, in the reconstruction's type
+  DBuf<char> divq;  // divergence factors of every rank (PGURE, patches up to 4 x 4)
   // SVD input variants, factors and U reconstructions: FP32, or FP64 for
   // the precision study (sh.fp64); sized in bytes
   DBuf<char> uvar, factors, B0;
@@ -658,6 +659,7 @@ struct Pipeline {
     b += 8ull * sh.vol * (optimize ? 2 : 1);             // 1/W, delta2/W
     b += es * sh.nw * (size_t)sh.p_sel * sh.m;           // B0
     b += es * nvar * (size_t)sh.p_sel * sh.fstride;      // factors
+    if (optimize && divq_needed(sh)) b += es * nvar * (size_t)sh.p_sel * ((sh.nw + 3) & ~3u);  // divq
     return b;
   }
 
@@ -708,6 +710,7 @@ struct Pipeline {
     if (optimize) dw.alloc((size_t)NB * sh.vol * (sh.fp64 ? sizeof(double) : sizeof(float)));
     B0.alloc((size_t)NB * sh.nw * sh.p_sel * sh.m * es);
     factors.alloc((size_t)NB * nvar * sh.p_sel * sh.fstride * es);
+    if (optimize && divq_needed(sh)) divq.alloc((size_t)NB * nvar * sh.p_sel * ((sh.nw + 3) & ~3u) * es);
     gather_blocks = pgsvt::gather_blocks(sh, 0);
     partial.alloc((size_t)NB * eval_blocks(sh, 3));
     partial_b.alloc((size_t)NB * gather_blocks * 5);
@@ -763,6 +766,12 @@ struct Pipeline {
                         s, svd_mapping),
              "svd");
     });
+    if (optimize && nvar_run == 3 && divq_needed(sh)) {
+      // units: matrices
+      prof->run("divergence_factors", (double)nb * nvar_run * sh.p_sel, [&] {
+        hipchk(launch_divq(factors.p, pos.p, sel_d.p, dw.p, nb, nvar_run, sh, divq.p, s), "divergence factors");
+      });
+    }
   }
 
   void decompose(std::vector<WinMeta>& ms, uint32_t nvar_run) {
@@ -800,7 +809,7 @@ struct Pipeline {
     prof->run("eval_reconstruct", (double)na * 3 * sh.p_sel, [&] {
       hipchk(launch_eval(factors.p, pos.p, sel_d.p, meta_d.p, wl_d.p, na, nvar, 3, sh,
                          p.exponential_weighting, dw.p, inv.p, B0.p, partial.p, s, nullptr,
-                         count ? partial_k.p : nullptr),
+                         count ? partial_k.p : nullptr, divq.p),
              "eval");
     });
     // units: voxels
@@ -1265,7 +1274,8 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
         for (uint32_t c0 = 0; c0 < slots; c0 += C) {
           const uint32_t na = std::min(C, slots - c0);
           hipchk(launch_eval(P.factors.p, P.pos.p, P.sel_d.p, P.meta_d.p, wl_g.p + c0, na, P.nvar, 3, P.sh,
-                             p.exponential_weighting, P.dw.p, P.inv.p, P.B0.p, P.partial.p, P.s, lam_g.p + c0),
+                             p.exponential_weighting, P.dw.p, P.inv.p, P.B0.p, P.partial.p, P.s, lam_g.p + c0,
+                             nullptr, P.divq.p),
                  "grid eval");
           hipchk(launch_gather(P.B0.p, P.off.p, P.invw.p, P.u64.p, P.meta_d.p, wl_g.p + c0, na, P.sh, 0, nullptr,
                                P.partial_b.p, P.s),

@@ -1051,16 +1051,8 @@ __global__ __launch_bounds__(256) void k_eval(
   if (kept) block_count(kept + (size_t)slot * gridDim.x + bx, (lane == 0 && valid) ? nkeep : 0);
 }
 
-// PGURE evaluation for patches up to 4 x 4: one 16-lane group (one DPP
-// row) per patch, lane q = row q of B.  The group reconstructs the three
-// variants U, U + eps2 delta2, U - eps2 delta2 in turn, stores the U row of
-// every frame at its bucket-sorted place in B0 and accumulates
-// sum_k (B1 - 2 B0 + B2)_k delta2_v / W_v in FP64 (src/pgure.hpp:136) in
-// registers: the second difference of FP32 values is exact in FP64, so
-// forming it as (-2 B0 + B1) + B2 across the three passes loses nothing
-// and no exchange between groups is needed.  Lane k holds the trajectory
-// position and bucket place of frame k; each frame takes them by row
-// broadcast.
+// Lane K of each 16-lane row, for 32-bit integers (trajectory positions,
+// bucket places).
 template <int K>
 __device__ __forceinline__ int32_t row_bcast(int32_t v) {
   return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xF, 0xF, true);
@@ -1075,12 +1067,81 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
+// Divergence factors for patches up to 4 x 4, once per window after the
+// SVD.  The divergence sum of src/pgure.hpp:136 is linear in the
+// reconstructions: for B = sum_r d_r w_r v_r^T (d_r = g(s_r)/s_r, the only
+// part that depends on lambda),
+//   sum over the patch's voxels of delta2_v / W_v B_v = sum_r d_r q_r,
+//   q_r = w_r^T D v_r,
+// D (m x n) the delta2/W volume at the patch's voxels along its trajectory.
+// One 16-lane group per (patch, variant): lane q holds row q of D, lane j
+// element j of each V^T row; t_q = sum_k D[q][k] v_r[k] and q_r = sum_q
+// w_r[q] t_q are formed in FP64 and stored in rank order,
+// divq [nb][nvar][p_sel][sr].
+template <int BS, int N, class T>
+__global__ __launch_bounds__(256) void k_divq(const T* __restrict__ factors, const int32_t* __restrict__ pos,
+                                              const uint32_t* __restrict__ sel, const T* __restrict__ dw,
+                                              uint32_t nvar, uint32_t p_sel, uint32_t p_all, uint32_t nr,
+                                              uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim,
+                                              uint32_t fstride, T* __restrict__ divq) {
+  constexpr int M = BS * BS, SR = (N + 3) & ~3;
+  static_assert(M <= 16 && SR <= 16, "one 16-lane row per patch");
+  uint32_t bx, b;
+  xcd_block(bx, b);
+  const int lane = threadIdx.x % 16;
+  const uint32_t g = bx * 16 + threadIdx.x / 16;  // (variant, patch), patch fastest
+  const bool valid = g < nvar * p_sel;
+  const uint32_t ga = valid ? g : 0;  // groups past the end read pair 0
+  const uint32_t var = ga / p_sel, p = ga % p_sel;
+  const uint32_t it = p_sel == p_all ? p : sel[p];
+  const T* F = factors + (((size_t)b * nvar + var) * p_sel + p) * fstride;
+  const T* FW = F + N * SR;
+  const int lk = min(lane, N - 1), lv = min(lane, SR - 1), lw = min(lane, M - 1);
+  const int rr = lw % BS, cc = lw / BS;
+  // every V^T and W^T row first, then D frame by frame (lane k: frame k's
+  // trajectory position)
+  T vr[N], wr[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    vr[r] = F[r * SR + lv];
+    wr[r] = FW[r * M + lw];
+  }
+  const int32_t pk_l = pos[((size_t)b * nw + lk) * p_all + it];
+  const T* DW = dw + (size_t)b * vol;
+  double d[N];
+  static_for<0, N>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    const int32_t pkk = row_bcast<k>(pk_l);
+    const uint32_t r0 = min((uint32_t)(pkk & 0xffff), lim), c0 = min((uint32_t)(pkk >> 16), lim);
+    d[k] = (double)DW[(uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc)];
+  });
+  double qv = 0.0;  // lane r: q_r
+  static_for<0, N>([&](auto rc) {
+    constexpr int r = decltype(rc)::value;
+    double t = 0.0;
+    static_for<0, N>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      t = fma(d[k], (double)row_bcast<k>(vr[r]), t);
+    });
+    const double y = group_sum<16>(lane < M ? (double)wr[r] * t : 0.0);
+    if (lane == r) qv = y;
+  });
+  if (valid && lane < SR) divq[(((size_t)b * nvar + var) * p_sel + p) * SR + lane] = (T)qv;
+}
+
+// PGURE evaluation for patches up to 4 x 4: one 16-lane group (one DPP
+// row) per patch, lane q = row q of B.  Only the U variant is
+// reconstructed: its row of every frame goes to its bucket-sorted place in
+// B0 (lane k holds frame k's bucket place, taken by row broadcast).  The
+// divergence sum sum_v (B1 - 2 B0 + B2)_v delta2_v / W_v of
+// src/pgure.hpp:136 comes from the divergence factors of the three
+// variants U, U + eps2 delta2, U - eps2 delta2 as
+// sum_r (d1_r q1_r - 2 d0_r q0_r + d2_r q2_r), in FP64.
 template <int BS, int N, class T>
 __global__ __launch_bounds__(256) void k_eval_pg(
-    const T* __restrict__ factors, const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel,
-    const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl, uint32_t nvar_stored, uint32_t p_sel,
-    uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim, uint32_t fstride, int exp_w,
-    const T* __restrict__ dw, const int32_t* __restrict__ inv, T* __restrict__ B0,
+    const T* __restrict__ factors, const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl,
+    uint32_t nvar_stored, uint32_t p_sel, uint32_t nw, uint32_t fstride,
+    int exp_w, const T* __restrict__ divq, const int32_t* __restrict__ inv, T* __restrict__ B0,
     double* __restrict__ partial, const double* __restrict__ slot_lam, double* __restrict__ kept) {
   constexpr int M = BS * BS, SR = (N + 3) & ~3;
   static_assert(M <= 16 && SR <= 16, "one 16-lane row per patch");
@@ -1096,26 +1157,23 @@ __global__ __launch_bounds__(256) void k_eval_pg(
   const double lam = slot_lam ? slot_lam[slot] : meta[b].lambda;
   const size_t vstep = (size_t)p_sel * fstride;  // one variant's factors to the next
   const T* F = factors + ((size_t)b * nvar_stored * p_sel + pa) * fstride;
-  // singular values of the three variants (lane r: s_r), then frame
-  // lk = min(lane, N - 1)'s trajectory position and bucket place
+  const T* Q = divq + ((size_t)b * nvar_stored * p_sel + pa) * SR;
+  // singular values and divergence factors of the three variants (lane r:
+  // s_r, q_r), frame lk = min(lane, N - 1)'s bucket place
   const int lk = min(lane, N - 1);
-  T sl[3], smax[3];
+  T sl[3], smax[3], ql[3];
 #pragma unroll
   for (int v = 0; v < 3; ++v) {
     sl[v] = F[v * vstep + N * SR + N * M + lk];
     smax[v] = F[v * vstep + N * SR + N * M];
+    ql[v] = Q[(size_t)v * p_sel * SR + lk];
   }
-  const uint32_t it = p_sel == p_all ? pa : sel[pa];
-  const int32_t pk_l = pos[((size_t)b * nw + lk) * p_all + it];
   const int32_t ib_l = inv[((size_t)b * nw + lk) * p_sel + pa];
   const int lv = min(lane, SR - 1), lw = min(lane, M - 1);
-  const int rr = lw % BS, cc = lw / BS;
-  const T* DW = dw + (size_t)b * vol;
-  double t2[N], dwv[N], s4 = 0.0;
+  double s4 = 0.0;
   int kr = 0;  // kept ranks of the three variants (lane 0 of the group)
   static_for<0, 3>([&](auto vc) {
     constexpr int v = decltype(vc)::value;
-    const T* Fv = F + v * vstep;
     // shrink factor g(s_r)/s_r in lane r < N, kept ranks of the group
     T dl = T(0);
     if (valid && lane < N) {
@@ -1126,43 +1184,29 @@ __global__ __launch_bounds__(256) void k_eval_pg(
     const uint32_t gbits = (uint32_t)((bal >> ((threadIdx.x % 64) & ~15u)) & 0xffffu);
     const int nkeep = gbits ? 32 - __clz(gbits) : 0;
     if (lane == 0 && valid) kr += nkeep;
-    // kept ranks of V^T (lane j: element j) and W^T (lane q: row q), read
-    // before the first product; entries past nkeep stay unset and unread
-    T vr[N], wr[N];
-#pragma unroll
-    for (int r = 0; r < N; ++r) {
-      if (r < nkeep) {
-        vr[r] = Fv[r * SR + lv];
-        wr[r] = Fv[N * SR + r * M + lw];
-      }
-    }
-    if constexpr (v == 1) {
-      // delta2 / W at the patch's voxels of every frame, for the last pass
-      static_for<0, N>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        const int32_t pkk = row_bcast<k>(pk_l);
-        const uint32_t r0 = min((uint32_t)(pkk & 0xffff), lim), c0 = min((uint32_t)(pkk >> 16), lim);
-        dwv[k] = DW[(uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc)];
-      });
-    }
-    T bq[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) bq[k] = T(0);
-    rank_products<0, N>(nkeep, dl, vr, wr, bq);
+    // lanes past the kept ranks have d_r = 0
+    s4 = fma((v == 0 ? -2.0 : 1.0) * (double)dl, (double)ql[v], s4);
     if constexpr (v == 0) {
+      // kept ranks of V^T (lane j: element j) and W^T (lane q: row q), read
+      // before the first product; entries past nkeep stay unset and unread
+      const T* Fv = F;
+      T vr[N], wr[N];
+#pragma unroll
+      for (int r = 0; r < N; ++r) {
+        if (r < nkeep) {
+          vr[r] = Fv[r * SR + lv];
+          wr[r] = Fv[N * SR + r * M + lw];
+        }
+      }
+      T bq[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) bq[k] = T(0);
+      rank_products<0, N>(nkeep, dl, vr, wr, bq);
       static_for<0, N>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         const int32_t ibk = row_bcast<k>(ib_l);
         if (row) B0[(((size_t)slot * nw + k) * p_sel + ibk) * M + lane] = bq[k];
-        t2[k] = -2.0 * (double)bq[k];
       });
-    } else if constexpr (v == 1) {
-#pragma unroll
-      for (int k = 0; k < N; ++k) t2[k] += (double)bq[k];
-    } else {
-#pragma unroll
-      for (int k = 0; k < N; ++k)
-        if (row) s4 = fma(t2[k] + (double)bq[k], dwv[k], s4);
     }
   });
   for (int o = 32; o > 0; o >>= 1) s4 += __shfl_xor(s4, o);
@@ -1609,22 +1653,22 @@ uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run) {
 hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
                        const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh,
                        int exp_w, const void* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s,
-                       const double* slot_lam, double* kept) {
+                       const double* slot_lam, double* kept, const void* divq) {
   const uint32_t lim = sh.nr - sh.bs;
   const uint32_t nblk = eval_blocks(sh, nvar_run);
 #define PG_EV(B, N, L, LS)                                                                                       \
   if (sh.bs == B && sh.nw == N) {                                                                              \
     if constexpr (L == 16) {                                                                                   \
       if (nvar_run == 3) {                                                                                     \
+        if (!divq) return hipErrorInvalidValue; /* launch_divq after the SVD */                               \
         if (sh.fp64)                                                                                           \
           hipLaunchKernelGGL((k_eval_pg<B, N, double>), dim3(nblk, na), dim3(256), 0, s,                       \
-                             (const double*)factors, pos, sel, meta, wl, nvar_stored, sh.p_sel, sh.p_all,       \
-                             sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, exp_w, (const double*)dw, inv, (double*)B0,        \
-                             partial, slot_lam, kept);                                                         \
+                             (const double*)factors, meta, wl, nvar_stored, sh.p_sel, sh.nw, sh.fstride, exp_w, \
+                             (const double*)divq, inv, (double*)B0, partial, slot_lam, kept);                  \
         else                                                                                                   \
           hipLaunchKernelGGL((k_eval_pg<B, N, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors, \
-                             pos, sel, meta, wl, nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, \
-                             lim, sh.fstride, exp_w, (const float*)dw, inv, (float*)B0, partial, slot_lam, kept);            \
+                             meta, wl, nvar_stored, sh.p_sel, sh.nw, sh.fstride, exp_w, (const float*)divq,     \
+                             inv, (float*)B0, partial, slot_lam, kept);                                        \
         return hipGetLastError();                                                                              \
       }                                                                                                        \
     }                                                                                                          \
@@ -1640,6 +1684,31 @@ hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* 
   }
   PG_SHAPES(PG_EV)
 #undef PG_EV
+  return hipErrorInvalidValue;
+}
+
+bool divq_needed(const Shape& sh) { return sh.m <= 16; }
+
+hipError_t launch_divq(const void* factors, const int32_t* pos, const uint32_t* sel, const void* dw, uint32_t nb,
+                       uint32_t nvar, const Shape& sh, void* divq, hipStream_t s) {
+  const uint32_t lim = sh.nr - sh.bs;
+  const dim3 grid((nvar * sh.p_sel + 15) / 16, nb);
+#define PG_DQ(B, N, L, LS)                                                                                       \
+  if (sh.bs == B && sh.nw == N) {                                                                              \
+    if constexpr (L == 16) {                                                                                   \
+      if (sh.fp64)                                                                                             \
+        hipLaunchKernelGGL((k_divq<B, N, double>), grid, dim3(256), 0, s, (const double*)factors, pos, sel,    \
+                           (const double*)dw, nvar, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim,       \
+                           sh.fstride, (double*)divq);                                                         \
+      else                                                                                                     \
+        hipLaunchKernelGGL((k_divq<B, N, float>), grid, dim3(256), 0, s, (const float*)factors, pos, sel,      \
+                           (const float*)dw, nvar, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim,        \
+                           sh.fstride, (float*)divq);                                                          \
+      return hipGetLastError();                                                                                \
+    }                                                                                                          \
+  }
+  PG_SHAPES(PG_DQ)
+#undef PG_DQ
   return hipErrorInvalidValue;
 }
 
