@@ -231,6 +231,9 @@ def roofline(prof):
     r["kernel"] = name
     r["avg_launch_ms"] = e["ms"] / e["launches"]
     r["units_per_launch"] = units_per_launch
+    # Launch sizes vary from run to run (windows join the lambda search as
+    # their host noise estimates finish); the time per unit does not.
+    r["ps_per_unit"] = e["ms"] * 1e9 / e["units"]
     return r, name
 
 
