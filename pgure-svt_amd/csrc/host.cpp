@@ -540,7 +540,7 @@ struct Pipeline {
   DBuf<double> partial_k, kept_d;  // kept ranks of each eval block / window (profiling)
   DBuf<double> u64, w64, invw, partial, partial_b, sums;
   DBuf<char> dw;  // delta2 / W per voxel, in the reconstruction's type
-  DBuf<char> divq;  // divergence factors of every rank (PGURE, patches up to 4 x 4)
+  DBuf<char> divq;  // divergence factors of every rank (PGURE)
   // SVD input variants, factors and U reconstructions: FP32, or FP64 for
   // the precision study (sh.fp64); sized in bytes
   DBuf<char> uvar, factors, B0;
@@ -659,7 +659,7 @@ struct Pipeline {
     b += 8ull * sh.vol * (optimize ? 2 : 1);             // 1/W, delta2/W
     b += es * sh.nw * (size_t)sh.p_sel * sh.m;           // B0
     b += es * nvar * (size_t)sh.p_sel * sh.fstride;      // factors
-    if (optimize && divq_needed(sh)) b += es * nvar * (size_t)sh.p_sel * ((sh.nw + 3) & ~3u);  // divq
+    if (optimize) b += es * nvar * (size_t)sh.p_sel * ((sh.nw + 3) & ~3u);  // divq
     return b;
   }
 
@@ -710,7 +710,7 @@ struct Pipeline {
     if (optimize) dw.alloc((size_t)NB * sh.vol * (sh.fp64 ? sizeof(double) : sizeof(float)));
     B0.alloc((size_t)NB * sh.nw * sh.p_sel * sh.m * es);
     factors.alloc((size_t)NB * nvar * sh.p_sel * sh.fstride * es);
-    if (optimize && divq_needed(sh)) divq.alloc((size_t)NB * nvar * sh.p_sel * ((sh.nw + 3) & ~3u) * es);
+    if (optimize) divq.alloc((size_t)NB * nvar * sh.p_sel * ((sh.nw + 3) & ~3u) * es);
     gather_blocks = pgsvt::gather_blocks(sh, 0);
     partial.alloc((size_t)NB * eval_blocks(sh, 3));
     partial_b.alloc((size_t)NB * gather_blocks * 5);
@@ -766,7 +766,7 @@ struct Pipeline {
                         s, svd_mapping),
              "svd");
     });
-    if (optimize && nvar_run == 3 && divq_needed(sh)) {
+    if (optimize && nvar_run == 3) {
       // units: matrices
       prof->run("divergence_factors", (double)nb * nvar_run * sh.p_sel, [&] {
         hipchk(launch_divq(factors.p, pos.p, sel_d.p, dw.p, nb, nvar_run, sh, divq.p, s), "divergence factors");
@@ -807,9 +807,8 @@ struct Pipeline {
     // factor rows it reads)
     const bool count = g_prof_on && !prof->suspended;
     prof->run("eval_reconstruct", (double)na * 3 * sh.p_sel, [&] {
-      hipchk(launch_eval(factors.p, pos.p, sel_d.p, meta_d.p, wl_d.p, na, nvar, 3, sh,
-                         p.exponential_weighting, dw.p, inv.p, B0.p, partial.p, s, nullptr,
-                         count ? partial_k.p : nullptr, divq.p),
+      hipchk(launch_eval(factors.p, meta_d.p, wl_d.p, na, nvar, 3, sh, p.exponential_weighting, divq.p, inv.p,
+                         B0.p, partial.p, s, nullptr, count ? partial_k.p : nullptr),
              "eval");
     });
     // units: voxels
@@ -842,8 +841,8 @@ struct Pipeline {
     hipchk(hipMemcpyAsync(meta_d.p, ms.data(), nb * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
     hipchk(hipMemcpyAsync(wl_d.p, lst.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
     prof->run("final_frame", (double)nb * sh.p_sel, [&] {
-      hipchk(launch_eval(factors.p, pos.p, sel_d.p, meta_d.p, wl_d.p, nb, nvar, 1, sh,
-                         p.exponential_weighting, nullptr, inv.p, B0.p, nullptr, s),
+      hipchk(launch_eval(factors.p, meta_d.p, wl_d.p, nb, nvar, 1, sh, p.exponential_weighting, nullptr, inv.p,
+                         B0.p, nullptr, s),
              "eval final");
       hipchk(launch_gather(B0.p, off.p, invw.p, u64.p, meta_d.p, wl_d.p, nb, sh, 1, Yout, nullptr, s),
              "gather final");
@@ -1273,9 +1272,8 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
                "grid lambdas");
         for (uint32_t c0 = 0; c0 < slots; c0 += C) {
           const uint32_t na = std::min(C, slots - c0);
-          hipchk(launch_eval(P.factors.p, P.pos.p, P.sel_d.p, P.meta_d.p, wl_g.p + c0, na, P.nvar, 3, P.sh,
-                             p.exponential_weighting, P.dw.p, P.inv.p, P.B0.p, P.partial.p, P.s, lam_g.p + c0,
-                             nullptr, P.divq.p),
+          hipchk(launch_eval(P.factors.p, P.meta_d.p, wl_g.p + c0, na, P.nvar, 3, P.sh, p.exponential_weighting,
+                             P.divq.p, P.inv.p, P.B0.p, P.partial.p, P.s, lam_g.p + c0),
                  "grid eval");
           hipchk(launch_gather(P.B0.p, P.off.p, P.invw.p, P.u64.p, P.meta_d.p, wl_g.p + c0, na, P.sh, 0, nullptr,
                                P.partial_b.p, P.s),
@@ -1286,8 +1284,8 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
         }
         hipchk(launch_grid_select(sums_g.p, noise_g.p, nb, G, OoN, P.eps2, lam_g.p, P.meta_d.p, risk_g.p, P.s),
                "grid select");
-        hipchk(launch_eval(P.factors.p, P.pos.p, P.sel_d.p, P.meta_d.p, P.wl_d.p, nb, P.nvar, 1, P.sh,
-                           p.exponential_weighting, nullptr, P.inv.p, P.B0.p, nullptr, P.s),
+        hipchk(launch_eval(P.factors.p, P.meta_d.p, P.wl_d.p, nb, P.nvar, 1, P.sh, p.exponential_weighting, nullptr,
+                           P.inv.p, P.B0.p, nullptr, P.s),
                "eval final");
         hipchk(launch_gather(P.B0.p, P.off.p, P.invw.p, P.u64.p, P.meta_d.p, P.wl_d.p, nb, P.sh, 1, P.Yout, nullptr,
                              P.s),
@@ -1744,8 +1742,8 @@ int pgsvt_window_reconstruct(const void* X, int dtype, uint32_t n_rows, uint32_t
     hipchk(hipMemcpyAsync(P.wl_d.p, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, P.s), "H2D list");
     DBuf<double> out;
     out.alloc(P.sh.vol);
-    hipchk(launch_eval(P.factors.p, P.pos.p, P.sel_d.p, P.meta_d.p, P.wl_d.p, 1, P.nvar, 1, P.sh,
-                       P.p.exponential_weighting, nullptr, P.inv.p, P.B0.p, nullptr, P.s),
+    hipchk(launch_eval(P.factors.p, P.meta_d.p, P.wl_d.p, 1, P.nvar, 1, P.sh, P.p.exponential_weighting, nullptr,
+                       P.inv.p, P.B0.p, nullptr, P.s),
            "eval");
     hipchk(launch_gather(P.B0.p, P.off.p, P.invw.p, P.u64.p, P.meta_d.p, P.wl_d.p, 1, P.sh, 2, out.p,
                          nullptr, P.s),

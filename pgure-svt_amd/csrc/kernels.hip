@@ -857,100 +857,81 @@ __device__ __forceinline__ void block_count(double* out, int v) {
 //   B = U diag(g(s)) V^T = sum over kept ranks r of (g(s_r)/s_r) w_r v_r^T
 // with w_r = X v_r = s_r u_r, stored by the SVD.  Factors are sorted by
 // descending s and g(s)/s grows with s, so the kept ranks are a prefix and
-// only their rows are read.  A group of L lanes owns one (patch, variant),
-// lane q one row of B.
-// nvar_run == 3 (PGURE): the groups of a block take (patch, variant) pairs
-// for the variants U, U + eps2 delta2, U - eps2 delta2 and exchange their
-// reconstructions through LDS; then, frame by frame, the U row goes to `B0`
-// ([slot][frame][patch in bucket order][row]) for the gather and
-// sum_c (B1_c - 2 B0_c + B2_c) delta2_v / W_v, the divergence sum of
-// src/pgure.hpp:136 (linear in the overlap-add), is accumulated in FP64.
-// nvar_run == 1: every group reconstructs U of its own patch.
+// only their rows are read.  A group of L lanes owns one patch, lane q one
+// row of B; the U reconstruction's row of every frame goes to its
+// bucket-sorted place in `B0` ([slot][frame][patch in bucket order][row])
+// for the gather.
+// nvar_run == 3 (PGURE): the divergence sum sum_v (B1 - 2 B0 + B2)_v
+// delta2_v / W_v of src/pgure.hpp:136 comes from the divergence factors of
+// the variants U, U + eps2 delta2, U - eps2 delta2 (k_divq) as
+// sum_r (d1_r q1_r - 2 d0_r q0_r + d2_r q2_r), in FP64.
 template <int BS, int N, int L, class T>
 __global__ __launch_bounds__(256) void k_eval(
-    const T* __restrict__ factors, const int32_t* __restrict__ pos, const uint32_t* __restrict__ sel,
-    const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl, uint32_t nvar_stored, uint32_t nvar_run,
-    uint32_t p_sel, uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim,
-    uint32_t fstride, int exp_w, const T* __restrict__ dw, const int32_t* __restrict__ inv,
-    T* __restrict__ B0, double* __restrict__ partial, const double* __restrict__ slot_lam,
-    double* __restrict__ kept) {
-  constexpr int M = BS * BS, G = 256 / L, SR = (N + 3) & ~3, P3 = G / 3;
-  __shared__ T xch[P3][3][M][N];  // reconstructions of the three variants
+    const T* __restrict__ factors, const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl,
+    uint32_t nvar_stored, uint32_t nvar_run, uint32_t p_sel, uint32_t nw, uint32_t fstride, int exp_w,
+    const T* __restrict__ divq, const int32_t* __restrict__ inv, T* __restrict__ B0,
+    double* __restrict__ partial, const double* __restrict__ slot_lam, double* __restrict__ kept) {
+  constexpr int M = BS * BS, G = 256 / L, SR = (N + 3) & ~3;
   __shared__ double red[4];
   uint32_t bx, slot;
   xcd_block(bx, slot);
   const uint32_t b = wl[slot];
   const int grp = threadIdx.x / L, lane = threadIdx.x % L;
-  int pl, var;
-  uint32_t p;
-  if (nvar_run == 3) {
-    pl = grp / 3;
-    var = (grp < 3 * P3) ? grp % 3 : -1;
-    p = bx * P3 + pl;
-  } else {
-    pl = grp;
-    var = 0;
-    p = bx * G + grp;
-  }
-  const bool valid = var >= 0 && p < p_sel;
+  const uint32_t p = bx * G + grp;
+  const bool valid = p < p_sel;
   const bool row = valid && lane < M;
+  const uint32_t pa = valid ? p : 0;  // addresses of groups past the end: patch 0
   const double lam = slot_lam ? slot_lam[slot] : meta[b].lambda;
-  const T* F = factors + (((size_t)b * nvar_stored + (valid ? var : 0)) * p_sel + (valid ? p : 0)) * fstride;
+  const size_t vstep = (size_t)p_sel * fstride;  // one variant's factors to the next
+  const T* F = factors + ((size_t)b * nvar_stored * p_sel + pa) * fstride;
   const T* FW = F + N * SR;
   const T* FS = FW + N * M;
-  // Loads that do not depend on the reconstruction go out first: the
-  // singular values, then (PGURE) the bucket places and trajectory of the
-  // frames this group finishes (var, var + 3, ...).
-  constexpr int KP = (N + 2) / 3;
-  const bool pg = nvar_run == 3 && row;
-  T sl = T(0), smax = T(0);
-  if (valid && lane < N) {
-    sl = FS[lane];
-    smax = FS[0];
-  }
-  // In a PGURE launch every lane issues these loads, lanes without a row
-  // (and the idle group) at in-bounds addresses of patch 0 / row M - 1:
-  // no lane skips a load, which keeps the memory waits below exact.
-  const bool pgl = nvar_run == 3;
-  const uint32_t pa = valid ? p : 0;
-  const int va = max(var, 0);
-  // every patch selected (step 1): the selection is the identity
-  const uint32_t it = pgl ? (p_sel == p_all ? pa : sel[pa]) : 0;
-  int32_t ib[KP], pk[KP];
-  if (pgl) {
-    const int32_t* I = inv + (size_t)b * nw * p_sel + pa;
+  const bool pgq = nvar_run == 3;
+  // singular values (lane r: s_r) of U, and with PGURE those of the
+  // perturbed variants and the divergence factors of all three
+  const int lk = min(lane, N - 1);
+  const T sl = FS[lk], smax = FS[0];
+  T sp[2] = {T(0), T(0)}, spm[2] = {T(0), T(0)}, ql[3] = {T(0), T(0), T(0)};
+  if (pgq) {
+    const T* Q = divq + ((size_t)b * nvar_stored * p_sel + pa) * SR;
 #pragma unroll
-    for (int i = 0; i < KP; ++i) {
-      const int k = min(va + 3 * i, N - 1);
-      ib[i] = I[(size_t)k * p_sel];
-      pk[i] = pos[((size_t)b * nw + k) * p_all + it];
+    for (int v = 0; v < 3; ++v) ql[v] = Q[(size_t)v * p_sel * SR + lk];
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      sp[v] = FS[(v + 1) * vstep + lk];
+      spm[v] = FS[(v + 1) * vstep];
     }
   }
-  // delta2 / W at the voxels of those frames (PGURE), also independent of
-  // the reconstruction.  Issued after the factor loads of the rank loop
-  // (loads complete in order, so the products wait for the factors only).
-  double dwv[KP];
-  auto load_dw = [&] {
-    const int lq = min(lane, M - 1);
-    const int rr = lq % BS, cc = lq / BS;
-    const T* DW = dw + (size_t)b * vol;
-#pragma unroll
-    for (int i = 0; i < KP; ++i) {
-      const int k = min(va + 3 * i, N - 1);
-      const uint32_t r0 = min((uint32_t)(pk[i] & 0xffff), lim), c0 = min((uint32_t)(pk[i] >> 16), lim);
-      dwv[i] = DW[(uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc)];
+  // shrink factor g(s_r)/s_r in lane r < N; kept ranks: 1 + highest lane
+  // of the group with g > 0
+  auto shrink = [&](T s, T s0) {
+    T d = T(0);
+    if (valid && lane < N) {
+      const T w = exp_w ? fabs(s0 * exp((T)(-0.5 * lam) * (s * s))) : (T)lam;
+      d = s > T(0) ? fmax(s - w, T(0)) / s : T(0);
     }
+    return d;
   };
-  // shrink factor g(s_r)/s_r in lane r < N; s_0 is the largest
-  T dl = T(0);
-  if (valid && lane < N) {
-    const T w = exp_w ? fabs(smax * exp((T)(-0.5 * lam) * (sl * sl))) : (T)lam;
-    dl = sl > T(0) ? fmax(sl - w, T(0)) / sl : T(0);
+  auto kept_ranks = [&](T d) {
+    const uint64_t bal = __ballot(d > T(0));
+    const uint32_t gbits =
+        (uint32_t)((bal >> ((threadIdx.x % 64) / L * L)) & ((L >= 64) ? ~0ull : ((1ull << L) - 1)));
+    return gbits ? 32 - __clz(gbits) : 0;
+  };
+  const T dl = shrink(sl, smax);
+  const int nkeep = kept_ranks(dl);
+  double s4 = 0.0;
+  int kr = nkeep;  // kept ranks of the evaluated variants (profiling)
+  if (pgq) {
+    // lanes past the kept ranks have d_r = 0
+    s4 = -2.0 * (double)dl * (double)ql[0];
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const T d = shrink(sp[v], spm[v]);
+      kr += kept_ranks(d);
+      s4 = fma((double)d, (double)ql[v + 1], s4);
+    }
   }
-  // kept ranks of this group: 1 + highest lane with g > 0
-  const uint64_t bal = __ballot(dl > T(0));
-  const uint32_t gbits = (uint32_t)((bal >> ((threadIdx.x % 64) / L * L)) & ((L >= 64) ? ~0ull : ((1ull << L) - 1)));
-  const int nkeep = gbits ? 32 - __clz(gbits) : 0;
   T bq[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) bq[k] = T(0);
@@ -971,12 +952,7 @@ __global__ __launch_bounds__(256) void k_eval(
         wr[r] = FW[r * M + lw];
       }
     }
-    if (pgl) {  // uniform: each path has its own exact waits
-      load_dw();
-      rank_products<0, N>(nkeep, dl, vr, wr, bq);
-    } else {
-      rank_products<0, N>(nkeep, dl, vr, wr, bq);
-    }
+    rank_products<0, N>(nkeep, dl, vr, wr, bq);
   } else {
   // rank loop, loads of rank r + 1 issued before the products of rank r
   // (FP32: the V^T row as SR/4 float4 loads; FP64: SR scalar loads)
@@ -991,7 +967,6 @@ __global__ __launch_bounds__(256) void k_eval(
     for (int j = 0; j < NV; ++j) vn[j] = vr[j];
   };
   if (nkeep > 0) fetch(0);
-  if (pgl) load_dw();
   for (int r = 0; r < nkeep; ++r) {
     const T y = __shfl(dl, r, L) * wn;
     VT vc[NV];
@@ -1012,30 +987,7 @@ __global__ __launch_bounds__(256) void k_eval(
     }
   }
   }
-  double s4 = 0.0;
-  if (nvar_run == 3) {
-    // The three groups of a patch share the per-frame work: group `var`
-    // takes frames var, var + 3, ...: the U row goes to its bucket-sorted
-    // place in B0 and (B1 - 2 B0 + B2) delta2_v / W_v is accumulated.
-    if (pg) {
-#pragma unroll
-      for (int k = 0; k < N; ++k) xch[pl][var][lane][k] = bq[k];
-    }
-    __syncthreads();
-    if (pg) {
-#pragma unroll
-      for (int i = 0; i < KP; ++i) {
-        const int k = var + 3 * i;
-        if (k < N) {
-          const T x0 = xch[pl][0][lane][k];
-          B0[(((size_t)slot * nw + k) * p_sel + ib[i]) * M + lane] = x0;
-          const double c2 =
-              ((double)xch[pl][1][lane][k] + (double)xch[pl][2][lane][k]) - 2.0 * (double)x0;
-          s4 = fma(c2, dwv[i], s4);
-        }
-      }
-    }
-  } else if (row) {
+  if (row) {
     const int32_t* I = inv + (size_t)b * nw * p_sel + p;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
@@ -1043,12 +995,12 @@ __global__ __launch_bounds__(256) void k_eval(
       B0[(((size_t)slot * nw + k) * p_sel + I[(size_t)k * p_sel]) * M + lane] = bq[k];
     }
   }
-  if (nvar_run != 3) return;
+  if (!pgq) return;  // uniform over the launch
   for (int o = 32; o > 0; o >>= 1) s4 += __shfl_xor(s4, o);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = s4;
   __syncthreads();
   if (threadIdx.x == 0) partial[(size_t)slot * gridDim.x + bx] = ((red[0] + red[1]) + red[2]) + red[3];
-  if (kept) block_count(kept + (size_t)slot * gridDim.x + bx, (lane == 0 && valid) ? nkeep : 0);
+  if (kept) block_count(kept + (size_t)slot * gridDim.x + bx, (lane == 0 && valid) ? kr : 0);
 }
 
 // Lane K of each 16-lane row, for 32-bit integers (trajectory positions,
@@ -1067,29 +1019,48 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
-// Divergence factors for patches up to 4 x 4, once per window after the
-// SVD.  The divergence sum of src/pgure.hpp:136 is linear in the
-// reconstructions: for B = sum_r d_r w_r v_r^T (d_r = g(s_r)/s_r, the only
-// part that depends on lambda),
+// Lane K of the group of L lanes (16: DPP row broadcast, 64: the wave's
+// lane K), in every lane of the group.
+template <int L, int K, class V>
+__device__ __forceinline__ V group_bcast(V v) {
+  static_assert(L == 16 || L == 64, "groups of 16 lanes or whole waves");
+  if constexpr (L == 16) {
+    return row_bcast<K>(v);
+  } else if constexpr (sizeof(V) == 8) {
+    const int2 h = __builtin_bit_cast(int2, v);
+    int2 o;
+    o.x = __builtin_amdgcn_readlane(h.x, K);
+    o.y = __builtin_amdgcn_readlane(h.y, K);
+    return __builtin_bit_cast(V, o);
+  } else {
+    return __builtin_bit_cast(V, __builtin_amdgcn_readlane(__builtin_bit_cast(int32_t, v), K));
+  }
+}
+
+// Divergence factors, once per window after the SVD.  The divergence sum of
+// src/pgure.hpp:136 is linear in the reconstructions: for
+// B = sum_r d_r w_r v_r^T (d_r = g(s_r)/s_r, the only part that depends on
+// lambda),
 //   sum over the patch's voxels of delta2_v / W_v B_v = sum_r d_r q_r,
 //   q_r = w_r^T D v_r,
 // D (m x n) the delta2/W volume at the patch's voxels along its trajectory.
-// One 16-lane group per (patch, variant): lane q holds row q of D, lane j
+// One group of L lanes per (patch, variant): lane q holds row q of D, lane j
 // element j of each V^T row; t_q = sum_k D[q][k] v_r[k] and q_r = sum_q
 // w_r[q] t_q are formed in FP64 and stored in rank order,
-// divq [nb][nvar][p_sel][sr].
-template <int BS, int N, class T>
+// divq [nb][nvar][p_sel][sr].  L = 16 (patches up to 4 x 4): every row is
+// loaded first; L = 64: one rank at a time.
+template <int BS, int N, int L, class T>
 __global__ __launch_bounds__(256) void k_divq(const T* __restrict__ factors, const int32_t* __restrict__ pos,
                                               const uint32_t* __restrict__ sel, const T* __restrict__ dw,
                                               uint32_t nvar, uint32_t p_sel, uint32_t p_all, uint32_t nr,
                                               uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim,
                                               uint32_t fstride, T* __restrict__ divq) {
-  constexpr int M = BS * BS, SR = (N + 3) & ~3;
-  static_assert(M <= 16 && SR <= 16, "one 16-lane row per patch");
+  constexpr int M = BS * BS, SR = (N + 3) & ~3, G = 256 / L;
+  static_assert(M <= L && SR <= L, "one lane per row of D and per element of a V^T row");
   uint32_t bx, b;
   xcd_block(bx, b);
-  const int lane = threadIdx.x % 16;
-  const uint32_t g = bx * 16 + threadIdx.x / 16;  // (variant, patch), patch fastest
+  const int lane = threadIdx.x % L;
+  const uint32_t g = bx * G + threadIdx.x / L;  // (variant, patch), patch fastest
   const bool valid = g < nvar * p_sel;
   const uint32_t ga = valid ? g : 0;  // groups past the end read pair 0
   const uint32_t var = ga / p_sel, p = ga % p_sel;
@@ -1098,34 +1069,45 @@ __global__ __launch_bounds__(256) void k_divq(const T* __restrict__ factors, con
   const T* FW = F + N * SR;
   const int lk = min(lane, N - 1), lv = min(lane, SR - 1), lw = min(lane, M - 1);
   const int rr = lw % BS, cc = lw / BS;
-  // every V^T and W^T row first, then D frame by frame (lane k: frame k's
-  // trajectory position)
-  T vr[N], wr[N];
-#pragma unroll
-  for (int r = 0; r < N; ++r) {
-    vr[r] = F[r * SR + lv];
-    wr[r] = FW[r * M + lw];
-  }
+  // D frame by frame (lane k: frame k's trajectory position)
   const int32_t pk_l = pos[((size_t)b * nw + lk) * p_all + it];
   const T* DW = dw + (size_t)b * vol;
   double d[N];
   static_for<0, N>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    const int32_t pkk = row_bcast<k>(pk_l);
+    const int32_t pkk = group_bcast<L, k>(pk_l);
     const uint32_t r0 = min((uint32_t)(pkk & 0xffff), lim), c0 = min((uint32_t)(pkk >> 16), lim);
     d[k] = (double)DW[(uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc)];
   });
-  double qv = 0.0;  // lane r: q_r
-  static_for<0, N>([&](auto rc) {
-    constexpr int r = decltype(rc)::value;
+  // q_r of rank r from lane j's V^T element and lane q's W^T element
+  auto factor = [&](T vr, T wr) {
     double t = 0.0;
     static_for<0, N>([&](auto kc) {
       constexpr int k = decltype(kc)::value;
-      t = fma(d[k], (double)row_bcast<k>(vr[r]), t);
+      t = fma(d[k], (double)group_bcast<L, k>(vr), t);
     });
-    const double y = group_sum<16>(lane < M ? (double)wr[r] * t : 0.0);
-    if (lane == r) qv = y;
-  });
+    return group_sum<L>(lane < M ? (double)wr * t : 0.0);
+  };
+  double qv = 0.0;  // lane r: q_r
+  if constexpr (L == 16) {
+    T vr[N], wr[N];
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      vr[r] = F[r * SR + lv];
+      wr[r] = FW[r * M + lw];
+    }
+    static_for<0, N>([&](auto rc) {
+      constexpr int r = decltype(rc)::value;
+      const double y = factor(vr[r], wr[r]);
+      if (lane == r) qv = y;
+    });
+  } else {
+#pragma unroll 1
+    for (int r = 0; r < N; ++r) {
+      const double y = factor(F[r * SR + lv], FW[r * M + lw]);
+      if (lane == r) qv = y;
+    }
+  }
   if (valid && lane < SR) divq[(((size_t)b * nvar + var) * p_sel + p) * SR + lane] = (T)qv;
 }
 
@@ -1644,23 +1626,21 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
 }
 
 uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run) {
+  (void)nvar_run;  // one group per patch either way
   const uint32_t G = (sh.m <= 16) ? 16 : (sh.m <= 32 ? 8 : 4);
-  // PGURE: one patch per 16-lane group (k_eval_pg), else three groups
-  const uint32_t per = (nvar_run == 3 && sh.m > 16) ? G / 3 : G;
-  return (sh.p_sel + per - 1) / per;
+  return (sh.p_sel + G - 1) / G;
 }
 
-hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
-                       const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh,
-                       int exp_w, const void* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s,
-                       const double* slot_lam, double* kept, const void* divq) {
-  const uint32_t lim = sh.nr - sh.bs;
+hipError_t launch_eval(const void* factors, const WinMeta* meta, const uint32_t* wl, uint32_t na,
+                       uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh, int exp_w, const void* divq,
+                       const int32_t* inv, void* B0, double* partial, hipStream_t s, const double* slot_lam,
+                       double* kept) {
+  if (nvar_run == 3 && !divq) return hipErrorInvalidValue;  // launch_divq after the SVD
   const uint32_t nblk = eval_blocks(sh, nvar_run);
 #define PG_EV(B, N, L, LS)                                                                                       \
   if (sh.bs == B && sh.nw == N) {                                                                              \
     if constexpr (L == 16) {                                                                                   \
       if (nvar_run == 3) {                                                                                     \
-        if (!divq) return hipErrorInvalidValue; /* launch_divq after the SVD */                               \
         if (sh.fp64)                                                                                           \
           hipLaunchKernelGGL((k_eval_pg<B, N, double>), dim3(nblk, na), dim3(256), 0, s,                       \
                              (const double*)factors, meta, wl, nvar_stored, sh.p_sel, sh.nw, sh.fstride, exp_w, \
@@ -1674,12 +1654,12 @@ hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* 
     }                                                                                                          \
     if (sh.fp64)                                                                                               \
       hipLaunchKernelGGL((k_eval<B, N, L, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)factors,    \
-                         pos, sel, meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw,    \
-                         sh.vol, lim, sh.fstride, exp_w, (const double*)dw, inv, (double*)B0, partial, slot_lam, kept);       \
+                         meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.nw, sh.fstride, exp_w,                   \
+                         (const double*)divq, inv, (double*)B0, partial, slot_lam, kept);                      \
     else                                                                                                       \
-      hipLaunchKernelGGL((k_eval<B, N, L, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors, pos, \
-                         sel, meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, \
-                         lim, sh.fstride, exp_w, (const float*)dw, inv, (float*)B0, partial, slot_lam, kept);                \
+      hipLaunchKernelGGL((k_eval<B, N, L, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors,      \
+                         meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.nw, sh.fstride, exp_w,                   \
+                         (const float*)divq, inv, (float*)B0, partial, slot_lam, kept);                        \
     return hipGetLastError();                                                                                  \
   }
   PG_SHAPES(PG_EV)
@@ -1687,25 +1667,21 @@ hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* 
   return hipErrorInvalidValue;
 }
 
-bool divq_needed(const Shape& sh) { return sh.m <= 16; }
-
 hipError_t launch_divq(const void* factors, const int32_t* pos, const uint32_t* sel, const void* dw, uint32_t nb,
                        uint32_t nvar, const Shape& sh, void* divq, hipStream_t s) {
   const uint32_t lim = sh.nr - sh.bs;
-  const dim3 grid((nvar * sh.p_sel + 15) / 16, nb);
 #define PG_DQ(B, N, L, LS)                                                                                       \
   if (sh.bs == B && sh.nw == N) {                                                                              \
-    if constexpr (L == 16) {                                                                                   \
-      if (sh.fp64)                                                                                             \
-        hipLaunchKernelGGL((k_divq<B, N, double>), grid, dim3(256), 0, s, (const double*)factors, pos, sel,    \
-                           (const double*)dw, nvar, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim,       \
-                           sh.fstride, (double*)divq);                                                         \
-      else                                                                                                     \
-        hipLaunchKernelGGL((k_divq<B, N, float>), grid, dim3(256), 0, s, (const float*)factors, pos, sel,      \
-                           (const float*)dw, nvar, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim,        \
-                           sh.fstride, (float*)divq);                                                          \
-      return hipGetLastError();                                                                                \
-    }                                                                                                          \
+    const dim3 grid((nvar * sh.p_sel + 256 / L - 1) / (256 / L), nb);                                          \
+    if (sh.fp64)                                                                                               \
+      hipLaunchKernelGGL((k_divq<B, N, L, double>), grid, dim3(256), 0, s, (const double*)factors, pos, sel,    \
+                         (const double*)dw, nvar, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim,         \
+                         sh.fstride, (double*)divq);                                                           \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_divq<B, N, L, float>), grid, dim3(256), 0, s, (const float*)factors, pos, sel,      \
+                         (const float*)dw, nvar, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim,          \
+                         sh.fstride, (float*)divq);                                                            \
+    return hipGetLastError();                                                                                  \
   }
   PG_SHAPES(PG_DQ)
 #undef PG_DQ

@@ -52,20 +52,17 @@ hipError_t launch_buckets(const int32_t* pos, const uint32_t* sel, uint32_t nb, 
 hipError_t launch_weights(const int32_t* off, const uint8_t* d2, uint32_t nb, const Shape& sh, double d2neg,
                           double d2pos, double* invw, void* dw, hipStream_t s);
 // one lambda evaluation of the listed slots; B0 [na][nw][p_sel][m] (rows in
-// bucket order); partial
-// [na][eval_blocks] (divergence sums, only with nvar_run == 3); kept
-// (optional, nvar_run == 3): [na][eval_blocks] kept ranks of each block
-// divq: the divergence factors of launch_divq, required with nvar_run == 3
-// when divq_needed(sh) (dw is then unused)
+// bucket order); partial [na][eval_blocks] (divergence sums, only with
+// nvar_run == 3, which needs divq); kept (optional, nvar_run == 3):
+// [na][eval_blocks] kept ranks of each block
 uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run);
-hipError_t launch_eval(const void* factors, const int32_t* pos, const uint32_t* sel, const WinMeta* meta,
-                       const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh,
-                       int exp_w, const void* dw, const int32_t* inv, void* B0, double* partial, hipStream_t s,
-                       const double* slot_lam = nullptr, double* kept = nullptr, const void* divq = nullptr);
+hipError_t launch_eval(const void* factors, const WinMeta* meta, const uint32_t* wl, uint32_t na,
+                       uint32_t nvar_stored, uint32_t nvar_run, const Shape& sh, int exp_w, const void* divq,
+                       const int32_t* inv, void* B0, double* partial, hipStream_t s,
+                       const double* slot_lam = nullptr, double* kept = nullptr);
 // divergence factors w_r^T D v_r of every rank of every (patch, variant),
-// D the patch's delta2/W along its trajectory: divq [nb][nvar][p_sel][sr]
-// (sr = nw rounded up to 4), after the SVD; for patches up to 4 x 4
-bool divq_needed(const Shape& sh);
+// D the patch's delta2/W along its trajectory (dw): divq [nb][nvar][p_sel][sr]
+// (sr = nw rounded up to 4), after the SVD
 hipError_t launch_divq(const void* factors, const int32_t* pos, const uint32_t* sel, const void* dw, uint32_t nb,
                        uint32_t nvar, const Shape& sh, void* divq, hipStream_t s);
 // overlap-add gather: mode 0 risk sums into partial [na][gather_blocks][5]; mode 1
