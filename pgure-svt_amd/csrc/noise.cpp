@@ -3,12 +3,14 @@
 // Same arithmetic, in the same order, as the reference: quadtree with the
 // 2.5% F-test, Huber IRLS robust mean, MAD robust variance of the
 // Laplacian-filtered leaf, robust WLS line fit of variance against mean.
-// Three changes make it fast without changing a single rounding:
+// Four changes make it fast without changing a single rounding:
 //   * the robust mean's interquartile distance is taken on r = A - m; since
 //     rounding is monotone, sorted(r)[k] = fl(sorted(A)[k] - m), so the two
 //     order statistics of A it needs are selected once per leaf instead of
 //     sorting r in every IRLS iteration;
 //   * medians and order statistics use nth_element instead of full sorts;
+//   * each robust-mean iteration is one pass over the leaf (the reference's
+//     two passes fused, every sum in its own order);
 //   * repeated quadtree nodes are evaluated once.
 #include "noise.hpp"
 
@@ -19,6 +21,13 @@
 
 namespace pgsvt {
 namespace {
+
+typedef double v2d __attribute__((vector_size(16)));
+typedef int64_t v2l __attribute__((vector_size(16)));
+inline v2d vabs(v2d x) {
+  const v2l mask = {INT64_MAX, INT64_MAX};
+  return (v2d)((v2l)x & mask);
+}
 
 // Armadillo accu order: even indices, odd indices, then their sum.
 template <class F>
@@ -81,45 +90,60 @@ double robust_mean(const std::vector<double>& A) {
   }
   const double tol = 1e-6, eps = 1e-12;
   double m = 0, m0 = 1e12;
-  std::vector<double> w(N, 1.0);
-  // Each sum keeps its own even/odd accumulator pair in index order, so
-  // fusing the passes changes no rounding: pass 1 forms sum(w A) and
-  // sum(w); pass 2 forms sum |A - m| and the next weights, which are only
-  // used when the loop goes on.
+  // sum(w A) and sum(w), each with its own even/odd accumulator pair in
+  // index order; the first weights are all 1
+  double wa1 = 0, wa2 = 0, w1 = 0, w2 = 0;
+  size_t k = 0;
+  for (; k + 1 < N; k += 2) {
+    wa1 += A[k];
+    wa2 += A[k + 1];
+    w1 += 1.;
+    w2 += 1.;
+  }
+  if (k < N) {
+    wa1 += A[k];
+    w1 += 1.;
+  }
+  // One pass per iteration: sum |A - m| of this iteration, the next
+  // weights and, from them, the next iteration's sum(w A) and sum(w).
+  // Every sum keeps its accumulators and order, so fusing the reference's
+  // two passes changes no rounding (the sums of the last weights go
+  // unused when the loop ends).
   for (uint32_t it = 0; it < 10000; ++it) {
-    double wa1 = 0, wa2 = 0, w1 = 0, w2 = 0;
-    size_t k = 0;
-    for (; k + 1 < N; k += 2) {
-      wa1 += w[k] * A[k];
-      wa2 += w[k + 1] * A[k + 1];
-      w1 += w[k];
-      w2 += w[k + 1];
-    }
-    if (k < N) {
-      wa1 += w[k] * A[k];
-      w1 += w[k];
-    }
     m = wa1 + wa2;
     const double aux = w1 + w2;
     m = (std::fabs(aux) < eps) ? m0 : m / aux;
     const double d = ((a_hi - m) - (a_lo - m)) + eps;
     const double inv = 1. / d;
-    double e1 = 0, e2 = 0;
-    auto step = [&](size_t q) {
-      const double r = A[q] - m;
-      const double a = std::fabs(r * inv);
-      // divide for outliers only (inliers keep weight 1)
-      if (__builtin_expect(a < 0.75, 1))
-        w[q] = 1.;
-      else
-        w[q] = 0.75 / a;
-      return std::fabs(r);
-    };
+    // two elements at a time: lane 0 carries the even-index sums, lane 1
+    // the odd ones, as the scalar accumulator pairs do; the Huber weight is
+    // a select (0.75 / a is exact either way), which keeps the outliers,
+    // about a third of a leaf, off the branch predictor
+    const v2d vm = {m, m}, vinv = {inv, inv}, vp = {0.75, 0.75}, one = {1., 1.};
+    v2d ve = {0., 0.}, vwa = {0., 0.}, vws = {0., 0.};
     for (k = 0; k + 1 < N; k += 2) {
-      e1 += step(k);
-      e2 += step(k + 1);
+      v2d av;
+      std::memcpy(&av, &A[k], sizeof av);
+      const v2d r = av - vm;
+      const v2d a = vabs(r * vinv);
+      const v2d wq = (a < vp) ? one : vp / a;
+      ve += vabs(r);
+      vwa += wq * av;
+      vws += wq;
     }
-    if (k < N) e1 += step(k);
+    double e1 = ve[0], e2 = ve[1];
+    wa1 = vwa[0];
+    wa2 = vwa[1];
+    w1 = vws[0];
+    w2 = vws[1];
+    if (k < N) {
+      const double r = A[k] - m;
+      const double a = std::fabs(r * inv);
+      const double wq = (a < 0.75) ? 1. : 0.75 / a;
+      e1 += std::fabs(r);
+      wa1 += wq * A[k];
+      w1 += wq;
+    }
     const double e = (e1 + e2) / (double)N;
     if (std::fabs(m0 - m) < tol || e < tol) break;
     m0 = m;
@@ -149,38 +173,64 @@ bool split_block(const double* F, uint32_t n, uint32_t i0, uint32_t j0, uint32_t
   const double OoR = 1.0 / R, OoRm1 = 1.0 / (R - 1);
   std::vector<double> a(R), rs(R);
   // element k = y + N x (column-major), residual of the 5-point stencil
-  // with the reference's wrap-around neighbours
+  // with the reference's wrap-around neighbours; rows 1 .. N - 2 two at a
+  // time (the same operations per lane), the wrapped rows 0 and N - 1 alone
+  const v2d v5 = {5., 5.}, vsq = {sq, sq};
   for (uint32_t x = 0; x < N; ++x) {
     const uint32_t xp = (x + 1 == N) ? 1 : x + 1, xm = (x == 0) ? N - 2 : x - 1;
-    for (uint32_t y = 0; y < N; ++y) {
+    const double* C = &F[i0 + (size_t)n * (j0 + x)];
+    const double* Cp = &F[i0 + (size_t)n * (j0 + xp)];
+    const double* Cm = &F[i0 + (size_t)n * (j0 + xm)];
+    double* ak = &a[(size_t)N * x];
+    double* rk = &rs[(size_t)N * x];
+    auto one = [&](uint32_t y) {
       const uint32_t yp = (y + 1 == N) ? 1 : y + 1, ym = (y == 0) ? N - 2 : y - 1;
-      const size_t k = y + (size_t)N * x;
-      a[k] = A(y, x);
-      rs[k] = (5u * A(y, x) - (A(yp, x) + A(ym, x) + A(y, xm) + A(y, xp))) / sq;
+      ak[y] = C[y];
+      rk[y] = (5u * C[y] - (C[yp] + C[ym] + Cm[y] + Cp[y])) / sq;
+    };
+    one(0);
+    uint32_t y = 1;
+    for (; y + 2 < N; y += 2) {
+      v2d c, cu, cd, cm, cp;
+      std::memcpy(&c, &C[y], sizeof c);
+      std::memcpy(&cu, &C[y + 1], sizeof cu);
+      std::memcpy(&cd, &C[y - 1], sizeof cd);
+      std::memcpy(&cm, &Cm[y], sizeof cm);
+      std::memcpy(&cp, &Cp[y], sizeof cp);
+      const v2d r = (v5 * c - (cu + cd + cm + cp)) / vsq;
+      std::memcpy(&ak[y], &c, sizeof c);
+      std::memcpy(&rk[y], &r, sizeof r);
     }
+    for (; y < N; ++y) one(y);
   }
-  // four sums, each with its own even/odd accumulator pair (as accu)
-  double z1 = 0, z2 = 0, r1 = 0, r2 = 0;
-  for (size_t k = 0; k + 1 < R; k += 2) {
-    z1 += a[k];
-    z2 += a[k + 1];
-    r1 += rs[k];
-    r2 += rs[k + 1];
+  // four sums, each with its own even/odd accumulator pair (as accu):
+  // lane 0 the even elements, lane 1 the odd ones
+  v2d vz = {0., 0.}, vr = {0., 0.};
+  size_t k = 0;
+  for (; k + 1 < R; k += 2) {
+    v2d av, rv;
+    std::memcpy(&av, &a[k], sizeof av);
+    std::memcpy(&rv, &rs[k], sizeof rv);
+    vz += av;
+    vr += rv;
   }
+  double z1 = vz[0], z2 = vz[1], r1 = vr[0], r2 = vr[1];
   if (R % 2) {
     z1 += a[R - 1];
     r1 += rs[R - 1];
   }
   const double accuZ = (z1 + z2) * OoR, accuR = (r1 + r2) * OoR;
-  double s1 = 0, s2 = 0, e1 = 0, e2 = 0;
-  for (size_t k = 0; k + 1 < R; k += 2) {
-    const double t1 = a[k] - accuZ, t2 = a[k + 1] - accuZ;
-    const double u1 = rs[k] - accuR, u2 = rs[k + 1] - accuR;
-    s1 += t1 * t1;
-    s2 += t2 * t2;
-    e1 += u1 * u1;
-    e2 += u2 * u2;
+  const v2d mz = {accuZ, accuZ}, mr = {accuR, accuR};
+  v2d vs = {0., 0.}, ve = {0., 0.};
+  for (k = 0; k + 1 < R; k += 2) {
+    v2d av, rv;
+    std::memcpy(&av, &a[k], sizeof av);
+    std::memcpy(&rv, &rs[k], sizeof rv);
+    const v2d t = av - mz, u = rv - mr;
+    vs += t * t;
+    ve += u * u;
   }
+  double s1 = vs[0], s2 = vs[1], e1 = ve[0], e2 = ve[1];
   if (R % 2) {
     const double t1 = a[R - 1] - accuZ, u1 = rs[R - 1] - accuR;
     s1 += t1 * t1;
