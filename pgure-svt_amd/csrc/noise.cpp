@@ -8,7 +8,8 @@
 //     rounding is monotone, sorted(r)[k] = fl(sorted(A)[k] - m), so the two
 //     order statistics of A it needs are selected once per leaf instead of
 //     sorting r in every IRLS iteration;
-//   * medians and order statistics use nth_element instead of full sorts;
+//   * medians and order statistics are selected, not sorted (a monotone
+//     histogram narrows large leaves to one bucket first);
 //   * each robust-mean iteration is one pass over the leaf (the reference's
 //     two passes fused, every sum in its own order);
 //   * repeated quadtree nodes are evaluated once.
@@ -42,28 +43,79 @@ inline double accu2f(size_t n, F f) {
   return a1 + a2;
 }
 
-// arma::median: nth_element, even counts average upper and lower middle as
-// upper + (lower - upper) / 2
-double median_inplace(std::vector<double>& v) {
-  const size_t n = v.size(), half = n / 2;
-  std::nth_element(v.begin(), v.begin() + half, v.end());
-  if (n % 2 == 0) {
-    const double v1 = v[half];
-    const double v2 = *std::max_element(v.begin(), v.begin() + half);
-    return v1 + (v2 - v1) / 2.0;
+// Order statistics ks[0 .. nk) (0-based ranks, ascending) of v[0 .. n),
+// exact.  Large inputs go through a histogram: the bucket index
+// floor((x - lo) * scale) never decreases with x (every step rounds
+// monotonically), so the element of rank k is the element of rank
+// k - (count of the lower buckets) within its bucket, selected there.  Small
+// inputs (and degenerate ranges) use nth_element on a copy.
+void order_stats(const double* v, size_t n, const size_t* ks, int nk, double* out) {
+  auto small = [&] {
+    std::vector<double> c(v, v + n);
+    for (int j = 0; j < nk; ++j) {
+      std::nth_element(c.begin(), c.begin() + ks[j], c.end());
+      out[j] = c[ks[j]];
+    }
+  };
+  if (n < 4096) return small();
+  double lo = v[0], hi = v[0];
+  for (size_t i = 1; i < n; ++i) {
+    lo = std::min(lo, v[i]);
+    hi = std::max(hi, v[i]);
   }
-  return v[half];
+  constexpr uint32_t B = 4096;
+  const double scale = (double)(B - 1) / (hi - lo);
+  if (!(hi > lo) || !std::isfinite(scale)) return small();
+  auto bucket = [&](double x) { return std::min((uint32_t)((x - lo) * scale), B - 1); };
+  std::vector<uint32_t> cnt(B, 0);
+  for (size_t i = 0; i < n; ++i) ++cnt[bucket(v[i])];
+  uint32_t bj[2];
+  size_t below[2];
+  for (int j = 0; j < nk; ++j) {
+    size_t c = 0;
+    uint32_t b = 0;
+    while (c + cnt[b] <= ks[j]) c += cnt[b++];
+    bj[j] = b;
+    below[j] = c;
+  }
+  std::vector<double> t[2];
+  for (int j = 0; j < nk; ++j) t[j].reserve(cnt[bj[j]]);
+  for (size_t i = 0; i < n; ++i) {
+    const uint32_t b = bucket(v[i]);
+    for (int j = 0; j < nk; ++j)
+      if (b == bj[j]) t[j].push_back(v[i]);
+  }
+  for (int j = 0; j < nk; ++j) {
+    const size_t r = ks[j] - below[j];
+    std::nth_element(t[j].begin(), t[j].begin() + r, t[j].end());
+    out[j] = t[j][r];
+  }
+}
+
+// arma::median: even counts average the upper and lower middle as
+// upper + (lower - upper) / 2
+double median_of(const std::vector<double>& v) {
+  const size_t n = v.size(), half = n / 2;
+  if (n % 2 == 0) {
+    const size_t ks[2] = {half, half - 1};
+    double o[2];
+    order_stats(v.data(), n, ks, 2, o);
+    return o[0] + (o[1] - o[0]) / 2.0;
+  }
+  double o;
+  order_stats(v.data(), n, &half, 1, &o);
+  return o;
 }
 
 uint32_t iqd_m(uint32_t N) { return (uint32_t)std::floor((std::floor((N + 1) / 2) + 1) / 2); }
 
 // interquartile distance of an unsorted vector (two order statistics)
-double iqd_select(std::vector<double> v) {
+double iqd_select(const std::vector<double>& v) {
   const uint32_t N = (uint32_t)v.size(), m = iqd_m(N);
-  std::nth_element(v.begin(), v.begin() + (N - m - 1), v.end());
-  const double hi = v[N - m - 1];
-  std::nth_element(v.begin(), v.begin() + (m - 1), v.begin() + (N - m - 1));
-  return hi - v[m - 1];
+  const size_t ks[2] = {N - m - 1, m - 1};
+  double o[2];
+  order_stats(v.data(), N, ks, 2, o);
+  return o[0] - o[1];
 }
 
 inline void huber(const double* x, double* w, size_t n) {
@@ -82,11 +134,11 @@ double robust_mean(const std::vector<double>& A) {
   const uint32_t mq = iqd_m((uint32_t)N);
   double a_hi, a_lo;
   {
-    std::vector<double> c(A);
-    std::nth_element(c.begin(), c.begin() + (N - mq - 1), c.end());
-    a_hi = c[N - mq - 1];
-    std::nth_element(c.begin(), c.begin() + (mq - 1), c.begin() + (N - mq - 1));
-    a_lo = c[mq - 1];
+    const size_t ks[2] = {N - mq - 1, mq - 1};
+    double o[2];
+    order_stats(A.data(), N, ks, 2, o);
+    a_hi = o[0];
+    a_lo = o[1];
   }
   const double tol = 1e-6, eps = 1e-12;
   double m = 0, m0 = 1e12;
@@ -153,10 +205,9 @@ double robust_mean(const std::vector<double>& A) {
 
 // RobustVarianceEstimate (src/noise.hpp:232-236)
 double robust_var(std::vector<double> a) {
-  std::vector<double> c(a);
-  const double med = median_inplace(c);
+  const double med = median_of(a);
   for (double& v : a) v = std::fabs(v - med);
-  const double sig = 1.4826 * median_inplace(a);
+  const double sig = 1.4826 * median_of(a);
   return sig * sig;
 }
 
@@ -167,7 +218,6 @@ const double kF0025[12] = {15.4392, 2.86209, 1.64602, 1.27893, 1.13046, 1.06318,
 // SplitBlockQ (src/noise.hpp:182-221) on the N x N block at (i0, j0)
 bool split_block(const double* F, uint32_t n, uint32_t i0, uint32_t j0, uint32_t N) {
   if (N <= 8) return false;
-  auto A = [&](uint32_t y, uint32_t x) { return F[(i0 + y) + (size_t)n * (j0 + x)]; };
   const double sq = std::sqrt((double)(5 * 5 + 5));
   const uint32_t R = N * N;
   const double OoR = 1.0 / R, OoRm1 = 1.0 / (R - 1);

@@ -158,6 +158,18 @@ def test_noise_estimator_matches_oracle(seed, n, t0):
     assert g == o and g[0] == 0.5 and g[2] == 0.1
 
 
+@pytest.mark.parametrize("levels", [7, 255])
+def test_noise_estimator_matches_oracle_on_quantised_frames(levels):
+    # integer-valued frames put long runs of equal values into every order
+    # statistic and histogram bucket of the estimator's selections, and flat
+    # clipped regions give leaves of a single value
+    clean, noisy = phantom.noisy_sequence(128, 15, drift=1, alpha=1.0, sigma=8.0, seed=3)
+    q = np.clip(np.round(noisy * (levels / noisy.max())), 0, levels - 2)
+    u = q.astype(np.float64) / q.max()
+    for method in (4, 2):
+        assert G.noise_estimate(u, method) == O.noise_estimate(u, method), method
+
+
 def test_noise_estimator_recovers_known_line():
     # a robust WLS line fit on exact (mean, var) pairs must recover the line:
     # pure Gaussian noise of std s on a flat frame gives alpha ~ 0 and sigma ~ s
