@@ -43,18 +43,27 @@ inline double accu2f(size_t n, F f) {
   return a1 + a2;
 }
 
-// Order statistics ks[0 .. nk) (0-based ranks, ascending) of v[0 .. n),
-// exact.  Large inputs go through a histogram: the bucket index
+// Order statistics of ranks ks[0] > ks[1] > ... (nk of them, 0-based) of
+// v[0 .. n), exact.  Large inputs go through a histogram: the bucket index
 // floor((x - lo) * scale) never decreases with x (every step rounds
 // monotonically), so the element of rank k is the element of rank
 // k - (count of the lower buckets) within its bucket, selected there.  Small
-// inputs (and degenerate ranges) use nth_element on a copy.
+// inputs (and degenerate ranges) use nth_element on a copy; after selecting
+// rank k, every smaller rank lies in the first k elements (the next rank
+// down is their maximum).
 void order_stats(const double* v, size_t n, const size_t* ks, int nk, double* out) {
   auto small = [&] {
-    std::vector<double> c(v, v + n);
+    thread_local std::vector<double> c;
+    c.assign(v, v + n);
+    size_t end = n;
     for (int j = 0; j < nk; ++j) {
-      std::nth_element(c.begin(), c.begin() + ks[j], c.end());
-      out[j] = c[ks[j]];
+      if (ks[j] + 1 == end)
+        out[j] = *std::max_element(c.begin(), c.begin() + end);
+      else {
+        std::nth_element(c.begin(), c.begin() + ks[j], c.begin() + end);
+        out[j] = c[ks[j]];
+      }
+      end = ks[j];
     }
   };
   if (n < 4096) return small();
