@@ -446,13 +446,27 @@ __device__ __forceinline__ float group_sum(float v) {
   return v;
 }
 
-// FP64 (the precision-study path): a lane-swap butterfly, same property
-// (every lane of the group ends with the bit-identical sum)
+// FP64 (the divergence factors and the precision-study path): the same
+// DPP steps on both halves of the double; after each step the lanes that
+// are paired next hold equal partial sums, so the result is the one of
+// the lane-swap (xor) butterfly and bit-identical in every lane
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int2 h = __builtin_bit_cast(int2, v);
+  int2 o;
+  o.x = __builtin_amdgcn_update_dpp(0, h.x, CTRL, 0xF, 0xF, false);
+  o.y = __builtin_amdgcn_update_dpp(0, h.y, CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, o);
+}
 template <int L>
 __device__ __forceinline__ double group_sum(double v) {
   asm volatile("" : "+v"(v));
-#pragma unroll
-  for (int o = 1; o < L; o <<= 1) v += __shfl_xor(v, o);
+  if (L >= 2) v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+  if (L >= 4) v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  if (L >= 8) v += dpp_d<0x141>(v);  // row_half_mirror
+  if (L >= 16) v += dpp_d<0x140>(v); // row_mirror
+  if (L >= 32) v += __shfl_xor(v, 16);
+  if (L >= 64) v += __shfl_xor(v, 32);
   return v;
 }
 
