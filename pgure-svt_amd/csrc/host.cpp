@@ -540,7 +540,7 @@ struct Pipeline {
   DBuf<double> partial_k, kept_d;  // kept ranks of each eval block / window (profiling)
   DBuf<double> u64, w64, invw, partial, partial_b, sums;
   DBuf<char> dw;  // delta2 / W per voxel, in the reconstruction's type
-  DBuf<char> divq;  // divergence factors of every rank (PGURE)
+  DBuf<char> divq;  // per (patch, variant) singular values and divergence factors (PGURE)
   // SVD input variants, factors and U reconstructions: FP32, or FP64 for
   // the precision study (sh.fp64); sized in bytes
   DBuf<char> uvar, factors, B0;
@@ -659,7 +659,7 @@ struct Pipeline {
     b += 8ull * sh.vol * (optimize ? 2 : 1);             // 1/W, delta2/W
     b += es * sh.nw * (size_t)sh.p_sel * sh.m;           // B0
     b += es * nvar * (size_t)sh.p_sel * sh.fstride;      // factors
-    if (optimize) b += es * nvar * (size_t)sh.p_sel * ((sh.nw + 3) & ~3u);  // divq
+    if (optimize) b += es * nvar * (size_t)sh.p_sel * 2 * ((sh.nw + 3) & ~3u);  // divq
     return b;
   }
 
@@ -710,7 +710,7 @@ struct Pipeline {
     if (optimize) dw.alloc((size_t)NB * sh.vol * (sh.fp64 ? sizeof(double) : sizeof(float)));
     B0.alloc((size_t)NB * sh.nw * sh.p_sel * sh.m * es);
     factors.alloc((size_t)NB * nvar * sh.p_sel * sh.fstride * es);
-    if (optimize) divq.alloc((size_t)NB * nvar * sh.p_sel * ((sh.nw + 3) & ~3u) * es);
+    if (optimize) divq.alloc((size_t)NB * nvar * sh.p_sel * 2 * ((sh.nw + 3) & ~3u) * es);
     gather_blocks = pgsvt::gather_blocks(sh, 0);
     partial.alloc((size_t)NB * eval_blocks(sh, 3));
     partial_b.alloc((size_t)NB * gather_blocks * 5);

@@ -896,8 +896,7 @@ __global__ __launch_bounds__(256) void k_eval(
   const bool row = valid && lane < M;
   const uint32_t pa = valid ? p : 0;  // addresses of groups past the end: patch 0
   const double lam = slot_lam ? slot_lam[slot] : meta[b].lambda;
-  const size_t vstep = (size_t)p_sel * fstride;  // one variant's factors to the next
-  const T* F = factors + ((size_t)b * nvar_stored * p_sel + pa) * fstride;
+  const T* F = factors + ((size_t)b * nvar_stored * p_sel + pa) * fstride;  // the U variant's factors
   const T* FW = F + N * SR;
   const T* FS = FW + N * M;
   const bool pgq = nvar_run == 3;
@@ -907,13 +906,14 @@ __global__ __launch_bounds__(256) void k_eval(
   const T sl = FS[lk], smax = FS[0];
   T sp[2] = {T(0), T(0)}, spm[2] = {T(0), T(0)}, ql[3] = {T(0), T(0), T(0)};
   if (pgq) {
-    const T* Q = divq + ((size_t)b * nvar_stored * p_sel + pa) * SR;
+    // the patch's record: per variant sr singular values, sr factors
+    const T* Q = divq + ((size_t)b * p_sel + pa) * nvar_stored * 2 * SR;
 #pragma unroll
-    for (int v = 0; v < 3; ++v) ql[v] = Q[(size_t)v * p_sel * SR + lk];
+    for (int v = 0; v < 3; ++v) ql[v] = Q[v * 2 * SR + SR + lk];
 #pragma unroll
     for (int v = 0; v < 2; ++v) {
-      sp[v] = FS[(v + 1) * vstep + lk];
-      spm[v] = FS[(v + 1) * vstep];
+      sp[v] = Q[(v + 1) * 2 * SR + lk];
+      spm[v] = Q[(v + 1) * 2 * SR];
     }
   }
   // shrink factor g(s_r)/s_r in lane r < N; kept ranks: 1 + highest lane
@@ -1060,9 +1060,11 @@ __device__ __forceinline__ V group_bcast(V v) {
 // D (m x n) the delta2/W volume at the patch's voxels along its trajectory.
 // One group of L lanes per (patch, variant): lane q holds row q of D, lane j
 // element j of each V^T row; t_q = sum_k D[q][k] v_r[k] and q_r = sum_q
-// w_r[q] t_q are formed in FP64 and stored in rank order,
-// divq [nb][nvar][p_sel][sr].  L = 16 (patches up to 4 x 4): every row is
-// loaded first; L = 64: one rank at a time.
+// w_r[q] t_q are formed in FP64 and stored in rank order in the patch's
+// record, divq [nb][p_sel][nvar][2][sr]: the variant's singular values
+// (zero-padded to sr), then its factors; the search rounds read s and q of
+// the three variants from there.  L = 16 (patches up to 4 x 4): every row
+// is loaded first; L = 64: one rank at a time.
 template <int BS, int N, int L, class T>
 __global__ __launch_bounds__(256) void k_divq(const T* __restrict__ factors, const int32_t* __restrict__ pos,
                                               const uint32_t* __restrict__ sel, const T* __restrict__ dw,
@@ -1083,6 +1085,7 @@ __global__ __launch_bounds__(256) void k_divq(const T* __restrict__ factors, con
   const T* FW = F + N * SR;
   const int lk = min(lane, N - 1), lv = min(lane, SR - 1), lw = min(lane, M - 1);
   const int rr = lw % BS, cc = lw / BS;
+  const T sv = FW[N * M + lk];  // s_lane
   // D frame by frame (lane k: frame k's trajectory position)
   const int32_t pk_l = pos[((size_t)b * nw + lk) * p_all + it];
   const T* DW = dw + (size_t)b * vol;
@@ -1122,7 +1125,11 @@ __global__ __launch_bounds__(256) void k_divq(const T* __restrict__ factors, con
       if (lane == r) qv = y;
     }
   }
-  if (valid && lane < SR) divq[(((size_t)b * nvar + var) * p_sel + p) * SR + lane] = (T)qv;
+  if (valid && lane < SR) {
+    T* rec = divq + (((size_t)b * p_sel + p) * nvar + var) * 2 * SR;
+    rec[lane] = lane < N ? sv : T(0);
+    rec[SR + lane] = (T)qv;
+  }
 }
 
 // PGURE evaluation for patches up to 4 x 4: one 16-lane group (one DPP
@@ -1151,18 +1158,18 @@ __global__ __launch_bounds__(256) void k_eval_pg(
   const bool row = valid && lane < M;
   const uint32_t pa = valid ? p : 0;  // addresses of groups past the end: patch 0
   const double lam = slot_lam ? slot_lam[slot] : meta[b].lambda;
-  const size_t vstep = (size_t)p_sel * fstride;  // one variant's factors to the next
-  const T* F = factors + ((size_t)b * nvar_stored * p_sel + pa) * fstride;
-  const T* Q = divq + ((size_t)b * nvar_stored * p_sel + pa) * SR;
+  const T* F = factors + ((size_t)b * nvar_stored * p_sel + pa) * fstride;  // the U variant's factors
   // singular values and divergence factors of the three variants (lane r:
-  // s_r, q_r), frame lk = min(lane, N - 1)'s bucket place
+  // s_r, q_r) from the patch's record (three whole 128-byte lines in FP32
+  // for n <= 16), frame lk = min(lane, N - 1)'s bucket place
+  const T* Q = divq + ((size_t)b * p_sel + pa) * nvar_stored * 2 * SR;
   const int lk = min(lane, N - 1);
   T sl[3], smax[3], ql[3];
 #pragma unroll
   for (int v = 0; v < 3; ++v) {
-    sl[v] = F[v * vstep + N * SR + N * M + lk];
-    smax[v] = F[v * vstep + N * SR + N * M];
-    ql[v] = Q[(size_t)v * p_sel * SR + lk];
+    sl[v] = Q[v * 2 * SR + lk];
+    smax[v] = Q[v * 2 * SR];
+    ql[v] = Q[v * 2 * SR + SR + lk];
   }
   const int32_t ib_l = inv[((size_t)b * nw + lk) * p_sel + pa];
   const int lv = min(lane, SR - 1), lw = min(lane, M - 1);

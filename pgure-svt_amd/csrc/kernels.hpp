@@ -61,8 +61,9 @@ hipError_t launch_eval(const void* factors, const WinMeta* meta, const uint32_t*
                        const int32_t* inv, void* B0, double* partial, hipStream_t s,
                        const double* slot_lam = nullptr, double* kept = nullptr);
 // divergence factors w_r^T D v_r of every rank of every (patch, variant),
-// D the patch's delta2/W along its trajectory (dw): divq [nb][nvar][p_sel][sr]
-// (sr = nw rounded up to 4), after the SVD
+// D the patch's delta2/W along its trajectory (dw), after the SVD: divq
+// [nb][p_sel][nvar][2][sr] (sr = nw rounded up to 4), per (patch, variant)
+// its singular values (zero-padded) then its factors
 hipError_t launch_divq(const void* factors, const int32_t* pos, const uint32_t* sel, const void* dw, uint32_t nb,
                        uint32_t nvar, const Shape& sh, void* divq, hipStream_t s);
 // overlap-add gather: mode 0 risk sums into partial [na][gather_blocks][5]; mode 1
