@@ -1065,6 +1065,54 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
       P.Yout = P.Yd.p;
     }
   }
+  // Buffers sized by the requested windows (small) come before the batch,
+  // whose allocation halves the batch until it fits beside concurrent calls.
+  // learned lambda: fixed patch sample, feature buffer, network
+  std::unique_ptr<Mlp> net;
+  DBuf<uint32_t> sample_d;
+  DBuf<float> feat_d, mean_d, istd_d, yout_d;
+  const uint32_t nlog = kFeatureSamples * g.Nw;
+  if (learned || rs.mode == 3) {
+    const std::vector<uint32_t> smp = feature_sample(P.sh.p_sel, kFeatureSamples);
+    sample_d.alloc(smp.size());
+    hipchk(hipMemcpyAsync(sample_d.p, smp.data(), smp.size() * 4, hipMemcpyHostToDevice, P.s), "H2D sample");
+    feat_d.alloc((size_t)count * n_in);
+    yout_d.alloc(count);
+    if (rs.mode == 1) {
+      net.reset(new Mlp(n_in, rs.model->hidden, P.s));
+      hipchk(hipMemcpyAsync(net->params(), rs.model->params, net->n_params() * 4, hipMemcpyHostToDevice, P.s),
+             "H2D model");
+      mean_d.alloc(n_in);
+      istd_d.alloc(n_in);
+      hipchk(hipMemcpyAsync(mean_d.p, rs.model->mean, n_in * 4, hipMemcpyHostToDevice, P.s), "H2D mean");
+      hipchk(hipMemcpyAsync(istd_d.p, rs.model->inv_std, n_in * 4, hipMemcpyHostToDevice, P.s), "H2D inv_std");
+    }
+    hipchk(hipStreamSynchronize(P.s), "model sync");  // host vectors above are local
+  }
+
+  // grid mode: per-batch buffers and the captured graph
+  const uint32_t G = rs.grid;
+  DBuf<double> lam_g, sums_g, risk_g, noise_g;
+  DBuf<uint32_t> wl_g;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  uint32_t graph_nb = 0;
+  struct GraphOwner {
+    hipGraph_t* g;
+    hipGraphExec_t* e;
+    ~GraphOwner() {
+      if (*e) (void)hipGraphExecDestroy(*e);
+      if (*g) (void)hipGraphDestroy(*g);
+    }
+  } graph_owner{&graph, &gexec};
+  int svd_mapping = 0;
+  if (rs.mode == 3) {
+    lam_g.alloc((size_t)count * G);
+    sums_g.alloc((size_t)count * G * 6);
+    risk_g.alloc((size_t)count * G);
+    wl_g.alloc((size_t)count * G);
+    noise_g.alloc((size_t)count * 3);
+  }
   {
     HostTimer t("host_alloc_batch");
     P.alloc_batch(count);
@@ -1148,52 +1196,6 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
     pool.reset(new PrepPool(std::move(jobs), count, thread_count(p.n_jobs)));
   }
 
-  // learned lambda: fixed patch sample, feature buffer, network
-  std::unique_ptr<Mlp> net;
-  DBuf<uint32_t> sample_d;
-  DBuf<float> feat_d, mean_d, istd_d, yout_d;
-  const uint32_t nlog = kFeatureSamples * g.Nw;
-  if (learned || rs.mode == 3) {
-    const std::vector<uint32_t> smp = feature_sample(P.sh.p_sel, kFeatureSamples);
-    sample_d.alloc(smp.size());
-    hipchk(hipMemcpyAsync(sample_d.p, smp.data(), smp.size() * 4, hipMemcpyHostToDevice, P.s), "H2D sample");
-    feat_d.alloc((size_t)P.NB * n_in);
-    yout_d.alloc(P.NB);
-    if (rs.mode == 1) {
-      net.reset(new Mlp(n_in, rs.model->hidden, P.s));
-      hipchk(hipMemcpyAsync(net->params(), rs.model->params, net->n_params() * 4, hipMemcpyHostToDevice, P.s),
-             "H2D model");
-      mean_d.alloc(n_in);
-      istd_d.alloc(n_in);
-      hipchk(hipMemcpyAsync(mean_d.p, rs.model->mean, n_in * 4, hipMemcpyHostToDevice, P.s), "H2D mean");
-      hipchk(hipMemcpyAsync(istd_d.p, rs.model->inv_std, n_in * 4, hipMemcpyHostToDevice, P.s), "H2D inv_std");
-    }
-    hipchk(hipStreamSynchronize(P.s), "model sync");  // host vectors above are local
-  }
-
-  // grid mode: per-batch buffers and the captured graph
-  const uint32_t G = rs.grid;
-  DBuf<double> lam_g, sums_g, risk_g, noise_g;
-  DBuf<uint32_t> wl_g;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  uint32_t graph_nb = 0;
-  struct GraphOwner {
-    hipGraph_t* g;
-    hipGraphExec_t* e;
-    ~GraphOwner() {
-      if (*e) (void)hipGraphExecDestroy(*e);
-      if (*g) (void)hipGraphDestroy(*g);
-    }
-  } graph_owner{&graph, &gexec};
-  int svd_mapping = 0;
-  if (rs.mode == 3) {
-    lam_g.alloc((size_t)P.NB * G);
-    sums_g.alloc((size_t)P.NB * G * 6);
-    risk_g.alloc((size_t)P.NB * G);
-    wl_g.alloc((size_t)P.NB * G);
-    noise_g.alloc((size_t)P.NB * 3);
-  }
 
   int status = 0;
   std::string warn;
