@@ -44,6 +44,7 @@
 #include <memory>
 #include <mutex>
 #include <random>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -476,6 +477,9 @@ struct NoiseEstimator {
       case 1: {
         const int L = (int)std::floor(1. * (Nx * Ny / m2.size()));
         const uint32_t ie = (uint32_t)std::round(0.05 * L);
+        // RestrictArray(a, 0, ie) is bounds-checked in the reference and
+        // aborts when the window has fewer leaves (noise.hpp:384-392)
+        if ((size_t)ie + 1 > rm.size()) throw std::runtime_error("noise method 1: too few quadtree leaves");
         std::vector<double> ra(rm.begin(), rm.begin() + ie + 1), rb(rv.begin(), rv.begin() + ie + 1);
         mu = (mu >= 0.) ? mu : ComputeMode(ra);
         dSi = ComputeMode(rb);
@@ -1104,12 +1108,16 @@ static WindowResult run_window(const void* X, int dtype, uint32_t nr, uint32_t n
   Window w = make_window(X, dtype, nr, nc, nf, g, t, Z);
   if (p.optimize_pgure) {
     NoiseEstimator ne(p.noise_method);
-    ne.Estimate(w.u, alpha, mu, sigma);
+    try {
+      ne.Estimate(w.u, alpha, mu, sigma);
+    } catch (const std::runtime_error&) {  // the reference aborts here
+      res.status = PGSVT_E_NUMERIC;
+      return res;
+    }
   }
   std::vector<int32_t> patches = window_patches(w, p, g, t, nf);
   PGURE opt(w.u, patches, alpha, sigma, mu, p.patch_size, p.patch_overlap, p.random_seed,
             p.exponential_weighting != 0, p.optimize_pgure != 0);
-  res.status = 0;
   if (p.optimize_pgure) {
     const double OoNxNyNt = 1.0 / (nc * nr * g.Nt);
     double start = (lambda >= 0.0) ? lambda : accu2(w.u.d.data(), w.u.n()) * OoNxNyNt;
@@ -1275,7 +1283,8 @@ int oracle_denoise(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t 
     est[t + (size_t)nf * 3] = res[t].sigma;
   }
   for (uint32_t t = 0; t < nf; ++t)
-    if (res[t].status) return fail(res[t].status, "optimiser failure");
+    if (res[t].status)
+      return fail(res[t].status, "window " + std::to_string(t) + ": noise estimate or optimiser failure");
   return 0;
 }
 
@@ -1353,8 +1362,63 @@ int oracle_noise_estimate(const double* u, uint32_t nr, uint32_t nc, uint32_t nt
   Cube c(nr, nc, nt);
   std::memcpy(c.d.data(), u, c.n() * sizeof(double));
   NoiseEstimator ne(method);
-  ne.Estimate(c, *alpha, *mu, *sigma);
+  try {
+    ne.Estimate(c, *alpha, *mu, *sigma);
+  } catch (const std::runtime_error& e) {
+    return fail(PGSVT_E_NUMERIC, e.what());
+  }
   return 0;
+}
+
+// The estimator's pieces on their own (src/noise.hpp), so that tests can
+// check each against a closed form.
+int oracle_noise_robust_mean(const double* a, uint32_t n, double* out) {  // noise.hpp:238-271
+  NoiseEstimator ne(4);
+  *out = ne.RobustMean(std::vector<double>(a, a + n));
+  return 0;
+}
+
+int oracle_noise_robust_var(const double* a, uint32_t n, double* out) {  // noise.hpp:232-236
+  *out = NoiseEstimator::RobustVar(std::vector<double>(a, a + n));
+  return 0;
+}
+
+int oracle_noise_mode(const double* a, uint32_t n, double* out) {  // noise.hpp:273-301
+  NoiseEstimator ne(4);
+  *out = ne.ComputeMode(std::vector<double>(a, a + n));
+  return 0;
+}
+
+int oracle_noise_wls(const double* x, const double* y, uint32_t n, double* a, double* b) {  // noise.hpp:328-383
+  NoiseEstimator ne(4);
+  ne.WLSFit(std::vector<double>(x, x + n), std::vector<double>(y, y + n), *a, *b);
+  return 0;
+}
+
+// F-test split decision for one N x N column-major block (noise.hpp:182-221)
+int oracle_noise_split(const double* block, uint32_t N, int32_t* split) {
+  NoiseEstimator ne(4);
+  *split = ne.SplitBlockQ(block, N, 0, 0, N) ? 1 : 0;
+  return 0;
+}
+
+// Quadtree nodes of one N x N frame after the parents are shed
+// (noise.hpp:55-72, 419-458): (row, col, size) triples in tree order;
+// returns the node count, writes at most cap triples.
+uint32_t oracle_noise_tree(const double* frame, uint32_t N, uint32_t* nodes, uint32_t cap) {
+  NoiseEstimator ne(4);
+  ne.tree.assign(1, {0u, 0u, N});
+  ne.del.clear();
+  ne.QuadTree(frame, N, 0);
+  std::vector<std::array<uint32_t, 3>> t2 = ne.tree;
+  std::vector<uint32_t> dl = ne.del;
+  std::sort(dl.begin(), dl.end());
+  dl.erase(std::unique(dl.begin(), dl.end()), dl.end());
+  if (!dl.empty())
+    for (size_t k = dl.size() - 1; k > 0; --k) t2.erase(t2.begin() + dl[k]);
+  for (size_t k = 0; k < t2.size() && k < cap; ++k)
+    for (int q = 0; q < 3; ++q) nodes[3 * k + q] = t2[k][q];
+  return (uint32_t)t2.size();
 }
 
 int oracle_median_filter(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, int r, double* Z) {

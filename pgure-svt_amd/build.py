@@ -37,8 +37,19 @@ SOURCES = [
 ]
 
 
-def _compile(src, flags):
+def _stale(obj, src):
+    """True when obj is missing or older than its source or any header."""
+    if not os.path.exists(obj):
+        return True
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    deps.append(os.path.join(HERE, "..", "include", "pgsvt.h"))
+    return any(os.path.getmtime(d) > os.path.getmtime(obj) for d in deps)
+
+
+def _compile(src, flags, incremental=False):
     obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
+    if incremental and not _stale(obj, src):
+        return obj
     cmd = [HIPCC] + COMMON + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -50,12 +61,14 @@ def _compile(src, flags):
     return obj
 
 
-def build(jobs=4):
+def build(jobs=4, incremental=False):
+    """Compile every source (incremental: only those older than a source or
+    header they may include) and link the library and the front end."""
     if not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found at " + HIPCC)
     os.makedirs(OUT_DIR, exist_ok=True)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda a: _compile(*a), SOURCES))
+        objs = list(ex.map(lambda a: _compile(a[0], a[1], incremental), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-pthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -77,5 +90,6 @@ def build(jobs=4):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--incremental", action="store_true", help="recompile only out-of-date objects")
     a = ap.parse_args()
-    print(build(a.jobs))
+    print(build(a.jobs, a.incremental))

@@ -457,7 +457,14 @@ uint64_t seed_of(const pgsvt_params& p) {
 // and windows become ready roughly in order, as the search admits them.
 struct WindowPrep {
   double alpha = -1, mu = -1, sigma = -1, start = 0;
-  std::string err;  // non-empty: the estimate failed
+  std::string err;       // non-empty: the estimate failed
+  bool numeric = false;  // ... because the estimate is not defined (NoiseError)
+  // rethrow a failure on the calling thread
+  void check() const {
+    if (err.empty()) return;
+    if (numeric) throw NoiseError(err);
+    throw std::runtime_error(err);
+  }
 };
 
 class PrepPool {
@@ -1185,6 +1192,9 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
           const double OoNxNyNt = 1.0 / (nc * nr * g.Nt);
           const double start = (lambda0 >= 0.0) ? lambda0 : P.host_window_accu(t) * OoNxNyNt;
           w.start = std::max(0.0, start);
+        } catch (const NoiseError& e) {
+          w.err = std::string("noise estimate: ") + e.what();
+          w.numeric = true;
         } catch (const std::exception& e) {
           w.err = std::string("noise estimate: ") + e.what();
         } catch (...) {
@@ -1211,7 +1221,7 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
         HostTimer tw("host_noise_wait");
         for (uint32_t b = 0; b < nb; ++b) {
           prep[b] = pool->wait(t0 + b - rs.tb);
-          if (!prep[b].err.empty()) throw std::runtime_error(prep[b].err);
+          prep[b].check();
           nz[3 * b] = prep[b].alpha;
           nz[3 * b + 1] = prep[b].mu;
           nz[3 * b + 2] = prep[b].sigma;
@@ -1328,7 +1338,7 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
         HostTimer tw("host_noise_wait");
         for (uint32_t b = 0; b < nb; ++b) {
           prep[b] = pool->wait(t0 + b - rs.tb);
-          if (!prep[b].err.empty()) throw std::runtime_error(prep[b].err);
+          prep[b].check();
           const double raw[3] = {prep[b].alpha, prep[b].mu, prep[b].sigma};
           for (int q = 0; q < 3; ++q) {
             float v = (float)raw[q];
@@ -1369,7 +1379,7 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
       uint32_t n_started = 0;
       auto admit = [&](uint32_t b) {
         prep[b] = pool->wait(t0 + b - rs.tb);
-        if (!prep[b].err.empty()) throw std::runtime_error(prep[b].err);
+        prep[b].check();
         const double start = prep[b].start;
         const double ub = std::max(100.0, start);
         opt[b].reset(new Sbplx1D(start, 0.0, ub, std::sqrt(start), p.tol, 1e-12, (int)p.max_iter));
@@ -1479,6 +1489,8 @@ template <class F>
 int guarded(F f) {
   try {
     return f();
+  } catch (const NoiseError& e) {
+    return fail(PGSVT_E_NUMERIC, e.what());
   } catch (const HipError& e) {
     return fail(PGSVT_E_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
   } catch (const std::bad_alloc&) {

@@ -19,6 +19,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <string>
 
 namespace pgsvt {
 namespace {
@@ -446,6 +447,9 @@ void estimate_from_leaves(const std::vector<std::shared_ptr<const FrameLeaves>>&
     case 1: {
       const int L = (int)std::floor(1. * ((uint32_t)(n * n) / (uint32_t)means.size()));
       const uint32_t ie = (uint32_t)std::round(0.05 * L);
+      if ((size_t)ie + 1 > rm.size())
+        throw NoiseError("noise method 1 needs " + std::to_string((size_t)ie + 1) + " quadtree leaves, the window has " +
+                         std::to_string(rm.size()));
       std::vector<double> ra(rm.begin(), rm.begin() + ie + 1), rb(rv.begin(), rv.begin() + ie + 1);
       mu = (mu >= 0.) ? mu : compute_mode(ra);
       const double dSi = compute_mode(rb);

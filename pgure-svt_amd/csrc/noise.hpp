@@ -3,9 +3,18 @@
 
 #include <cstdint>
 #include <memory>
+#include <stdexcept>
 #include <vector>
 
 namespace pgsvt {
+
+// An estimate the reference cannot form: noise method 1 reads the first
+// round(0.05 * L) + 1 leaf statistics of the window (L = pixels per frame /
+// leaves), through a bounds-checked copy that aborts the reference when
+// fewer leaves exist (src/noise.hpp:115-121, 384-392).
+struct NoiseError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
 
 // Robust (mean, variance) of every quadtree leaf of one normalised frame, in
 // leaf order (src/noise.hpp:55-100).

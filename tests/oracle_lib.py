@@ -83,6 +83,13 @@ def lib():
         L.oracle_perturbations.argtypes = [i64, u64, vp, vp]
         L.oracle_patch_indices.argtypes = [u32, u32, u32, vp]
         L.oracle_patch_indices.restype = u32
+        L.oracle_noise_robust_mean.argtypes = [vp, u32, dp]
+        L.oracle_noise_robust_var.argtypes = [vp, u32, dp]
+        L.oracle_noise_mode.argtypes = [vp, u32, dp]
+        L.oracle_noise_wls.argtypes = [vp, vp, u32, dp, dp]
+        L.oracle_noise_split.argtypes = [vp, u32, C.POINTER(i32)]
+        L.oracle_noise_tree.argtypes = [vp, u32, vp, u32]
+        L.oracle_noise_tree.restype = u32
         FN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
         L.FN = FN
         L.oracle_sbplx.argtypes = [FN, vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -225,4 +232,52 @@ def hotpixel(seq, threshold):
     out = np.asfortranarray(seq.astype(np.uint16, copy=True))
     nr, nc, nf = out.shape
     _check(lib().oracle_hotpixel(out.ctypes.data, nr, nc, nf, threshold))
+    return out
+
+
+# ---- the noise estimator's pieces (src/noise.hpp) ----
+def _vec(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def noise_robust_mean(a):
+    a = _vec(a)
+    out = C.c_double()
+    _check(lib().oracle_noise_robust_mean(a.ctypes.data, len(a), C.byref(out)))
+    return out.value
+
+
+def noise_robust_var(a):
+    a = _vec(a)
+    out = C.c_double()
+    _check(lib().oracle_noise_robust_var(a.ctypes.data, len(a), C.byref(out)))
+    return out.value
+
+
+def noise_mode(a):
+    a = _vec(a)
+    out = C.c_double()
+    _check(lib().oracle_noise_mode(a.ctypes.data, len(a), C.byref(out)))
+    return out.value
+
+
+def noise_wls(x, y):
+    x, y = _vec(x), _vec(y)
+    a, b = C.c_double(), C.c_double()
+    _check(lib().oracle_noise_wls(x.ctypes.data, y.ctypes.data, len(x), C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
+def noise_split(block):
+    b = np.asfortranarray(block, dtype=np.float64)
+    out = C.c_int32()
+    _check(lib().oracle_noise_split(b.ctypes.data, b.shape[0], C.byref(out)))
+    return bool(out.value)
+
+
+def noise_tree(frame):
+    f = np.asfortranarray(frame, dtype=np.float64)
+    n = lib().oracle_noise_tree(f.ctypes.data, f.shape[0], None, 0)
+    out = np.zeros((n, 3), dtype=np.uint32)
+    lib().oracle_noise_tree(f.ctypes.data, f.shape[0], out.ctypes.data, n)
     return out

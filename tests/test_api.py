@@ -178,3 +178,40 @@ def test_noise_estimator_recovers_known_line():
     a, m, s = G.noise_estimate(u, 4)
     assert abs(m - 0.3) < 2e-3
     assert abs(s - 0.01) < 2e-3
+
+
+def test_noise_method1_too_few_leaves_is_numeric_error():
+    # an all-zero window has one quadtree leaf per frame, fewer than noise
+    # method 1 reads (round(0.05 * 4369) + 1 = 219 of 15); the reference
+    # aborts in its bounds-checked copy (src/noise.hpp:115-121, 384-392), the
+    # library returns PGSVT_E_NUMERIC with the counts in the message
+    u = np.zeros((256, 256, 15))
+    with pytest.raises(P.PGSVTError, match="error -3.*219 quadtree leaves.*15"):
+        G.noise_estimate(u, 1)
+    # the other methods need no such count
+    for method in (2, 3, 4):
+        assert G.noise_estimate(u, method) == O.noise_estimate(u, method)
+
+
+@pytest.mark.parametrize("method", [1, 2, 3])
+@pytest.mark.parametrize("kind", ["flat", "textured"])
+def test_noise_methods_1_to_3_match_oracle(method, kind):
+    # methods 1-3 (the modes of the leaf statistics, src/noise.hpp:115-147)
+    # on a flat noisy frame (few, large leaves) and a textured one (many
+    # small leaves): the product's estimator equals the oracle bit for bit
+    rng = np.random.default_rng(7 + method)
+    if kind == "flat":
+        u = 0.4 + 0.02 * rng.standard_normal((64, 64, 9))
+    else:
+        clean, noisy = phantom.noisy_sequence(64, 9, drift=1, alpha=0.5, sigma=5.0, seed=method)
+        u = noisy.astype(np.float64)
+    u = u / u.max()
+    if kind == "flat" and method == 1:
+        # a flat frame is one leaf, 9 leaves against the 24 method 1 reads:
+        # both refuse (the reference aborts)
+        with pytest.raises(P.PGSVTError, match="error -3"):
+            G.noise_estimate(u, method)
+        with pytest.raises(RuntimeError):
+            O.noise_estimate(u, method)
+        return
+    assert G.noise_estimate(u, method) == O.noise_estimate(u, method)
