@@ -45,7 +45,11 @@ std::vector<uint32_t> entry_values(const Reader& R, size_t e) {
   const uint32_t n = R.u32(e + 4);
   const size_t size = (type == 3) ? 2 : (type == 4) ? 4 : 0;
   if (size == 0) throw std::runtime_error("TIFF: unsupported field type");
+  // every value must lie inside the file before anything is allocated
+  if (n == 0) throw std::runtime_error("TIFF: empty field");
+  if (size * (size_t)n > R.buf.size()) throw std::runtime_error("TIFF: field larger than the file");
   const size_t at = (size * n <= 4) ? e + 8 : R.u32(e + 8);
+  if (at + size * (size_t)n > R.buf.size()) throw std::runtime_error("TIFF: field outside the file");
   std::vector<uint32_t> v(n);
   for (uint32_t k = 0; k < n; ++k) v[k] = (size == 2) ? R.u16(at + 2 * k) : R.u32(at + 4 * k);
   return v;
@@ -122,6 +126,8 @@ void tiff_read_u16(const char* path, uint32_t first, uint32_t count, uint16_t* o
     if (p.spp != 1) throw std::runtime_error("TIFF: only one sample per pixel is supported");
     if (p.bits != 8 && p.bits != 16) throw std::runtime_error("TIFF: only 8-bit and 16-bit pages are supported");
     if (p.offsets.size() != p.counts.size()) throw std::runtime_error("TIFF: bad strip tables");
+    if (p.rows_per_strip == 0) throw std::runtime_error("TIFF: RowsPerStrip is 0");
+    if (W == 0 || H == 0) throw std::runtime_error("TIFF: empty page");
     const uint32_t bpp = p.bits / 8, rps = std::min(p.rows_per_strip, H);
     uint16_t* F = out + (size_t)k * W * H;
     for (uint32_t row = 0; row < H; ++row) {

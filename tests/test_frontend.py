@@ -79,8 +79,9 @@ def test_tiff_roundtrip(tmp_path):
     assert np.array_equal(sub, cube[:, :, 2:4])
 
 
-def _write_baseline(path, pages, bits, endian):
-    """Hand-built uncompressed TIFF with one strip per page."""
+def _write_baseline(path, pages, bits, endian, override=None):
+    """Hand-built uncompressed TIFF with one strip per page; override maps a
+    tag to the (type, count, value) written in its place."""
     e = endian
     H, W = pages[0].shape
     dt = np.dtype(e + ("u2" if bits == 16 else "u1"))
@@ -95,6 +96,7 @@ def _write_baseline(path, pages, bits, endian):
         out[link:link + 4] = struct.pack(e + "I", ifd)
         ents = [(256, 4, 1, W), (257, 4, 1, H), (258, 3, 1, bits), (259, 3, 1, 1), (262, 3, 1, 1),
                 (273, 4, 1, off), (277, 3, 1, 1), (278, 4, 1, H), (279, 4, 1, W * H * bits // 8)]
+        ents = [(t, *override[t]) if override and t in override else (t, ty, c, v) for t, ty, c, v in ents]
         out += struct.pack(e + "H", len(ents))
         for tag, typ, cnt, val in ents:
             v = struct.pack(e + "HH", val, 0) if typ == 3 else struct.pack(e + "I", val)
@@ -114,6 +116,26 @@ def test_tiff_reads_baseline_files(tmp_path, bits, endian):
     assert b == bits
     for k in range(3):
         assert np.array_equal(cube[:, :, k], pages[k])
+
+
+@pytest.mark.parametrize("override,msg", [
+    ({278: (4, 1, 0)}, b"RowsPerStrip is 0"),                   # row / rows_per_strip would divide by 0
+    ({256: (4, 0, 9)}, b"empty field"),                         # a width with no value
+    ({273: (4, 0x3fffffff, 8)}, b"field larger than the file"),  # count far beyond the file: no allocation
+    ({279: (4, 40, 1 << 30)}, b"field outside the file"),       # values stored past the end
+])
+def test_tiff_rejects_malformed_fields(tmp_path, override, msg):
+    # malformed IFD entries are errors (PGSVT_E_ARG with a message), never a
+    # crash, an out-of-range read or a huge allocation
+    L, P = _lib()
+    f = str(tmp_path / "bad.tif")
+    _write_baseline(f, [np.zeros((9, 9))], 16, "<", override)
+    out = np.zeros(81, np.uint16)
+    w, h, b, n = (C.c_uint32() for _ in range(4))
+    rc = L.pgsvt_tiff_info(f.encode(), C.byref(w), C.byref(h), C.byref(b), C.byref(n))
+    if rc == 0:
+        rc = L.pgsvt_tiff_read_u16(f.encode(), 0, 1, out.ctypes.data)
+    assert rc == -1 and msg in L.pgsvt_last_error()
 
 
 def test_tiff_rejects_bad_files(tmp_path):
