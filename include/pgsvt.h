@@ -204,6 +204,17 @@ int pgsvt_mlp_train(const float* X, const float* t, uint32_t n,
  * counterpart (the reference is FP64 throughout). */
 int pgsvt_set_precision(int bits);
 
+/* Lanes-per-matrix mapping of the batched Jacobi SVD for FP32 calls started
+ * afterwards, process-wide: 0 = the primary mapping of each shape (default),
+ * 1 = the alternative mapping (e.g. 8 instead of 4 lanes per 16 x 15
+ * matrix), 2 = the lambda-grid path times both on the first batch of each
+ * shape and keeps the faster (other paths use 0).  The mappings sum the
+ * column dot products in different orders, so results differ in the last
+ * bits between them; with 2 they are reproducible within a process only.
+ * No reference counterpart (the reference calls LAPACK per patch,
+ * src/svt.hpp:111). */
+int pgsvt_set_svd_mapping(int mode);
+
 /* Device memory is cached per device between calls (buffers of one call
  * are reused by the next call of the same shape).  pgsvt_release_cache
  * waits for the current device and frees its cache.  No reference

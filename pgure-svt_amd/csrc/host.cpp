@@ -52,6 +52,11 @@ thread_local std::string g_err;
 // now on: 32 (default) or 64 bits, see pgsvt_set_precision
 std::atomic<int> g_precision{32};
 
+// SVD lanes-per-matrix mapping (pgsvt_set_svd_mapping): 0 the primary
+// mapping of each shape (default), 1 the alternative, 2 autotuned per shape
+// by the grid path
+std::atomic<int> g_svd_mapping{0};
+
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -783,7 +788,7 @@ struct Pipeline {
 
   void decompose(std::vector<WinMeta>& ms, uint32_t nvar_run) {
     upload_batch(ms);
-    decompose_kernels((uint32_t)ms.size(), nvar_run);
+    decompose_kernels((uint32_t)ms.size(), nvar_run, (!sh.fp64 && g_svd_mapping.load() == 1) ? 1 : 0);
   }
 
   // PGURE risk (src/pgure.hpp:136) from the GPU sums; noise as the driver
@@ -1008,10 +1013,15 @@ constexpr uint32_t kFeatureSamples = 256;
 // over a grid of lambdas per window, least risk chosen, the whole GPU work of
 // a batch captured in one graph.
 
-// Autotuned lanes-per-matrix mapping of the SVD per (patch, window) shape:
-// the first batch of a shape times both mappings, later ones reuse the
-// faster.  Used by the grid mode only (the reference path keeps the fixed
-// mapping, whose results its parity tests pin).
+// Lanes-per-matrix mapping of the SVD per (patch, window) shape.  The two
+// mappings sum the column dot products in different orders, so their
+// results differ in the last bits; a mapping that depends on timing would
+// make a call's lambdas and frames depend on the process.  The default is
+// therefore the fixed primary mapping of every shape; pgsvt_set_svd_mapping
+// forces the alternative (1) or the autotuner (2): the first batch of a
+// shape in the process times both mappings on its own matrices and the
+// faster is kept for the shape (results then reproducible within the
+// process only).
 struct Autotune {
   std::mutex mu;
   std::map<std::pair<uint32_t, uint32_t>, int> svd;
@@ -1234,7 +1244,10 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
       hipchk(hipMemcpyAsync(wl_g.p, wlg.data(), wlg.size() * 4, hipMemcpyHostToDevice, P.s), "H2D grid list");
       hipchk(hipMemcpyAsync(P.wl_d.p, wlf.data(), wlf.size() * 4, hipMemcpyHostToDevice, P.s), "H2D list");
       hipchk(hipStreamSynchronize(P.s), "grid upload sync");
-      if (!gexec) {  // first batch of the call: the SVD mapping (autotuned once per shape)
+      if (!gexec) {  // first batch of the call: the SVD mapping
+        svd_mapping = P.sh.fp64 ? 0 : std::min(g_svd_mapping.load(), 1);
+      }
+      if (!gexec && g_svd_mapping.load() == 2) {  // autotuned once per shape
         auto& at = Autotune::get();
         const auto key = std::make_pair(P.sh.bs, P.sh.nw);
         bool known;
@@ -1549,6 +1562,12 @@ const char* pgsvt_version(void) { return "pgsvt 0.1.0 (gfx950)"; }
 int pgsvt_set_precision(int bits) {
   if (bits != 32 && bits != 64) return fail(PGSVT_E_ARG, "precision must be 32 or 64 bits");
   g_precision.store(bits);
+  return 0;
+}
+
+int pgsvt_set_svd_mapping(int mode) {
+  if (mode < 0 || mode > 2) return fail(PGSVT_E_ARG, "SVD mapping must be 0, 1 or 2");
+  g_svd_mapping.store(mode);
   return 0;
 }
 

@@ -96,3 +96,8 @@ def delta2_signs(seed, n):
 
 def rel_l2(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def set_svd_mapping(mode):
+    """pgsvt_set_svd_mapping: 0 primary, 1 alternative, 2 autotuned (grid)."""
+    P.check(P.load().pgsvt_set_svd_mapping(mode))

@@ -40,7 +40,7 @@ def test_delta2_signs(gpu):
 
 
 @pytest.mark.parametrize("t", [0, 3, 7, 12, 18, 21, 23])
-@pytest.mark.parametrize("mf", [-1, 2])
+@pytest.mark.parametrize("mf", [-1, 2, 5])  # 5: the API default median radius
 def test_trajectories_bit_exact(gpu, seq64, t, mf):
     log("trajectories t=%d motion_filter=%d" % (t, mf))
     g = G.window_trajectories(seq64, t, motion_filter=mf)
@@ -122,16 +122,32 @@ def test_denoise_pgure_motion_off(gpu):
     assert lam_rel.max() < 1e-2
 
 
-def test_denoise_pgure_motion_on(gpu, seq64):
-    log("denoise PGURE motion on")
-    Yg, eg = G.denoise(seq64)
-    Yo, eo = O.denoise(seq64)
+@pytest.mark.parametrize("mf", [-1, 5])
+def test_denoise_pgure_motion_on(gpu, seq64, mf):
+    log("denoise PGURE motion on, motion_filter=%d" % mf)
+    Yg, eg = G.denoise(seq64, motion_filter=mf)
+    Yo, eo = O.denoise(seq64, motion_filter=mf)
     assert np.array_equal(eg[:, 1:], eo[:, 1:])
     lam_rel = np.abs(eg[:, 0] - eo[:, 0]) / np.abs(eo[:, 0])
     log("  lambda rel diff max =", lam_rel.max(), " frames rel L2 =", G.rel_l2(Yg, Yo))
     same = lam_rel == 0
     log("  windows with identical lambda: %d / %d" % (same.sum(), len(same)))
     assert lam_rel.max() < 1e-2
+    # frames at the GPU's lambda: first, middle and last window, with their
+    # trajectories bit-exact
+    nf = seq64.shape[2]
+    for t in (0, nf // 2, nf - 1):
+        assert np.array_equal(G.window_trajectories(seq64, t, motion_filter=mf),
+                              O.window_trajectories(seq64, t, motion_filter=mf))
+        g = G.window_reconstruct(seq64, t, eg[t, 0], motion_filter=mf)
+        o = O.window_reconstruct(seq64, t, eg[t, 0], motion_filter=mf)
+        e = G.rel_l2(g, o)
+        log("  t=%d window rel L2 at the GPU lambda = %.3g" % (t, e))
+        assert e < FRAME_TOL
+        # and the output frame itself is that window's reference frame
+        ref = t if t < 7 else (t - (nf - 15) if t >= nf - 7 else 7)
+        mx = seq64[:, :, t - ref:t - ref + 15].max()
+        assert G.rel_l2(Yg[:, :, t], o[:, :, ref] * mx) < FRAME_TOL
 
 
 # Every other instantiated (patch, window) family, at a size the oracle
@@ -149,6 +165,7 @@ SHAPES = [
     (8, 2, 7, True),
     (8, 2, 15, True),
     (8, 1, 31, True),
+    (6, 1, 31, True),
 ]
 
 
@@ -158,9 +175,21 @@ def seq64x40():
     return noisy
 
 
+@pytest.fixture
+def svd_mapping():
+    """Select an SVD lanes-per-matrix mapping for one test, restore the
+    default afterwards."""
+    yield G.set_svd_mapping
+    G.set_svd_mapping(0)
+
+
+@pytest.mark.parametrize("mapping", [0, 1])
 @pytest.mark.parametrize("bs,step,T,expw", SHAPES)
-def test_shapes_fixed_lambda_and_risk(gpu, seq64x40, bs, step, T, expw):
-    log("shape", bs, step, T, expw)
+def test_shapes_fixed_lambda_and_risk(gpu, seq64x40, svd_mapping, bs, step, T, expw, mapping):
+    # both lanes-per-matrix mappings of the SVD of each shape (the primary
+    # one is the default, the alternative one a choice of the autotuner)
+    log("shape", bs, step, T, expw, "mapping", mapping)
+    svd_mapping(mapping)
     kw = dict(patch_size=bs, patch_overlap=step, trajectory_length=T, exponential_weighting=expw)
     for t in (0, 20, 39):  # ARPS trajectories bit-exact at this block size
         tg = G.window_trajectories(seq64x40, t, **kw)
