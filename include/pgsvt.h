@@ -92,6 +92,24 @@ int pgsvt_denoise_device(const void* X_dev, int dtype, uint32_t n_rows,
                          const pgsvt_params* p, double* Y_dev,
                          double* estimates);
 
+/* pgsvt_denoise_device that allocates no device memory.  No reference
+ * counterpart: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Windows
+ * are processed one at a time in a statically sized device array of 64 MiB
+ * that each device the library runs on holds until the process ends; a
+ * shape whose window needs more returns PGSVT_E_ARG with the size in the
+ * message (64 x 64 frames fit with PGURE, 128 x 128 with a fixed lambda at
+ * patch 4 and 15 frames).  Work goes to `stream` (a hipStream_t; NULL = the
+ * null stream) and the call returns after Y is complete.  Calls of one
+ * process are serialised (they share the array, which keeps the last
+ * call's contents); calls that overlap on one device from several
+ * processes each use their own copy.  FP32 only (pgsvt_set_precision(32)).
+ * Results equal pgsvt_denoise_device's bit for bit. */
+int pgsvt_denoise_device_noalloc(const void* X_dev, int dtype, uint32_t n_rows,
+                                 uint32_t n_cols, uint32_t n_frames,
+                                 const pgsvt_params* p, double* Y_dev,
+                                 double* estimates, void* stream);
+
 /* pgsvt_denoise restricted to output frames t_begin .. t_end-1 (the windows
  * of those frames read the input frames they need from the whole cube).  Y
  * holds the t_end - t_begin output frames in order, estimates is a

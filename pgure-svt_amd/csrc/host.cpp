@@ -1513,6 +1513,260 @@ int guarded(F f) {
   }
 }
 
+// --------------------------------------------- entry point without allocation --
+// pgsvt_denoise_device_noalloc runs the reference driver window by window
+// with every device buffer carved from one statically sized array of device
+// memory, so that callers that cannot allocate (code under HIP graph capture,
+// serving loops) can use it.  Each device the library runs on holds its own
+// copy of the array from the first such call until the process ends; calls
+// are serialised within the process (they share the array), and the array
+// keeps the last call's contents.  The kernels, their launch shapes and the
+// order of every sum are those of pgsvt_denoise_device, so the results are
+// the same bit for bit.
+constexpr size_t kNoAllocScratch = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_at;  // device -> address of its copy of `scratch`
+
+// address of the current device's copy (g_scratch_mu held): the host-side
+// name `scratch` is not a device address
+unsigned char* scratch_address() {
+  int dev = 0;
+  hipchk(hipGetDevice(&dev), "hipGetDevice");
+  auto it = g_scratch_at.find(dev);
+  if (it != g_scratch_at.end()) return it->second;
+  void* p = nullptr;
+  hipchk(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  g_scratch_at[dev] = (unsigned char*)p;
+  return (unsigned char*)p;
+}
+
+// 256-byte aligned pieces of a region; with base == nullptr it only counts
+struct Carve {
+  unsigned char* base = nullptr;
+  size_t used = 0;
+  template <class T>
+  T* take(size_t count) {
+    used = (used + 255) & ~(size_t)255;
+    T* p = base ? reinterpret_cast<T*>(base + used) : nullptr;
+    used += count * sizeof(T);
+    return p;
+  }
+};
+
+// PGURE risk of src/pgure.hpp:136 from the six GPU sums (see Pipeline::risk)
+double window_risk_of(const double* S, double alpha, double mu, double sigma, const Shape& sh, double eps2) {
+  const double mup = sigma, sigp = mu;  // constructor receives (alpha, sigma, mu)
+  const double sigmasq = sigp * sigp;
+  const double OoN = 1.0 / (sh.nr * sh.nc * sh.nw);
+  const double S1 = S[0] - 2.0 * S[3] + S[2];
+  return OoN * (S1 - (alpha + mup) * S[1] + (2 / 1e-4 * 0.0) - (2 * sigmasq * alpha / (eps2 * eps2) * S[4]) +
+                (2 * mup * S[5]) + mup) -
+         sigmasq;
+}
+
+int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, const pgsvt_params* pp, double* Y,
+                double* est, hipStream_t s) {
+  if (int rc = validate(X, dtype, nr, nc, nf, pp)) return rc;
+  if (!Y || !est) return fail(PGSVT_E_ARG, "null output pointer");
+  if (g_precision.load() != 32) return fail(PGSVT_E_ARG, "the entry point without allocation runs in FP32 only");
+  if (int rc = check_device()) return rc;
+  const pgsvt_params p = *pp;
+  const Geometry g = geometry(p);
+  const bool optimize = p.optimize_pgure != 0, median = p.motion_filter > 0;
+  const uint32_t nvar = optimize ? 3 : 1;
+  Shape sh;
+  sh.nr = nr;
+  sh.nc = nc;
+  sh.fs = nr * nc;
+  sh.nw = g.Nw;
+  sh.vol = g.Nw * sh.fs;
+  sh.bs = p.patch_size;
+  sh.m = sh.bs * sh.bs;
+  sh.n1 = nr - sh.bs + 1;
+  sh.p_all = sh.n1 * sh.n1;
+  const std::vector<uint32_t> sel = patch_indices(nr, sh.bs, p.patch_overlap);
+  sh.p_sel = (uint32_t)sel.size();
+  const uint32_t sr = (g.Nw + 3) & ~3u;
+  sh.fstride = (g.Nw * sr + g.Nw * sh.m + g.Nw + 3) & ~3u;
+  const size_t npos = (size_t)sh.n1 * sh.n1, vol = sh.vol;
+  const uint32_t max_jobs = 2 * g.Nw, eblk = eval_blocks(sh, 3), gblk = pgsvt::gather_blocks(sh, 0);
+  const double vP = 0.5 + 0.5 * 1. / std::sqrt(1. * 1. + 4), vQ = 1 - vP;
+  const double d2neg = -1 * std::sqrt(vQ / vP), d2pos = std::sqrt(vP / vQ), eps2 = 1e-2;
+  // the layout: per call, then the window being processed
+  struct Layout {
+    uint8_t* d2;
+    uint32_t *sel, *wl, *stats;
+    double *fmax, *sums, *Xw, *Zw, *u64, *w64, *invw, *partial, *partial_b;
+    WinMeta* meta;
+    int16_t* jobs;
+    float *uvar, *dw, *B0, *factors, *divq;
+    int32_t *pos, *cnt, *off, *list, *inv;
+  } L{};
+  auto carve = [&](Carve& c) {
+    L.d2 = optimize ? c.take<uint8_t>(vol) : nullptr;
+    L.sel = c.take<uint32_t>(sh.p_sel);
+    L.wl = c.take<uint32_t>(1);
+    L.stats = c.take<uint32_t>(1);
+    L.fmax = c.take<double>(2 * g.Nw);
+    L.sums = c.take<double>(6);
+    L.meta = c.take<WinMeta>(1);
+    L.jobs = c.take<int16_t>((size_t)max_jobs * 4);
+    L.Xw = c.take<double>(vol);
+    L.Zw = median ? c.take<double>(vol) : nullptr;
+    L.u64 = c.take<double>(vol);
+    L.w64 = median ? c.take<double>(vol) : nullptr;
+    L.uvar = c.take<float>((size_t)nvar * vol);
+    L.pos = c.take<int32_t>((size_t)g.Nw * sh.p_all);
+    L.cnt = c.take<int32_t>((size_t)g.Nw * npos);
+    L.off = c.take<int32_t>((size_t)g.Nw * (npos + 1));
+    L.list = c.take<int32_t>((size_t)g.Nw * sh.p_sel);
+    L.inv = c.take<int32_t>((size_t)g.Nw * sh.p_sel);
+    L.invw = c.take<double>(vol);
+    L.dw = optimize ? c.take<float>(vol) : nullptr;
+    L.B0 = c.take<float>((size_t)g.Nw * sh.p_sel * sh.m);
+    L.factors = c.take<float>((size_t)nvar * sh.p_sel * sh.fstride);
+    L.divq = optimize ? c.take<float>((size_t)nvar * sh.p_sel * 2 * sr) : nullptr;
+    L.partial = c.take<double>(eblk);
+    L.partial_b = c.take<double>((size_t)gblk * 5);
+  };
+  Carve count;
+  carve(count);
+  if (count.used > kNoAllocScratch)
+    return fail(PGSVT_E_ARG, "this shape needs " + std::to_string(count.used) + " bytes of device scratch per window; "
+                             "the entry point without allocation has " + std::to_string(kNoAllocScratch));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  Carve at;
+  at.base = scratch_address();
+  carve(at);
+
+  const size_t dsz = dtype_size(dtype);
+  const uint64_t seed = seed_of(p);
+  const double lambda0 = (p.lambda1 >= 0.0) ? p.lambda1 : -1.0;
+  const double alpha0 = (p.noise_alpha >= 0.0) ? p.noise_alpha : -1.0;
+  const double mu0 = (p.noise_mu >= 0.0) ? p.noise_mu : -1.0;
+  const double sigma0 = (p.noise_sigma >= 0.0) ? p.noise_sigma : -1.0;
+  const uint32_t zero = 0;
+  hipchk(hipMemcpyAsync(L.sel, sel.data(), sel.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D sel");
+  hipchk(hipMemcpyAsync(L.wl, &zero, sizeof zero, hipMemcpyHostToDevice, s), "H2D list");
+  if (optimize) hipchk(launch_delta2((int64_t)seed, vol, L.d2, s), "delta2");
+  hipchk(hipStreamSynchronize(s), "setup sync");  // the host vectors above are local
+  std::vector<double> fmax(2 * g.Nw), sums(6), hw;
+  int status = 0;
+  std::string warn;
+  for (uint32_t t = 0; t < nf; ++t) {
+    WinMeta m{};
+    m.t = t;
+    m.out = t;
+    uint32_t s0 = 0;
+    window_of(t, nf, g, s0, m.ref);
+    m.s0 = 0;  // the window's frames are staged at the start of Xw
+    hipchk(launch_to_double((const char*)X + (size_t)s0 * sh.fs * dsz, dtype, L.Xw, vol, s), "to_double");
+    hipchk(launch_frame_max(L.Xw, sh.fs, g.Nw, L.fmax, s), "frame max");
+    if (median) {
+      hipchk(launch_median(L.Xw, L.Zw, nr, nc, g.Nw, (int)p.motion_filter, s), "median");
+      hipchk(launch_frame_max(L.Zw, sh.fs, g.Nw, L.fmax + g.Nw, s), "frame max Z");
+    }
+    hipchk(hipMemcpyAsync(fmax.data(), L.fmax, fmax.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H max");
+    hipchk(hipStreamSynchronize(s), "max sync");
+    m.uMax = *std::max_element(fmax.begin(), fmax.begin() + g.Nw);
+    m.wMax = median ? *std::max_element(fmax.begin() + g.Nw, fmax.end()) : m.uMax;
+    m.lambda = 0;
+    std::vector<int16_t> jobs((size_t)max_jobs * 4, (int16_t)-1);
+    if (p.motion_estimation) {
+      std::vector<int16_t> j;
+      if (!arps_jobs(t, nf, g, j)) throw HipError{hipErrorInvalidValue, "unsupported ARPS schedule"};
+      std::copy(j.begin(), j.end(), jobs.begin());
+    }
+    hipchk(hipMemcpyAsync(L.meta, &m, sizeof m, hipMemcpyHostToDevice, s), "H2D meta");
+    hipchk(hipMemcpyAsync(L.jobs, jobs.data(), jobs.size() * sizeof(int16_t), hipMemcpyHostToDevice, s), "H2D jobs");
+    // decomposition (Pipeline::decompose_kernels for one window)
+    hipchk(launch_prepare(L.Xw, L.Zw, L.meta, 1, L.d2, sh, nvar, eps2, d2neg, d2pos, L.u64, L.w64, L.uvar, s),
+           "prepare");
+    hipchk(launch_fill32(L.pos, (size_t)g.Nw * sh.p_all, 0, s), "clear pos");
+    hipchk(launch_fill_ref(L.meta, 1, sh, L.pos, s), "fill_ref");
+    if (p.motion_estimation)
+      hipchk(launch_arps(median ? L.w64 : L.u64, L.jobs, max_jobs, 1, sh, (int)p.motion_window, L.pos, s), "arps");
+    hipchk(launch_buckets(L.pos, L.sel, 1, sh, L.cnt, L.off, L.list, L.inv, s), "buckets");
+    hipchk(launch_weights(L.off, L.d2, 1, sh, d2neg, d2pos, L.invw, L.dw, s), "weights");
+    hipchk(launch_fill32(L.stats, 1, 0, s), "clear stats");
+    hipchk(launch_svd(L.uvar, L.pos, L.sel, 1, nvar, sh, L.factors, 30, 1e-6, L.stats, s,
+                      g_svd_mapping.load() == 1 ? 1 : 0),
+           "svd");
+    if (optimize) hipchk(launch_divq(L.factors, L.pos, L.sel, L.dw, 1, nvar, sh, L.divq, s), "divergence factors");
+    double alpha = alpha0, mu = mu0, sigma = sigma0, lambda = lambda0;
+    if (optimize) {
+      // noise estimate and start point from the normalised window on the host
+      hw.resize(vol);
+      hipchk(hipMemcpyAsync(hw.data(), L.Xw, vol * sizeof(double), hipMemcpyDeviceToHost, s), "D2H window");
+      hipchk(hipStreamSynchronize(s), "window sync");
+      for (double& v : hw) v /= m.uMax;
+      estimate_noise(hw.data(), nr, g.Nw, p.noise_method, alpha, mu, sigma);
+      double a1 = 0, a2 = 0;  // accu(u) in Armadillo's even / odd order
+      size_t i = 0;
+      for (; i + 1 < vol; i += 2) {
+        a1 += hw[i];
+        a2 += hw[i + 1];
+      }
+      if (i < vol) a1 += hw[i];
+      const double start = std::max(0.0, (lambda0 >= 0.0) ? lambda0 : (a1 + a2) * (1.0 / (nc * nr * g.Nt)));
+      Sbplx1D opt(start, 0.0, std::max(100.0, start), std::sqrt(start), p.tol, 1e-12, (int)p.max_iter);
+      std::vector<std::pair<double, double>> seen;  // repeated lambdas answered as in the batched driver
+      int rc;
+      while ((rc = opt.run()) == Sbplx1D::kNeed) {
+        const double l = opt.pending();
+        double r = 0;
+        bool known = false;
+        for (const auto& e : seen)
+          if (e.first == l) {
+            r = e.second;
+            known = true;
+            break;
+          }
+        if (!known) {
+          m.lambda = l;
+          hipchk(hipMemcpyAsync(L.meta, &m, sizeof m, hipMemcpyHostToDevice, s), "H2D meta");
+          hipchk(launch_eval(L.factors, L.meta, L.wl, 1, nvar, 3, sh, p.exponential_weighting, L.divq, L.inv, L.B0,
+                             L.partial, s),
+                 "eval");
+          hipchk(launch_gather(L.B0, L.off, L.invw, L.u64, L.meta, L.wl, 1, sh, 0, nullptr, L.partial_b, s), "gather");
+          hipchk(launch_risk_reduce(L.partial, eblk, L.partial_b, gblk, 1, L.sums, s), "reduce");
+          hipchk(hipMemcpyAsync(sums.data(), L.sums, 6 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H sums");
+          hipchk(hipStreamSynchronize(s), "risk sync");
+          r = window_risk_of(sums.data(), alpha, mu, sigma, sh, eps2);
+          seen.emplace_back(l, r);
+        }
+        opt.supply(r);
+      }
+      lambda = opt.last_x();
+      if (rc < 0) {
+        status = PGSVT_E_NUMERIC;
+        warn = "lambda optimisation failed (NLopt-equivalent error code " + std::to_string(rc) + ")";
+      } else if (rc == Sbplx1D::kMaxEval) {
+        std::fprintf(stderr,
+                     "WARNING: optimization terminated after max_eval (%u) was reached.\n"
+                     "Consider increasing the max_eval or increasing the convergence tolerance (tol=%g).\n",
+                     p.max_iter, p.tol);
+      }
+    }
+    // the output frame at the window's lambda (Pipeline::finish)
+    m.lambda = lambda;
+    hipchk(hipMemcpyAsync(L.meta, &m, sizeof m, hipMemcpyHostToDevice, s), "H2D meta");
+    hipchk(launch_eval(L.factors, L.meta, L.wl, 1, nvar, 1, sh, p.exponential_weighting, nullptr, L.inv, L.B0, nullptr,
+                       s),
+           "eval final");
+    hipchk(launch_gather(L.B0, L.off, L.invw, L.u64, L.meta, L.wl, 1, sh, 1, Y, nullptr, s), "gather final");
+    hipchk(hipStreamSynchronize(s), "window sync");  // m and jobs are reused by the next window
+    est[t + (size_t)nf * 0] = lambda;
+    est[t + (size_t)nf * 1] = alpha;
+    est[t + (size_t)nf * 2] = mu;
+    est[t + (size_t)nf * 3] = sigma;
+  }
+  if (status) return fail(status, warn);
+  return 0;
+}
+
 // one window through the GPU stages; used by the window-level entry points
 int one_window(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, const pgsvt_params* pp, uint32_t t,
                bool want_risk, Pipeline& P, std::vector<WinMeta>& ms) {
@@ -1591,6 +1845,13 @@ int pgsvt_denoise_device(const void* X_dev, int dtype, uint32_t n_rows, uint32_t
                          const pgsvt_params* p, double* Y_dev, double* estimates) {
   if (!Y_dev || !estimates) return fail(PGSVT_E_ARG, "null output pointer");
   return guarded([&] { return run_denoise(X_dev, dtype, n_rows, n_cols, n_frames, p, Y_dev, estimates, true); });
+}
+
+int pgsvt_denoise_device_noalloc(const void* X_dev, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
+                                 const pgsvt_params* p, double* Y_dev, double* estimates, void* stream) {
+  return guarded([&] {
+    return run_noalloc(X_dev, dtype, n_rows, n_cols, n_frames, p, Y_dev, estimates, (hipStream_t)stream);
+  });
 }
 
 int pgsvt_denoise_range(const void* X, int dtype, uint32_t n_rows, uint32_t n_cols, uint32_t n_frames,
