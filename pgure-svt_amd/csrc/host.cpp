@@ -67,7 +67,40 @@ struct HipError {
   const char* what;
 };
 
+// Fault injection for tests of the error paths: PGSVT_FAULT="alloc:N" makes
+// the N-th device buffer request of a call fail as out of memory (whether
+// the cache or hipMalloc would serve it), "hip:N" the
+// N-th checked HIP call (launch, copy, synchronisation) fail as a launch
+// failure, after the real call succeeded.  Read at the start of every
+// denoise call (fault_arm) and counted per calling thread; the failure
+// unwinds like a real one.
+struct FaultPlan {
+  int kind = 0;  // 0 none, 1 allocation, 2 HIP call
+  long left = 0;
+};
+thread_local FaultPlan g_fault;
+
+void fault_arm() {
+  g_fault = FaultPlan{};
+  const char* e = std::getenv("PGSVT_FAULT");
+  if (!e) return;
+  const std::string v(e);
+  const size_t c = v.find(':');
+  if (c == std::string::npos) return;
+  const std::string kind = v.substr(0, c);
+  g_fault.left = std::atol(v.c_str() + c + 1);
+  g_fault.kind = kind == "alloc" ? 1 : kind == "hip" ? 2 : 0;
+}
+
+bool fault_hit(int kind) {
+  if (g_fault.kind != kind || g_fault.left <= 0) return false;
+  if (--g_fault.left > 0) return false;
+  g_fault.kind = 0;
+  return true;
+}
+
 inline void hipchk(hipError_t e, const char* what) {
+  if (e == hipSuccess && fault_hit(2)) throw HipError{hipErrorLaunchFailure, "injected failure after"};
   if (e != hipSuccess) throw HipError{e, what};
 }
 
@@ -190,6 +223,7 @@ class DeviceCache {
     return *c;
   }
   void* take(size_t bytes) {
+    if (fault_hit(1)) throw HipError{hipErrorOutOfMemory, "hipMalloc (injected failure)"};
     int dev = 0;
     hipchk(hipGetDevice(&dev), "hipGetDevice");
     {
@@ -297,6 +331,16 @@ struct HBuf {
     n = count;
   }
 };
+
+// one lock per device for the batch sizing of concurrent calls
+std::mutex& batch_lock(int dev) {
+  static std::mutex mu;
+  static std::map<int, std::unique_ptr<std::mutex>>* locks = new std::map<int, std::unique_ptr<std::mutex>>;
+  std::lock_guard<std::mutex> g(mu);
+  auto& m = (*locks)[dev];
+  if (!m) m.reset(new std::mutex);
+  return *m;
+}
 
 struct Geometry {
   uint32_t Nt, fw, Nw;
@@ -433,16 +477,19 @@ int validate(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
   if (dtype < 0 || dtype > 3) return fail(PGSVT_E_ARG, "invalid dtype");
   if (nr != nc) return fail(PGSVT_E_ARG, "frames must be square: the reference patch tables assume n_rows == n_cols");
   if (nr > 65535) return fail(PGSVT_E_ARG, "frame side above 65535");
-  if (p->patch_size < 2 || p->patch_size > 8 || p->patch_size > nr)
-    return fail(PGSVT_E_ARG, "patch_size must be in [2, 8] and fit the frame");
+  if (p->patch_size < 2 || (p->patch_size > 8 && p->patch_size != 16) || p->patch_size > nr)
+    return fail(PGSVT_E_ARG, "patch_size must be 2 to 8 or 16 and fit the frame");
   if (p->patch_overlap < 1 || p->patch_overlap > p->patch_size) return fail(PGSVT_E_ARG, "invalid patch_overlap");
   if (p->trajectory_length < 1 || p->trajectory_length % 2 == 0)
     return fail(PGSVT_E_ARG, "trajectory_length must be a positive odd integer");
   const Geometry g = geometry(*p);
   if (g.Nt < 1 || nf < g.Nw) return fail(PGSVT_E_ARG, "fewer frames than the temporal window");
-  if (p->motion_estimation && (p->motion_window < 3 || p->motion_window % 2 == 0 || p->motion_window > 15))
-    return fail(PGSVT_E_ARG, "motion_window must be odd, >= 3 and <= 15");
-  if (!svd_shape_supported(p->patch_size, g.Nw)) return fail(PGSVT_E_ARG, "unsupported (patch_size, window) shape");
+  if (p->motion_estimation && (p->motion_window < 3 || p->motion_window % 2 == 0 || p->motion_window > 31))
+    return fail(PGSVT_E_ARG, "motion_window must be odd, >= 3 and <= 31");
+  if (!svd_shape_supported(p->patch_size, g.Nw, g_precision.load() == 64))
+    return fail(PGSVT_E_ARG, "unsupported shape: patch " + std::to_string(p->patch_size) + " with a window of " +
+                                 std::to_string(g.Nw) + " frames (patches 2-8 take windows up to 31 frames, patch 16 "
+                                 "up to 15; the FP64 path only the instantiated table)");
   if (p->optimize_pgure && p->noise_method < 1) return fail(PGSVT_E_ARG, "invalid noise_method");
   return 0;
 }
@@ -555,8 +602,10 @@ struct Pipeline {
   DBuf<char> divq;  // per (patch, variant) singular values and divergence factors (PGURE)
   // SVD input variants, factors and U reconstructions: FP32, or FP64 for
   // the precision study (sh.fp64); sized in bytes
-  DBuf<char> uvar, factors, B0;
+  DBuf<char> uvar, factors, B0;  // B0: reconstructed rows (patches above 4 x 4) or the factors d_r (dr)
   DBuf<int32_t> pos, cnt, off, list, inv;
+  DBuf<char> nkb;  // round records per (slot, frame, bucket place) of the fused evaluation (patches up to 4 x 4)
+  DBuf<char> w0b, v0b;  // static rows of the fused evaluation: w_0 and v_0[k] per bucket place
   DBuf<int16_t> jobs_d;
   uint32_t max_jobs = 0, gather_blocks = 0;
   DBuf<uint32_t> stats;
@@ -593,8 +642,7 @@ struct Pipeline {
     sh.p_all = sh.n1 * sh.n1;
     sel = patch_indices(nr, sh.bs, p.patch_overlap);
     sh.p_sel = (uint32_t)sel.size();
-    const uint32_t sr = (g.Nw + 3) & ~3u;  // padded V^T row
-    sh.fstride = (g.Nw * sr + g.Nw * sh.m + g.Nw + 3) & ~3u;
+    sh.fstride = factor_stride(g.Nw, sh.m);
     sh.fp64 = g_precision.load() == 64;
     const double vP = 0.5 + 0.5 * 1. / std::sqrt(1. * 1. + 4);
     const double vQ = 1 - vP;
@@ -669,26 +717,33 @@ struct Pipeline {
     b += 8ull * sh.nw * ((size_t)sh.n1 * sh.n1 + 1);     // bucket counts + offsets
     b += 8ull * sh.nw * sh.p_sel;                        // bucket lists + inverse
     b += 8ull * sh.vol * (optimize ? 2 : 1);             // 1/W, delta2/W
-    b += es * sh.nw * (size_t)sh.p_sel * sh.m;           // B0
+    b += es * sh.nw * (size_t)sh.p_sel * (fused_eval(sh) ? 1 : sh.m);  // B0 (or dr)
+    if (fused_eval(sh)) {
+      b += round_record_bytes(sh);                           // round records
+      b += es * sh.nw * (size_t)sh.p_sel * (sh.m + 1);       // static rows
+    }
     b += es * nvar * (size_t)sh.p_sel * sh.fstride;      // factors
     if (optimize) b += es * nvar * (size_t)sh.p_sel * 2 * ((sh.nw + 3) & ~3u);  // divq
     return b;
   }
 
   void alloc_batch(uint32_t want) {
-    size_t freeb = 0, total = 0;
-    hipchk(hipMemGetInfo(&freeb, &total), "hipMemGetInfo");
     int dev = 0;
     hipchk(hipGetDevice(&dev), "hipGetDevice");
+    // Calls of this process size and allocate their batches one at a time
+    // per device, so that a call sizing its batch sees the memory the calls
+    // before it took (pgsvt_denoise_batch runs several calls at once).
+    std::lock_guard<std::mutex> reserve(batch_lock(dev));
+    size_t freeb = 0, total = 0;
+    hipchk(hipMemGetInfo(&freeb, &total), "hipMemGetInfo");
     freeb += DeviceCache::get().held(dev);  // cached blocks are ours to reuse
     const size_t budget = std::min<size_t>((size_t)(freeb * 0.6), 64ull << 30);
     const size_t per = bytes_per_window();
     NB = (uint32_t)std::max<size_t>(1, std::min<size_t>(want, budget / per));
     if (const char* e = std::getenv("PGSVT_BATCH")) NB = std::max(1u, std::min<uint32_t>(NB, (uint32_t)atoi(e)));
-    // The budget is an estimate: calls running concurrently on the device
-    // size their batches from the same free memory.  A batch that does not
-    // fit is halved (buffers of the failed attempt go back to the cache,
-    // which a failing hipMalloc empties).
+    // Other processes can still take memory between the estimate and the
+    // allocation: a batch that does not fit is halved (buffers of the failed
+    // attempt go back to the cache, which a failing hipMalloc empties).
     for (;;) {
       try {
         alloc_batch_buffers();
@@ -720,7 +775,12 @@ struct Pipeline {
     inv.alloc((size_t)NB * sh.nw * sh.p_sel);
     invw.alloc((size_t)NB * sh.vol);
     if (optimize) dw.alloc((size_t)NB * sh.vol * (sh.fp64 ? sizeof(double) : sizeof(float)));
-    B0.alloc((size_t)NB * sh.nw * sh.p_sel * sh.m * es);
+    B0.alloc((size_t)NB * sh.nw * sh.p_sel * (fused_eval(sh) ? 1 : sh.m) * es);
+    if (fused_eval(sh)) {
+      nkb.alloc((size_t)NB * round_record_bytes(sh));
+      w0b.alloc((size_t)NB * sh.nw * sh.p_sel * sh.m * es);
+      v0b.alloc((size_t)NB * sh.nw * sh.p_sel * es);
+    }
     factors.alloc((size_t)NB * nvar * sh.p_sel * sh.fstride * es);
     if (optimize) divq.alloc((size_t)NB * nvar * sh.p_sel * 2 * ((sh.nw + 3) & ~3u) * es);
     gather_blocks = pgsvt::gather_blocks(sh, 0);
@@ -784,6 +844,35 @@ struct Pipeline {
         hipchk(launch_divq(factors.p, pos.p, sel_d.p, dw.p, nb, nvar_run, sh, divq.p, s), "divergence factors");
       });
     }
+    if (fused_eval(sh)) {
+      // units: bucket places
+      prof->run("static_rows", (double)nb * sh.nw * sh.p_sel, [&] {
+        hipchk(launch_static_rows(factors.p, list.p, nb, nvar, sh, w0b.p, v0b.p, s), "static rows");
+      });
+    }
+  }
+
+  // what the gather reads (see GatherSrc)
+  GatherSrc gather_src() const {
+    GatherSrc g;
+    g.B0 = B0.p;
+    g.factors = factors.p;
+    g.dr = B0.p;
+    g.rec = nkb.p;
+    g.W0 = w0b.p;
+    g.V0 = v0b.p;
+    g.list = list.p;
+    g.nvar_stored = nvar;
+    return g;
+  }
+
+  // one lambda evaluation of the slots listed in wl (device), their lambdas
+  // from meta or slot_lam
+  void evaluate(const uint32_t* wl, uint32_t na, uint32_t nvar_run, const double* slot_lam, double* part,
+                double* kept) {
+    hipchk(launch_evaluation(sh, factors.p, divq.p, meta_d.p, wl, na, nvar, nvar_run, p.exponential_weighting, inv.p,
+                             nkb.p, B0.p, part, slot_lam, kept, s),
+           "eval");
   }
 
   void decompose(std::vector<WinMeta>& ms, uint32_t nvar_run) {
@@ -818,18 +907,19 @@ struct Pipeline {
     // while profiling, the evaluation also counts the ranks it keeps (the
     // factor rows it reads)
     const bool count = g_prof_on && !prof->suspended;
-    prof->run("eval_reconstruct", (double)na * 3 * sh.p_sel, [&] {
-      hipchk(launch_eval(factors.p, meta_d.p, wl_d.p, na, nvar, 3, sh, p.exponential_weighting, divq.p, inv.p,
-                         B0.p, partial.p, s, nullptr, count ? partial_k.p : nullptr),
-             "eval");
-    });
-    // units: voxels
-    prof->run("gather_risk", (double)na * sh.vol, [&] {
-      hipchk(launch_gather(B0.p, off.p, invw.p, u64.p, meta_d.p, wl_d.p, na, sh, 0, nullptr, partial_b.p, s),
-             "gather");
-      hipchk(launch_risk_reduce(partial.p, eval_blocks(sh, 3), partial_b.p, gather_blocks, na, sums.p, s,
-                                count ? partial_k.p : nullptr, count ? kept_d.p : nullptr),
-             "reduce");
+    // units of the whole evaluation: voxels of the evaluated windows
+    prof->run("search_eval", (double)na * sh.vol, [&] {
+      prof->run("eval_reconstruct", (double)na * 3 * sh.p_sel,
+                [&] { evaluate(wl_d.p, na, 3, nullptr, partial.p, count ? partial_k.p : nullptr); });
+      // units: voxels
+      prof->run("gather_risk", (double)na * sh.vol, [&] {
+        hipchk(launch_gather(gather_src(), off.p, invw.p, u64.p, meta_d.p, wl_d.p, na, sh, 0, nullptr, partial_b.p,
+                             s),
+               "gather");
+        hipchk(launch_risk_reduce(partial.p, eval_blocks(sh, 3), partial_b.p, gather_blocks, na, sums.p, s,
+                                  count ? partial_k.p : nullptr, count ? kept_d.p : nullptr),
+               "reduce");
+      });
     });
     out.resize((size_t)na * 6);
     hipchk(hipMemcpyAsync(sums_h.p, sums.p, (size_t)na * 6 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H sums");
@@ -853,10 +943,8 @@ struct Pipeline {
     hipchk(hipMemcpyAsync(meta_d.p, ms.data(), nb * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
     hipchk(hipMemcpyAsync(wl_d.p, lst.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
     prof->run("final_frame", (double)nb * sh.p_sel, [&] {
-      hipchk(launch_eval(factors.p, meta_d.p, wl_d.p, nb, nvar, 1, sh, p.exponential_weighting, nullptr, inv.p,
-                         B0.p, nullptr, s),
-             "eval final");
-      hipchk(launch_gather(B0.p, off.p, invw.p, u64.p, meta_d.p, wl_d.p, nb, sh, 1, Yout, nullptr, s),
+      evaluate(wl_d.p, nb, 1, nullptr, nullptr, nullptr);
+      hipchk(launch_gather(gather_src(), off.p, invw.p, u64.p, meta_d.p, wl_d.p, nb, sh, 1, Yout, nullptr, s),
              "gather final");
     });
     hipchk(hipStreamSynchronize(s), "finish sync");
@@ -1043,6 +1131,7 @@ struct RunSpec {
 // mu, sigma) as a (te - tb) x 4 column-major matrix.
 int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, const pgsvt_params* pp, double* Y,
              double* est, bool on_device, const RunSpec& rs) {
+  fault_arm();
   if (int rc = validate(X, dtype, nr, nc, nf, pp)) return rc;
   if (!(rs.tb < rs.te && rs.te <= nf)) return fail(PGSVT_E_ARG, "window range must satisfy t_begin < t_end <= n_frames");
   if (!est || (rs.mode != 2 && !Y) || (rs.mode == 2 && !rs.features)) return fail(PGSVT_E_ARG, "null output pointer");
@@ -1297,11 +1386,9 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
                "grid lambdas");
         for (uint32_t c0 = 0; c0 < slots; c0 += C) {
           const uint32_t na = std::min(C, slots - c0);
-          hipchk(launch_eval(P.factors.p, P.meta_d.p, wl_g.p + c0, na, P.nvar, 3, P.sh, p.exponential_weighting,
-                             P.divq.p, P.inv.p, P.B0.p, P.partial.p, P.s, lam_g.p + c0),
-                 "grid eval");
-          hipchk(launch_gather(P.B0.p, P.off.p, P.invw.p, P.u64.p, P.meta_d.p, wl_g.p + c0, na, P.sh, 0, nullptr,
-                               P.partial_b.p, P.s),
+          P.evaluate(wl_g.p + c0, na, 3, lam_g.p + c0, P.partial.p, nullptr);
+          hipchk(launch_gather(P.gather_src(), P.off.p, P.invw.p, P.u64.p, P.meta_d.p, wl_g.p + c0, na, P.sh, 0,
+                               nullptr, P.partial_b.p, P.s),
                  "grid gather");
           hipchk(launch_risk_reduce(P.partial.p, eval_blocks(P.sh, 3), P.partial_b.p, P.gather_blocks, na,
                                     sums_g.p + (size_t)c0 * 6, P.s),
@@ -1309,11 +1396,9 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
         }
         hipchk(launch_grid_select(sums_g.p, noise_g.p, nb, G, OoN, P.eps2, lam_g.p, P.meta_d.p, risk_g.p, P.s),
                "grid select");
-        hipchk(launch_eval(P.factors.p, P.meta_d.p, P.wl_d.p, nb, P.nvar, 1, P.sh, p.exponential_weighting, nullptr,
-                           P.inv.p, P.B0.p, nullptr, P.s),
-               "eval final");
-        hipchk(launch_gather(P.B0.p, P.off.p, P.invw.p, P.u64.p, P.meta_d.p, P.wl_d.p, nb, P.sh, 1, P.Yout, nullptr,
-                             P.s),
+        P.evaluate(P.wl_d.p, nb, 1, nullptr, nullptr, nullptr);
+        hipchk(launch_gather(P.gather_src(), P.off.p, P.invw.p, P.u64.p, P.meta_d.p, P.wl_d.p, nb, P.sh, 1, P.Yout,
+                             nullptr, P.s),
                "gather final");
         const hipError_t ce = hipStreamEndCapture(P.s, &graph);
         P.prof->suspended = false;
@@ -1589,7 +1674,7 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
   const std::vector<uint32_t> sel = patch_indices(nr, sh.bs, p.patch_overlap);
   sh.p_sel = (uint32_t)sel.size();
   const uint32_t sr = (g.Nw + 3) & ~3u;
-  sh.fstride = (g.Nw * sr + g.Nw * sh.m + g.Nw + 3) & ~3u;
+  sh.fstride = factor_stride(g.Nw, sh.m);
   const size_t npos = (size_t)sh.n1 * sh.n1, vol = sh.vol;
   const uint32_t max_jobs = 2 * g.Nw, eblk = eval_blocks(sh, 3), gblk = pgsvt::gather_blocks(sh, 0);
   const double vP = 0.5 + 0.5 * 1. / std::sqrt(1. * 1. + 4), vQ = 1 - vP;
@@ -1601,8 +1686,9 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     double *fmax, *sums, *Xw, *Zw, *u64, *w64, *invw, *partial, *partial_b;
     WinMeta* meta;
     int16_t* jobs;
-    float *uvar, *dw, *B0, *factors, *divq;
+    float *uvar, *dw, *B0, *factors, *divq, *w0, *v0;
     int32_t *pos, *cnt, *off, *list, *inv;
+    char* rec;
   } L{};
   auto carve = [&](Carve& c) {
     L.d2 = optimize ? c.take<uint8_t>(vol) : nullptr;
@@ -1625,7 +1711,10 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     L.inv = c.take<int32_t>((size_t)g.Nw * sh.p_sel);
     L.invw = c.take<double>(vol);
     L.dw = optimize ? c.take<float>(vol) : nullptr;
-    L.B0 = c.take<float>((size_t)g.Nw * sh.p_sel * sh.m);
+    L.B0 = c.take<float>((size_t)g.Nw * sh.p_sel * (fused_eval(sh) ? 1 : sh.m));
+    L.rec = fused_eval(sh) ? c.take<char>(round_record_bytes(sh)) : nullptr;
+    L.w0 = fused_eval(sh) ? c.take<float>((size_t)g.Nw * sh.p_sel * sh.m) : nullptr;
+    L.v0 = fused_eval(sh) ? c.take<float>((size_t)g.Nw * sh.p_sel) : nullptr;
     L.factors = c.take<float>((size_t)nvar * sh.p_sel * sh.fstride);
     L.divq = optimize ? c.take<float>((size_t)nvar * sh.p_sel * 2 * sr) : nullptr;
     L.partial = c.take<double>(eblk);
@@ -1652,6 +1741,15 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
   hipchk(hipMemcpyAsync(L.wl, &zero, sizeof zero, hipMemcpyHostToDevice, s), "H2D list");
   if (optimize) hipchk(launch_delta2((int64_t)seed, vol, L.d2, s), "delta2");
   hipchk(hipStreamSynchronize(s), "setup sync");  // the host vectors above are local
+  GatherSrc gsrc;
+  gsrc.B0 = L.B0;
+  gsrc.factors = L.factors;
+  gsrc.dr = L.B0;
+  gsrc.rec = L.rec;
+  gsrc.W0 = L.w0;
+  gsrc.V0 = L.v0;
+  gsrc.list = L.list;
+  gsrc.nvar_stored = nvar;
   std::vector<double> fmax(2 * g.Nw), sums(6), hw;
   int status = 0;
   std::string warn;
@@ -1695,6 +1793,7 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
                       g_svd_mapping.load() == 1 ? 1 : 0),
            "svd");
     if (optimize) hipchk(launch_divq(L.factors, L.pos, L.sel, L.dw, 1, nvar, sh, L.divq, s), "divergence factors");
+    if (fused_eval(sh)) hipchk(launch_static_rows(L.factors, L.list, 1, nvar, sh, L.w0, L.v0, s), "static rows");
     double alpha = alpha0, mu = mu0, sigma = sigma0, lambda = lambda0;
     if (optimize) {
       // noise estimate and start point from the normalised window on the host
@@ -1727,10 +1826,10 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
         if (!known) {
           m.lambda = l;
           hipchk(hipMemcpyAsync(L.meta, &m, sizeof m, hipMemcpyHostToDevice, s), "H2D meta");
-          hipchk(launch_eval(L.factors, L.meta, L.wl, 1, nvar, 3, sh, p.exponential_weighting, L.divq, L.inv, L.B0,
-                             L.partial, s),
+          hipchk(launch_evaluation(sh, L.factors, L.divq, L.meta, L.wl, 1, nvar, 3, p.exponential_weighting, L.inv,
+                                   L.rec, L.B0, L.partial, nullptr, nullptr, s),
                  "eval");
-          hipchk(launch_gather(L.B0, L.off, L.invw, L.u64, L.meta, L.wl, 1, sh, 0, nullptr, L.partial_b, s), "gather");
+          hipchk(launch_gather(gsrc, L.off, L.invw, L.u64, L.meta, L.wl, 1, sh, 0, nullptr, L.partial_b, s), "gather");
           hipchk(launch_risk_reduce(L.partial, eblk, L.partial_b, gblk, 1, L.sums, s), "reduce");
           hipchk(hipMemcpyAsync(sums.data(), L.sums, 6 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H sums");
           hipchk(hipStreamSynchronize(s), "risk sync");
@@ -1753,10 +1852,10 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     // the output frame at the window's lambda (Pipeline::finish)
     m.lambda = lambda;
     hipchk(hipMemcpyAsync(L.meta, &m, sizeof m, hipMemcpyHostToDevice, s), "H2D meta");
-    hipchk(launch_eval(L.factors, L.meta, L.wl, 1, nvar, 1, sh, p.exponential_weighting, nullptr, L.inv, L.B0, nullptr,
-                       s),
+    hipchk(launch_evaluation(sh, L.factors, nullptr, L.meta, L.wl, 1, nvar, 1, p.exponential_weighting, L.inv, L.rec,
+                             L.B0, nullptr, nullptr, nullptr, s),
            "eval final");
-    hipchk(launch_gather(L.B0, L.off, L.invw, L.u64, L.meta, L.wl, 1, sh, 1, Y, nullptr, s), "gather final");
+    hipchk(launch_gather(gsrc, L.off, L.invw, L.u64, L.meta, L.wl, 1, sh, 1, Y, nullptr, s), "gather final");
     hipchk(hipStreamSynchronize(s), "window sync");  // m and jobs are reused by the next window
     est[t + (size_t)nf * 0] = lambda;
     est[t + (size_t)nf * 1] = alpha;
@@ -2036,10 +2135,8 @@ int pgsvt_window_reconstruct(const void* X, int dtype, uint32_t n_rows, uint32_t
     hipchk(hipMemcpyAsync(P.wl_d.p, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, P.s), "H2D list");
     DBuf<double> out;
     out.alloc(P.sh.vol);
-    hipchk(launch_eval(P.factors.p, P.meta_d.p, P.wl_d.p, 1, P.nvar, 1, P.sh, P.p.exponential_weighting, nullptr,
-                       P.inv.p, P.B0.p, nullptr, P.s),
-           "eval");
-    hipchk(launch_gather(P.B0.p, P.off.p, P.invw.p, P.u64.p, P.meta_d.p, P.wl_d.p, 1, P.sh, 2, out.p,
+    P.evaluate(P.wl_d.p, 1, 1, nullptr, nullptr, nullptr);
+    hipchk(launch_gather(P.gather_src(), P.off.p, P.invw.p, P.u64.p, P.meta_d.p, P.wl_d.p, 1, P.sh, 2, out.p,
                          nullptr, P.s),
            "gather");
     hipchk(hipMemcpyAsync(vol, out.p, out.bytes(), hipMemcpyDeviceToHost, P.s), "D2H vol");
@@ -2062,7 +2159,7 @@ int pgsvt_window_singular_values(const void* X, int dtype, uint32_t n_rows, uint
       return P.sh.fp64 ? reinterpret_cast<const double*>(f.data())[i] : (double)reinterpret_cast<const float*>(f.data())[i];
     };
     const uint32_t n = P.sh.nw;
-    const uint32_t s_at = n * ((n + 3) & ~3u) + n * P.sh.m;  // after V^T and W^T
+    const uint32_t s_at = n * ((n + 3) & ~3u) + n * P.sh.m;  // after the factor lines
     for (uint32_t k = 0; k < P.sh.p_sel; ++k) {
       std::vector<double> v(n);
       for (uint32_t j = 0; j < n; ++j) v[j] = at((size_t)k * P.sh.fstride + s_at + j);

@@ -7,7 +7,7 @@
 //   u32      float  [nb][nvar][vol]         variant windows (SVD, reconstruct)
 //   pos      int32  [nb][nw][p_all]         patch position, row | col << 16
 //   W        int32  [nb][vol]               overlap count per voxel
-//   factors  float  [nb][nvar][p_sel][fstride]  V (n x n row-major), s (n)
+//   factors  float  [nb][nvar][p_sel][fstride]  V^T (n x sr), W^T (n x m), s (n)
 //   acc      int64  [nb][nvar][vol]         fixed-point overlap-add sums
 #include <hip/hip_runtime.h>
 
@@ -231,12 +231,40 @@ hipError_t launch_fill_ref(const WinMeta* meta, uint32_t nb, const Shape& sh, in
 // reference does; ties go to the first index.  The visited map (checkMat)
 // lives in LDS, one bit per candidate offset, laid out word-major so that
 // consecutive threads hit consecutive banks.
-constexpr int kArpsMaxMw = 15;
-constexpr int kArpsWords = ((2 * kArpsMaxMw + 1) * (2 * kArpsMaxMw + 1) + 31) / 32;
-constexpr int kArpsThreads = 128;
+// Search windows up to kArpsMaxMw (2 * 15 + 1 offsets a side) run 128
+// threads per block; wider ones, up to kArpsMaxMwWide, 64 threads with a
+// larger visited map each.
+constexpr int kArpsMaxMw = 15, kArpsMaxMwWide = 31;
+template <int MW>
+struct ArpsCfg {
+  static constexpr int kWords = ((2 * MW + 1) * (2 * MW + 1) + 31) / 32;
+  static constexpr int kThreads = MW <= 15 ? 128 : 64;
+};
+
+// The reference block: held in registers up to 8 x 8, read from the source
+// frame (a cached, unchanging window) above that.
+template <int BS, bool REG = (BS <= 8)>
+struct ArpsRef {
+  double v[BS * BS];
+  __device__ void load(const double* S, uint32_t nr) {
+#pragma unroll
+    for (int q = 0; q < BS * BS; ++q) v[q] = S[(q % BS) + (size_t)nr * (q / BS)];
+  }
+  __device__ double operator[](int q) const { return v[q]; }
+};
+template <int BS>
+struct ArpsRef<BS, false> {
+  const double* S;
+  uint32_t nr;
+  __device__ void load(const double* S_, uint32_t nr_) {
+    S = S_;
+    nr = nr_;
+  }
+  __device__ double operator[](int q) const { return S[(q % BS) + (size_t)nr * (q / BS)]; }
+};
 
 template <int BS>
-__device__ __forceinline__ double arps_cost(const double (&ref)[BS * BS], const double* __restrict__ F, uint32_t nr,
+__device__ __forceinline__ double arps_cost(const ArpsRef<BS>& ref, const double* __restrict__ F, uint32_t nr,
                                             int r0, int c0, double inv) {
 #pragma clang fp contract(off)
   double a1 = 0.0, a2 = 0.0;
@@ -279,10 +307,8 @@ __device__ void arps_search(const double* __restrict__ win, uint32_t nr, uint32_
   const int i = it % n1, j = it / n1;
   const int Nx = (int)nr, Ny = (int)nc;
   const double inv = 1.0 / (BS * BS);
-  double ref[BS * BS];
-  const double* S = win + (size_t)src * fs + i + (size_t)nr * j;
-#pragma unroll
-  for (int q = 0; q < BS * BS; ++q) ref[q] = S[(q % BS) + (size_t)nr * (q / BS)];
+  ArpsRef<BS> ref;
+  ref.load(win + (size_t)src * fs + i + (size_t)nr * j, nr);
   const double* D = win + (size_t)dst * fs;
   const int Wd = 2 * mw + 1;
   vis.clear((Wd * Wd + 31) / 32);
@@ -364,18 +390,20 @@ __device__ void arps_search(const double* __restrict__ win, uint32_t nr, uint32_
 
 // jobs: int16 [nb][max_jobs][4] = (src_a, dst_a, src_b, dst_b); pass b (if
 // src_b >= 0) reads the motion pass a left in the same slot.
-template <int BS>
-__global__ __launch_bounds__(kArpsThreads) void k_arps(const double* __restrict__ w64, const int16_t* __restrict__ jobs,
-                                                       uint32_t max_jobs, uint32_t nr, uint32_t nc, uint32_t fs,
-                                                       uint32_t nw, uint32_t vol, uint32_t p_all, int n1, int mw,
-                                                       int32_t* __restrict__ pos) {
-  __shared__ uint32_t sv[kArpsWords * kArpsThreads];
+template <int BS, int MW>
+__global__ __launch_bounds__(ArpsCfg<MW>::kThreads) void k_arps(const double* __restrict__ w64,
+                                                                const int16_t* __restrict__ jobs, uint32_t max_jobs,
+                                                                uint32_t nr, uint32_t nc, uint32_t fs, uint32_t nw,
+                                                                uint32_t vol, uint32_t p_all, int n1, int mw,
+                                                                int32_t* __restrict__ pos) {
+  constexpr int kThreads = ArpsCfg<MW>::kThreads;
+  __shared__ uint32_t sv[ArpsCfg<MW>::kWords * kThreads];
   const uint32_t b = blockIdx.z, job = blockIdx.y;
   const int it = blockIdx.x * blockDim.x + threadIdx.x;
   const int16_t* J = jobs + ((size_t)b * max_jobs + job) * 4;
   const int sa = J[0], da = J[1], sb = J[2], db = J[3];
   if (sa < 0 || it >= (int)p_all) return;
-  Visit vis{sv + threadIdx.x, kArpsThreads, 2 * mw + 1};
+  Visit vis{sv + threadIdx.x, kThreads, 2 * mw + 1};
   const double* win = w64 + (size_t)b * vol;
   int y, x;
   arps_search<BS>(win, nr, nc, fs, n1, it, sa, da, 0, 0, mw, vis, y, x);
@@ -388,22 +416,30 @@ __global__ __launch_bounds__(kArpsThreads) void k_arps(const double* __restrict_
   }
 }
 
-hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs, uint32_t nb, const Shape& sh,
-                       int mw, int32_t* pos, hipStream_t s) {
-  if (mw > kArpsMaxMw) return hipErrorInvalidValue;
-  const dim3 grid((sh.p_all + kArpsThreads - 1) / kArpsThreads, max_jobs, nb);
-  const dim3 blk(kArpsThreads);
+template <int MW>
+hipError_t launch_arps_mw(const double* w64, const int16_t* jobs, uint32_t max_jobs, uint32_t nb, const Shape& sh,
+                          int mw, int32_t* pos, hipStream_t s) {
+  constexpr int kThreads = ArpsCfg<MW>::kThreads;
+  const dim3 grid((sh.p_all + kThreads - 1) / kThreads, max_jobs, nb);
+  const dim3 blk(kThreads);
 #define PG_ARPS(B)                                                                                              \
   case B:                                                                                                       \
-    hipLaunchKernelGGL(k_arps<B>, grid, blk, 0, s, w64, jobs, max_jobs, sh.nr, sh.nc, sh.fs, sh.nw, sh.vol,     \
+    hipLaunchKernelGGL((k_arps<B, MW>), grid, blk, 0, s, w64, jobs, max_jobs, sh.nr, sh.nc, sh.fs, sh.nw, sh.vol, \
                        sh.p_all, (int)sh.n1, mw, pos);                                                          \
     break;
   switch (sh.bs) {
-    PG_ARPS(2) PG_ARPS(3) PG_ARPS(4) PG_ARPS(5) PG_ARPS(6) PG_ARPS(7) PG_ARPS(8)
+    PG_ARPS(2) PG_ARPS(3) PG_ARPS(4) PG_ARPS(5) PG_ARPS(6) PG_ARPS(7) PG_ARPS(8) PG_ARPS(16)
     default: return hipErrorInvalidValue;
   }
 #undef PG_ARPS
   return hipGetLastError();
+}
+
+hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs, uint32_t nb, const Shape& sh,
+                       int mw, int32_t* pos, hipStream_t s) {
+  if (mw <= kArpsMaxMw) return launch_arps_mw<kArpsMaxMw>(w64, jobs, max_jobs, nb, sh, mw, pos, s);
+  if (mw <= kArpsMaxMwWide) return launch_arps_mw<kArpsMaxMwWide>(w64, jobs, max_jobs, nb, sh, mw, pos, s);
+  return hipErrorInvalidValue;
 }
 
 // --------------------------------------------------- batched Jacobi SVD --
@@ -479,13 +515,18 @@ __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
 __device__ __forceinline__ float rsq_(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsq_(double x) { return 1.0 / sqrt(x); }
 
-template <int BS, int N, int L, class T>
+// RT: the window length n = nw is known at run time only (n <= N): the
+// matrix is held as N columns, the last N - n zero, and every loop stops at
+// n (uniform branches); the stored layout follows n.
+template <int BS, int N, int L, class T, bool RT = false>
 __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const int32_t* __restrict__ pos,
                                              const uint32_t* __restrict__ sel, uint32_t nvar, uint32_t p_sel,
                                              uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol,
                                              uint32_t lim, uint32_t fstride, int max_sweeps, T tol2,
                                              T* __restrict__ factors, uint32_t* __restrict__ stats) {
-  constexpr int M = BS * BS, R = (M + L - 1) / L, SR = (N + 3) & ~3;
+  constexpr int M = BS * BS, R = (M + L - 1) / L, SRN = (N + 3) & ~3;
+  const int n = RT ? (int)nw : N;
+  const int SR = RT ? ((n + 3) & ~3) : SRN;
   const uint32_t b = blockIdx.y;
   const int lane = threadIdx.x % L;
   const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / L;
@@ -498,12 +539,13 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
   auto load_x = [&](T (&y)[R][N]) {
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-      const int32_t pk = valid ? pos[((size_t)b * nw + k) * p_all + it] : 0;
+      const bool kin = !RT || k < n;
+      const int32_t pk = (valid && kin) ? pos[((size_t)b * nw + k) * p_all + it] : 0;
       const uint32_t row = min((uint32_t)(pk & 0xffff), lim), col = min((uint32_t)(pk >> 16), lim);
 #pragma unroll
       for (int h = 0; h < R; ++h) {
         const int q = lane + L * h;
-        y[h][k] = (valid && q < M) ? U[(size_t)k * fs + (row + q % BS) + (size_t)nr * (col + q / BS)] : T(0);
+        y[h][k] = (valid && kin && q < M) ? U[(size_t)k * fs + (row + q % BS) + (size_t)nr * (col + q / BS)] : T(0);
       }
     }
   };
@@ -525,8 +567,10 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
     T tsum = T(0);
 #pragma unroll
     for (int i = 0; i < N - 1; ++i) {
+      if (RT && i + 1 >= n) break;
 #pragma unroll
       for (int j = i + 1; j < N; ++j) {
+        if (RT && j >= n) break;
         // rotation zeroing the (i, j) inner product: zeta = (b - a) / 2c,
         // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), cs = 1/sqrt(1+t^2)
         T cp = x[0][i] * x[0][j];
@@ -572,8 +616,8 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
   // W = X V = U S (the rotated Casorati columns), then the sorted values.
   // rank(l) counts the values that sort before column l (ties by index).
   T* F = factors + (((size_t)b * nvar + var) * p_sel + p) * fstride;
-  T* FW = F + N * SR;
-  T* FS = FW + N * M;
+  T* FW = F + n * SR;
+  T* FS = FW + n * M;
   int rank[N];
   T inv[N];
   T amax = a[0];
@@ -582,10 +626,12 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
   constexpr T eps = std::numeric_limits<T>::epsilon();
 #pragma unroll
   for (int l = 0; l < N; ++l) {
+    // (RT: the zero columns past n rank after every real one)
     rank[l] = 0;
 #pragma unroll
     for (int m = 0; m < N; ++m) rank[l] += (a[m] > a[l] || (a[m] == a[l] && m < l)) ? 1 : 0;
     inv[l] = a[l] > eps * eps * amax ? T(1) / a[l] : T(0);
+    if (RT && l >= n) continue;
 #pragma unroll
     for (int h = 0; h < R; ++h) {
       const int q = lane + L * h;
@@ -596,8 +642,9 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
   // V^T, column by column: element k of v_l = X^T w_l / s_l^2 from column
   // k of X (reloaded), stored by lane k % L
 #pragma unroll
-  for (int k = 0; k < SR; ++k) {
-    if (k < N) {
+  for (int k = 0; k < SRN; ++k) {
+    if (RT && k >= SR) break;
+    if (k < n) {
       const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
       const uint32_t row = min((uint32_t)(pk & 0xffff), lim), col = min((uint32_t)(pk >> 16), lim);
       T xk[R];
@@ -608,6 +655,7 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
       }
 #pragma unroll
       for (int l = 0; l < N; ++l) {
+        if (RT && l >= n) break;  // uniform: the zero columns
         T pv = xk[0] * x[0][l];
 #pragma unroll
         for (int h = 1; h < R; ++h) pv = fma(xk[h], x[h][l], pv);
@@ -617,7 +665,7 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
     } else {
 #pragma unroll
       for (int l = 0; l < N; ++l)
-        if (lane == k % L) F[rank[l] * SR + k] = T(0);
+        if ((!RT || l < n) && lane == k % L) F[rank[l] * SR + k] = T(0);
     }
   }
 }
@@ -816,29 +864,6 @@ __device__ __forceinline__ double row_bcast(double v) {
   return __builtin_bit_cast(double, o);
 }
 
-// bq[k] += y * (element k of v's row), k < N
-template <int K, int N, class T>
-__device__ __forceinline__ void axpy_row(T y, T v, T* bq) {
-  if constexpr (K < N) {
-    bq[K] = fma(y, row_bcast<K>(v), bq[K]);
-    axpy_row<K + 1, N>(y, v, bq);
-  }
-}
-
-// ranks R, R + 1, ... < nkeep of a reconstruction row: y = (g(s_r)/s_r,
-// held by lane r) * w_r[q], then bq += y v_r^T; same operations and order
-// as the general rank loop of k_eval.
-template <int R, int N, class T>
-__device__ __forceinline__ void rank_products(int nkeep, T dl, const T* vr, const T* wr, T* bq) {
-  if constexpr (R < N) {
-    if (R < nkeep) {
-      const T y = row_bcast<R>(dl) * wr[R];
-      axpy_row<0, N>(y, vr[R], bq);
-      rank_products<R + 1, N>(nkeep, dl, vr, wr, bq);
-    }
-  }
-}
-
 // Workgroups of a launch are handed to the 8 XCDs round-robin by linear
 // id, and each XCD has its own L2.  xcd_block maps the linear id so that XCD
 // x runs one contiguous range of logical (x, y) blocks: neighbouring blocks,
@@ -871,38 +896,42 @@ __device__ __forceinline__ void block_count(double* out, int v) {
 //   B = U diag(g(s)) V^T = sum over kept ranks r of (g(s_r)/s_r) w_r v_r^T
 // with w_r = X v_r = s_r u_r, stored by the SVD.  Factors are sorted by
 // descending s and g(s)/s grows with s, so the kept ranks are a prefix and
-// only their rows are read.  A group of L lanes owns one patch, lane q one
-// row of B; the U reconstruction's row of every frame goes to its
+// only their rows are read.  Patches above 4 x 4: a group of L = 64 lanes
+// owns one patch, lane q one row of B; the U reconstruction's row of every
+// frame goes to its
 // bucket-sorted place in `B0` ([slot][frame][patch in bucket order][row])
 // for the gather.
 // nvar_run == 3 (PGURE): the divergence sum sum_v (B1 - 2 B0 + B2)_v
 // delta2_v / W_v of src/pgure.hpp:136 comes from the divergence factors of
 // the variants U, U + eps2 delta2, U - eps2 delta2 (k_divq) as
 // sum_r (d1_r q1_r - 2 d0_r q0_r + d2_r q2_r), in FP64.
-template <int BS, int N, int L, class T>
+// RT: window length n = nw at run time (n <= N).  Patches above 8 x 8 give
+// each lane several rows of B (lane q rows q, q + L, ...).
+template <int BS, int N, int L, class T, bool RT = false>
 __global__ __launch_bounds__(256) void k_eval(
     const T* __restrict__ factors, const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl,
     uint32_t nvar_stored, uint32_t nvar_run, uint32_t p_sel, uint32_t nw, uint32_t fstride, int exp_w,
     const T* __restrict__ divq, const int32_t* __restrict__ inv, T* __restrict__ B0,
     double* __restrict__ partial, const double* __restrict__ slot_lam, double* __restrict__ kept) {
-  constexpr int M = BS * BS, G = 256 / L, SR = (N + 3) & ~3;
+  constexpr int M = BS * BS, G = 256 / L, SRN = (N + 3) & ~3, RQ = (M + L - 1) / L;
   __shared__ double red[4];
+  const int n = RT ? (int)nw : N;
+  const int SR = RT ? ((n + 3) & ~3) : SRN;
   uint32_t bx, slot;
   xcd_block(bx, slot);
   const uint32_t b = wl[slot];
   const int grp = threadIdx.x / L, lane = threadIdx.x % L;
   const uint32_t p = bx * G + grp;
   const bool valid = p < p_sel;
-  const bool row = valid && lane < M;
   const uint32_t pa = valid ? p : 0;  // addresses of groups past the end: patch 0
   const double lam = slot_lam ? slot_lam[slot] : meta[b].lambda;
   const T* F = factors + ((size_t)b * nvar_stored * p_sel + pa) * fstride;  // the U variant's factors
-  const T* FW = F + N * SR;
-  const T* FS = FW + N * M;
+  const T* FW = F + n * SR;
+  const T* FS = FW + n * M;
   const bool pgq = nvar_run == 3;
   // singular values (lane r: s_r) of U, and with PGURE those of the
   // perturbed variants and the divergence factors of all three
-  const int lk = min(lane, N - 1);
+  const int lk = min(lane, n - 1);
   const T sl = FS[lk], smax = FS[0];
   T sp[2] = {T(0), T(0)}, spm[2] = {T(0), T(0)}, ql[3] = {T(0), T(0), T(0)};
   if (pgq) {
@@ -920,7 +949,7 @@ __global__ __launch_bounds__(256) void k_eval(
   // of the group with g > 0
   auto shrink = [&](T s, T s0) {
     T d = T(0);
-    if (valid && lane < N) {
+    if (valid && lane < n) {
       const T w = exp_w ? fabs(s0 * exp((T)(-0.5 * lam) * (s * s))) : (T)lam;
       d = s > T(0) ? fmax(s - w, T(0)) / s : T(0);
     }
@@ -946,67 +975,62 @@ __global__ __launch_bounds__(256) void k_eval(
       s4 = fma((double)d, (double)ql[v + 1], s4);
     }
   }
-  T bq[N];
+  T bq[RQ][N];
 #pragma unroll
-  for (int k = 0; k < N; ++k) bq[k] = T(0);
-  if constexpr (L == 16) {
-    // A group is one 16-lane DPP row.  Lane j loads element j of every
-    // kept V^T row and its own W^T element, all before the first product
-    // (one memory wait for the whole rank loop); the products take V^T
-    // elements and g(s_r)/s_r from their lanes by row broadcasts.
-    // Entries r >= nkeep stay unset and are never read.  Lanes past the
-    // row length read an in-bounds element that is never broadcast (V^T)
-    // or stored (W^T: lanes >= M hold no row of B).
-    T vr[N], wr[N];
-    const int lv = min(lane, SR - 1), lw = min(lane, M - 1);
+  for (int h = 0; h < RQ; ++h)
 #pragma unroll
-    for (int r = 0; r < N; ++r) {
-      if (r < nkeep) {
-        vr[r] = F[r * SR + lv];
-        wr[r] = FW[r * M + lw];
-      }
-    }
-    rank_products<0, N>(nkeep, dl, vr, wr, bq);
-  } else {
+    for (int k = 0; k < N; ++k) bq[h][k] = T(0);
   // rank loop, loads of rank r + 1 issued before the products of rank r
-  // (FP32: the V^T row as SR/4 float4 loads; FP64: SR scalar loads)
-  constexpr int VW = (sizeof(T) == 4) ? 4 : 1, NV = SR / VW;
+  // (FP32: the V^T row as SR/4 float4 loads; FP64: SR scalar loads; with a
+  // window length known at run time the loads cover SRN values, within the
+  // factor block, and the values past n are never stored)
+  constexpr int VW = (sizeof(T) == 4) ? 4 : 1, NV = SRN / VW;
   typedef typename std::conditional<sizeof(T) == 4, float4, double>::type VT;
-  T wn = T(0);
+  T wn[RQ];
   VT vn[NV];
   auto fetch = [&](int r) {
-    wn = row ? FW[r * M + lane] : T(0);
+#pragma unroll
+    for (int h = 0; h < RQ; ++h) wn[h] = (valid && lane + L * h < M) ? FW[r * M + lane + L * h] : T(0);
     const VT* vr = reinterpret_cast<const VT*>(F + r * SR);
 #pragma unroll
     for (int j = 0; j < NV; ++j) vn[j] = vr[j];
   };
   if (nkeep > 0) fetch(0);
   for (int r = 0; r < nkeep; ++r) {
-    const T y = __shfl(dl, r, L) * wn;
+    const T dr = __shfl(dl, r, L);
+    T y[RQ];
+#pragma unroll
+    for (int h = 0; h < RQ; ++h) y[h] = dr * wn[h];
     VT vc[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) vc[j] = vn[j];
     if (r + 1 < nkeep) fetch(r + 1);
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      if constexpr (VW == 4) {
-        const float4 t = vc[j];
-        if (4 * j < N) bq[4 * j] = fma(y, t.x, bq[4 * j]);
-        if (4 * j + 1 < N) bq[4 * j + 1] = fma(y, t.y, bq[4 * j + 1]);
-        if (4 * j + 2 < N) bq[4 * j + 2] = fma(y, t.z, bq[4 * j + 2]);
-        if (4 * j + 3 < N) bq[4 * j + 3] = fma(y, t.w, bq[4 * j + 3]);
-      } else {
-        if (j < N) bq[j] = fma(y, (T)vc[j], bq[j]);
+    for (int h = 0; h < RQ; ++h) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        if constexpr (VW == 4) {
+          const float4 t = vc[j];
+          if (4 * j < N) bq[h][4 * j] = fma(y[h], t.x, bq[h][4 * j]);
+          if (4 * j + 1 < N) bq[h][4 * j + 1] = fma(y[h], t.y, bq[h][4 * j + 1]);
+          if (4 * j + 2 < N) bq[h][4 * j + 2] = fma(y[h], t.z, bq[h][4 * j + 2]);
+          if (4 * j + 3 < N) bq[h][4 * j + 3] = fma(y[h], t.w, bq[h][4 * j + 3]);
+        } else {
+          if (j < N) bq[h][j] = fma(y[h], (T)vc[j], bq[h][j]);
+        }
       }
     }
   }
-  }
-  if (row) {
+  if (valid) {
     const int32_t* I = inv + (size_t)b * nw * p_sel + p;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
+      if (RT && k >= n) break;
       // row of patch p for frame k at its bucket-sorted place
-      B0[(((size_t)slot * nw + k) * p_sel + I[(size_t)k * p_sel]) * M + lane] = bq[k];
+      const size_t at = (((size_t)slot * nw + k) * p_sel + I[(size_t)k * p_sel]) * M;
+#pragma unroll
+      for (int h = 0; h < RQ; ++h)
+        if (lane + L * h < M) B0[at + lane + L * h] = bq[h][k];
     }
   }
   if (!pgq) return;  // uniform over the launch
@@ -1060,19 +1084,26 @@ __device__ __forceinline__ V group_bcast(V v) {
 // D (m x n) the delta2/W volume at the patch's voxels along its trajectory.
 // One group of L lanes per (patch, variant): lane q holds row q of D, lane j
 // element j of each V^T row; t_q = sum_k D[q][k] v_r[k] and q_r = sum_q
-// w_r[q] t_q are formed in FP64 and stored in rank order in the patch's
-// record, divq [nb][p_sel][nvar][2][sr]: the variant's singular values
-// (zero-padded to sr), then its factors; the search rounds read s and q of
-// the three variants from there.  L = 16 (patches up to 4 x 4): every row
-// is loaded first; L = 64: one rank at a time.
-template <int BS, int N, int L, class T>
+// w_r[q] t_q are formed in FP64 and stored with the variant's singular
+// values (zero-padded to sr): the search rounds read s and q of the three
+// variants from there.  L = 16 (patches up to 4 x 4): every row is loaded
+// first, and the values go to rank-major planes divq [nb][nvar][2][sr][p_sel]
+// (s, then q), so that a round reads one coalesced value per patch and kept
+// rank; L = 64: one rank at a time, into per-patch records divq
+// [nb][p_sel][nvar][2][sr].
+// RT: window length n = nw known at run time only (n <= N), as in k_svd.
+// Patches above 8 x 8 give each lane several rows of D (lane q rows q,
+// q + L, ...).
+template <int BS, int N, int L, class T, bool RT = false>
 __global__ __launch_bounds__(256) void k_divq(const T* __restrict__ factors, const int32_t* __restrict__ pos,
                                               const uint32_t* __restrict__ sel, const T* __restrict__ dw,
                                               uint32_t nvar, uint32_t p_sel, uint32_t p_all, uint32_t nr,
                                               uint32_t fs, uint32_t nw, uint32_t vol, uint32_t lim,
                                               uint32_t fstride, T* __restrict__ divq) {
-  constexpr int M = BS * BS, SR = (N + 3) & ~3, G = 256 / L;
-  static_assert(M <= L && SR <= L, "one lane per row of D and per element of a V^T row");
+  constexpr int M = BS * BS, SRN = (N + 3) & ~3, G = 256 / L, RQ = (M + L - 1) / L;
+  static_assert(SRN <= L, "one lane per element of a V^T row");
+  const int n = RT ? (int)nw : N;
+  const int SR = RT ? ((n + 3) & ~3) : SRN;
   uint32_t bx, b;
   xcd_block(bx, b);
   const int lane = threadIdx.x % L;
@@ -1082,141 +1113,195 @@ __global__ __launch_bounds__(256) void k_divq(const T* __restrict__ factors, con
   const uint32_t var = ga / p_sel, p = ga % p_sel;
   const uint32_t it = p_sel == p_all ? p : sel[p];
   const T* F = factors + (((size_t)b * nvar + var) * p_sel + p) * fstride;
-  const T* FW = F + N * SR;
-  const int lk = min(lane, N - 1), lv = min(lane, SR - 1), lw = min(lane, M - 1);
-  const int rr = lw % BS, cc = lw / BS;
-  const T sv = FW[N * M + lk];  // s_lane
+  const T* FW = F + n * SR;
+  const int lk = min(lane, n - 1), lv = min(lane, SR - 1);
+  const T sv = FW[n * M + lk];  // s_lane
   // D frame by frame (lane k: frame k's trajectory position)
   const int32_t pk_l = pos[((size_t)b * nw + lk) * p_all + it];
   const T* DW = dw + (size_t)b * vol;
-  double d[N];
+  double d[RQ][N];
   static_for<0, N>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     const int32_t pkk = group_bcast<L, k>(pk_l);
     const uint32_t r0 = min((uint32_t)(pkk & 0xffff), lim), c0 = min((uint32_t)(pkk >> 16), lim);
-    d[k] = (double)DW[(uint32_t)k * fs + (r0 + rr) + nr * (c0 + cc)];
+#pragma unroll
+    for (int h = 0; h < RQ; ++h) {
+      const int lw = min(lane + L * h, M - 1);
+      d[h][k] = (!RT || k < n) ? (double)DW[(uint32_t)k * fs + (r0 + lw % BS) + nr * (c0 + lw / BS)] : 0.0;
+    }
   });
-  // q_r of rank r from lane j's V^T element and lane q's W^T element
-  auto factor = [&](T vr, T wr) {
-    double t = 0.0;
+  // q_r of rank r from lane j's V^T element and lane q's W^T elements
+  auto factor = [&](T vr, const T (&wr)[RQ]) {
+    double t[RQ];
+#pragma unroll
+    for (int h = 0; h < RQ; ++h) t[h] = 0.0;
     static_for<0, N>([&](auto kc) {
       constexpr int k = decltype(kc)::value;
-      t = fma(d[k], (double)group_bcast<L, k>(vr), t);
+      if (!RT || k < n) {
+        const double vk = (double)group_bcast<L, k>(vr);
+#pragma unroll
+        for (int h = 0; h < RQ; ++h) t[h] = fma(d[h][k], vk, t[h]);
+      }
     });
-    return group_sum<L>(lane < M ? (double)wr * t : 0.0);
+    double y = 0.0;
+#pragma unroll
+    for (int h = 0; h < RQ; ++h) y = (lane + L * h < M) ? fma((double)wr[h], t[h], y) : y;
+    return group_sum<L>(y);
+  };
+  auto wrow = [&](int r, T (&wr)[RQ]) {
+#pragma unroll
+    for (int h = 0; h < RQ; ++h) wr[h] = FW[r * M + min(lane + L * h, M - 1)];
   };
   double qv = 0.0;  // lane r: q_r
   if constexpr (L == 16) {
-    T vr[N], wr[N];
+    T vr[N], wr[N][RQ];
 #pragma unroll
     for (int r = 0; r < N; ++r) {
+      if (RT && r >= n) break;
       vr[r] = F[r * SR + lv];
-      wr[r] = FW[r * M + lw];
+      wrow(r, wr[r]);
     }
     static_for<0, N>([&](auto rc) {
       constexpr int r = decltype(rc)::value;
-      const double y = factor(vr[r], wr[r]);
-      if (lane == r) qv = y;
+      if (!RT || r < n) {
+        const double y = factor(vr[r], wr[r]);
+        if (lane == r) qv = y;
+      }
     });
   } else {
 #pragma unroll 1
-    for (int r = 0; r < N; ++r) {
-      const double y = factor(F[r * SR + lv], FW[r * M + lw]);
+    for (int r = 0; r < n; ++r) {
+      T wr[RQ];
+      wrow(r, wr);
+      const double y = factor(F[r * SR + lv], wr);
       if (lane == r) qv = y;
     }
   }
   if (valid && lane < SR) {
-    T* rec = divq + (((size_t)b * p_sel + p) * nvar + var) * 2 * SR;
-    rec[lane] = lane < N ? sv : T(0);
-    rec[SR + lane] = (T)qv;
+    if constexpr (L == 16) {
+      // rank-major planes [b][var][s | q][rank][patch]: a search round reads
+      // only the ranks it keeps, one coalesced value per patch
+      T* rec = divq + (((size_t)b * nvar + var) * 2 * SR + lane) * p_sel + p;
+      rec[0] = lane < n ? sv : T(0);
+      rec[(size_t)SR * p_sel] = (T)qv;
+    } else {
+      T* rec = divq + (((size_t)b * p_sel + p) * nvar + var) * 2 * SR;
+      rec[lane] = lane < n ? sv : T(0);
+      rec[SR + lane] = (T)qv;
+    }
   }
 }
 
-// PGURE evaluation for patches up to 4 x 4: one 16-lane group (one DPP
-// row) per patch, lane q = row q of B.  Only the U variant is
-// reconstructed: its row of every frame goes to its bucket-sorted place in
-// B0 (lane k holds frame k's bucket place, taken by row broadcast).  The
-// divergence sum sum_v (B1 - 2 B0 + B2)_v delta2_v / W_v of
-// src/pgure.hpp:136 comes from the divergence factors of the three
-// variants U, U + eps2 delta2, U - eps2 delta2 as
-// sum_r (d1_r q1_r - 2 d0_r q0_r + d2_r q2_r), in FP64.
-template <int BS, int N, class T>
-__global__ __launch_bounds__(256) void k_eval_pg(
-    const T* __restrict__ factors, const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl,
-    uint32_t nvar_stored, uint32_t p_sel, uint32_t nw, uint32_t fstride,
-    int exp_w, const T* __restrict__ divq, const int32_t* __restrict__ inv, T* __restrict__ B0,
-    double* __restrict__ partial, const double* __restrict__ slot_lam, double* __restrict__ kept) {
-  constexpr int M = BS * BS, SR = (N + 3) & ~3;
-  static_assert(M <= 16 && SR <= 16, "one 16-lane row per patch");
+// Lambda evaluation for patches up to 4 x 4, per patch (one thread each).
+// The reconstruction of a patch is B = sum_r d_r w_r v_r^T (src/svt.hpp:
+// 135-146 with src/utils.hpp:96-106), d_r = g(s_r)/s_r, w_r = X v_r = s_r u_r;
+// factors are sorted by descending s and g(s)/s grows with s, so the kept
+// ranks (d_r > 0) are a prefix.  The thread writes the patch's round record
+// (d_0, kept-rank count) at the patch's bucket place in every frame, rec
+// [slot][k][inv[b][k][p]], and the factors d_r of further kept ranks to dr
+// [slot][p][r].  The gather forms the patch's rows from them, the window's
+// static rows (k_static_rows: w_0 and v_0[k] at every bucket place) and, for
+// further ranks, w_r and v_r from the factors, as it stages them, so the
+// reconstruction itself is never written.  With PGURE (qdiv set) the thread
+// also adds the patch's divergence sum sum_r (d1_r q1_r - 2 d0_r q0_r +
+// d2_r q2_r) (src/pgure.hpp:136, see k_divq) to the FP64 partial of its
+// block.  The singular values and divergence factors come from the
+// rank-major planes of k_divq, or (qdiv null) the U variant's singular
+// values from its factors.  RT: window length n = nw at run time (n <= N).
+template <int BS, int N, class T, bool RT = false>
+__global__ __launch_bounds__(256) void k_round_prep(const T* __restrict__ factors, const T* __restrict__ qdiv,
+                                                    const int32_t* __restrict__ inv, const WinMeta* __restrict__ meta,
+                                                    const uint32_t* __restrict__ wl, uint32_t nvar_stored,
+                                                    uint32_t p_sel, uint32_t nw, uint32_t fstride, int exp_w,
+                                                    const double* __restrict__ slot_lam, T* __restrict__ rec,
+                                                    T* __restrict__ dr, double* __restrict__ partial,
+                                                    double* __restrict__ kept) {
+  constexpr int M = BS * BS;
   __shared__ double red[4];
-  uint32_t bx, slot;
-  xcd_block(bx, slot);
-  const uint32_t b = wl[slot];
-  const int lane = threadIdx.x % 16;
-  const uint32_t p = bx * 16 + threadIdx.x / 16;
+  const int n = RT ? (int)nw : N;
+  const int SR = (n + 3) & ~3;
+  const uint32_t slot = blockIdx.y, b = wl[slot];
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = p < p_sel;
-  const bool row = valid && lane < M;
-  const uint32_t pa = valid ? p : 0;  // addresses of groups past the end: patch 0
+  const uint32_t pa = valid ? p : 0;
   const double lam = slot_lam ? slot_lam[slot] : meta[b].lambda;
   const T* F = factors + ((size_t)b * nvar_stored * p_sel + pa) * fstride;  // the U variant's factors
-  // singular values and divergence factors of the three variants (lane r:
-  // s_r, q_r) from the patch's record (three whole 128-byte lines in FP32
-  // for n <= 16), frame lk = min(lane, N - 1)'s bucket place
-  const T* Q = divq + ((size_t)b * p_sel + pa) * nvar_stored * 2 * SR;
-  const int lk = min(lane, N - 1);
-  T sl[3], smax[3], ql[3];
-#pragma unroll
-  for (int v = 0; v < 3; ++v) {
-    sl[v] = Q[v * 2 * SR + lk];
-    smax[v] = Q[v * 2 * SR];
-    ql[v] = Q[v * 2 * SR + SR + lk];
-  }
-  const int32_t ib_l = inv[((size_t)b * nw + lk) * p_sel + pa];
-  const int lv = min(lane, SR - 1), lw = min(lane, M - 1);
+  const T* FW = F + n * SR;                                                  // W^T rows
+  auto sval = [&](int v, int r) -> T {
+    return qdiv ? qdiv[(((size_t)b * nvar_stored + v) * 2 * SR + r) * p_sel + pa] : FW[n * M + r];
+  };
+  auto qval = [&](int v, int r) -> T { return qdiv[(((size_t)b * nvar_stored + v) * 2 * SR + SR + r) * p_sel + pa]; };
+  // shrink factor of rank r (the expression of the reference's threshold,
+  // rounded as in k_eval)
+  auto shrink = [&](T s, T s0) -> T {
+    const T w = exp_w ? fabs(s0 * exp((T)(-0.5 * lam) * (s * s))) : (T)lam;
+    return s > T(0) ? fmax(s - w, T(0)) / s : T(0);
+  };
   double s4 = 0.0;
-  int kr = 0;  // kept ranks of the three variants (lane 0 of the group)
-  static_for<0, 3>([&](auto vc) {
-    constexpr int v = decltype(vc)::value;
-    // shrink factor g(s_r)/s_r in lane r < N, kept ranks of the group
-    T dl = T(0);
-    if (valid && lane < N) {
-      const T w = exp_w ? fabs(smax[v] * exp((T)(-0.5 * lam) * (sl[v] * sl[v]))) : (T)lam;
-      dl = sl[v] > T(0) ? fmax(sl[v] - w, T(0)) / sl[v] : T(0);
-    }
-    const uint64_t bal = __ballot(dl > T(0));
-    const uint32_t gbits = (uint32_t)((bal >> ((threadIdx.x % 64) & ~15u)) & 0xffffu);
-    const int nkeep = gbits ? 32 - __clz(gbits) : 0;
-    if (lane == 0 && valid) kr += nkeep;
-    // lanes past the kept ranks have d_r = 0
-    s4 = fma((v == 0 ? -2.0 : 1.0) * (double)dl, (double)ql[v], s4);
-    if constexpr (v == 0) {
-      // kept ranks of V^T (lane j: element j) and W^T (lane q: row q), read
-      // before the first product; entries past nkeep stay unset and unread
-      const T* Fv = F;
-      T vr[N], wr[N];
-#pragma unroll
-      for (int r = 0; r < N; ++r) {
-        if (r < nkeep) {
-          vr[r] = Fv[r * SR + lv];
-          wr[r] = Fv[N * SR + r * M + lw];
-        }
+  int kr = 0;  // kept ranks of the evaluated variants (profiling)
+  const int nvar_run = qdiv ? 3 : 1;
+  T d0 = T(0);
+  for (int v = 0; v < nvar_run && valid; ++v) {
+    const T s0 = sval(v, 0);
+    int r = 0;
+    for (; r < n; ++r) {
+      const T sr = (r == 0) ? s0 : sval(v, r);
+      const T d = shrink(sr, s0);
+      if (!(d > T(0))) break;
+      if (qdiv) s4 = fma((v == 0 ? -2.0 : 1.0) * (double)d, (double)qval(v, r), s4);
+      if (v == 0) {
+        if (r == 0)
+          d0 = d;
+        else
+          dr[((size_t)slot * p_sel + p) * nw + r] = d;
       }
-      T bq[N];
-#pragma unroll
-      for (int k = 0; k < N; ++k) bq[k] = T(0);
-      rank_products<0, N>(nkeep, dl, vr, wr, bq);
-      static_for<0, N>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        const int32_t ibk = row_bcast<k>(ib_l);
-        if (row) B0[(((size_t)slot * nw + k) * p_sel + ibk) * M + lane] = bq[k];
-      });
     }
-  });
-  for (int o = 32; o > 0; o >>= 1) s4 += __shfl_xor(s4, o);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = s4;
-  __syncthreads();
-  if (threadIdx.x == 0) partial[(size_t)slot * gridDim.x + bx] = ((red[0] + red[1]) + red[2]) + red[3];
-  if (kept) block_count(kept + (size_t)slot * gridDim.x + bx, kr);
+    if (v == 0) {
+      // d_0 and the kept-rank count (a small integer, exact in T) at the
+      // patch's place in every frame
+      typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type T2;
+      T2 rv;
+      rv.x = d0;
+      rv.y = (T)r;
+      const int32_t* I = inv + (size_t)b * nw * p_sel + p;
+      T2* Rb = reinterpret_cast<T2*>(rec) + (size_t)slot * nw * p_sel;
+      for (int k = 0; k < n; ++k) Rb[(size_t)k * p_sel + I[(size_t)k * p_sel]] = rv;
+    }
+    kr += r;
+  }
+  if (!qdiv && !kept) return;  // uniform over the launch
+  if (qdiv) {
+    for (int o = 32; o > 0; o >>= 1) s4 += __shfl_xor(s4, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = s4;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(size_t)slot * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+  }
+  if (kept) block_count(kept + (size_t)slot * gridDim.x + blockIdx.x, kr);
+}
+
+// The lambda-independent part of every patch row the fused gather forms,
+// once per window after the SVD, in the gather's access order: at bucket
+// place j of frame k (list[b][k][j] = patch id), W0 [b][k][j][m] = w_0 of
+// the patch (row 0 of its W^T) and V0 [b][k][j] = v_0[k].  A lambda
+// evaluation then forms the rank-0 row as fma(d_0 w_0[q], v_0[k], 0), the
+// FMA chain of the reconstruction, with d_0 from the patch's round record.
+template <int BS, class T>
+__global__ __launch_bounds__(256) void k_static_rows(const T* __restrict__ factors, const int32_t* __restrict__ list,
+                                                     uint32_t nvar_stored, uint32_t p_sel, uint32_t nw,
+                                                     uint32_t fstride, T* __restrict__ W0, T* __restrict__ V0) {
+  constexpr int M = BS * BS;
+  const uint32_t b = blockIdx.z, k = blockIdx.y;
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= p_sel) return;
+  const size_t at = ((size_t)b * nw + k) * p_sel + j;
+  const int32_t id = list[at];
+  const uint32_t sr = (nw + 3) & ~3u;
+  const T* F = factors + ((size_t)b * nvar_stored * p_sel + id) * fstride;  // the U variant's factors
+  const T* w = F + nw * sr;                                                 // row 0 of W^T
+  T* o = W0 + at * M;
+#pragma unroll
+  for (int q = 0; q < M; ++q) o[q] = w[q];
+  V0[at] = F[k];
 }
 
 // ----------------------------------------------------------- gather --
@@ -1332,35 +1417,39 @@ __global__ __launch_bounds__(256) void k_gather_direct(const T* __restrict__ B0,
         ((sm[threadIdx.x][0] + sm[threadIdx.x][1]) + sm[threadIdx.x][2]) + sm[threadIdx.x][3];
 }
 
-// Tiled gather for patches up to 4 x 4.  A block covers a tile of TR x TC
-// voxels of one frame; the patches that can cover the tile sit in one
-// contiguous bucket range per patch column, which the block copies into LDS
-// with coalesced float4 loads, the column ranges packed one after another
-// (TOT rows per tile), stored as one plane per patch row (lanes on
-// neighbouring voxels read neighbouring words).  Each thread then sums its
-// voxel: first the leading patch of each of its bs x bs buckets (one LDS
-// read and a select, no branch), then, only in waves where a bucket holds
-// several patches, the remaining ones.  The order is fixed, so the sums are
-// reproducible.  A tile with more than TOT patch rows, or a column range
-// longer than COLCAP (trajectories piling up), reads global memory in the
-// order of k_gather_direct instead.
+// Tiled gather for patches up to 4 x 4, fused with the reconstruction.  A
+// block covers a tile of TR x TC voxels of one frame k; the patches that can
+// cover the tile sit in one contiguous bucket range per patch column.  The
+// block forms those patches' rows for frame k, B[q][k] = sum over kept ranks
+// r of a_r[q] v_r[k] (a_r = d_r w_r from k_round_prep, v_r[k] an element of
+// the stored V^T; the FMA chain of the reconstruction, in rank order), and
+// stores them in LDS, the column ranges packed one after another (TOT rows
+// per tile), one plane per patch row (lanes on neighbouring voxels read
+// neighbouring words).  Each thread then sums its voxel: first the leading
+// patch of each of its bs x bs buckets (one LDS read and a select, no
+// branch), then, only in waves where a bucket holds several patches, the
+// remaining ones.  The order is fixed, so the sums are reproducible.  A tile
+// with more than TOT patch rows, or a column range longer than COLCAP
+// (trajectories piling up), forms each contribution where it is summed, in
+// the same order.  Blocks of one tile run frame after frame, so that the
+// factor rows the tile's patches use in every frame meet in one L2.
+//   mode 0: risk sums per block: sum Uhat^2, sum Uhat, sum Uhat U, sum U, sum U^2
+//   mode 1: final frame: Y(:,:,out) = uMax * Uhat(:,:,ref)
+//   mode 2: whole window volume of slot 0 into out
 template <int BS, class T>
-__global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, const int32_t* __restrict__ off,
-                                                     const double* __restrict__ invw,
-                                                     const double* __restrict__ u64,
-                                                     const WinMeta* __restrict__ meta,
-                                                     const uint32_t* __restrict__ wl, uint32_t nr, uint32_t fs,
-                                                     uint32_t nw, uint32_t vol, uint32_t n1, uint32_t p_sel,
-                                                     int mode, double* __restrict__ out,
-                                                     double* __restrict__ partial) {
+__global__ __launch_bounds__(256) void k_gather_tile(
+    const T* __restrict__ factors, const T* __restrict__ dr, const T* __restrict__ rec, const T* __restrict__ W0,
+    const T* __restrict__ V0,
+    const int32_t* __restrict__ list, const int32_t* __restrict__ off, const double* __restrict__ invw,
+    const double* __restrict__ u64, const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl, uint32_t nr,
+    uint32_t fs, uint32_t nw, uint32_t vol, uint32_t n1, uint32_t p_sel, uint32_t fstride, uint32_t nvar_stored,
+    int mode, double* __restrict__ out, double* __restrict__ partial) {
   // TOT: staged patch rows per tile, the bucket ranges of its NC patch
   // columns packed one after another; COLCAP: rows per patch column
   constexpr int M = BS * BS, TR = 32, TC = 8, NC = TC + BS - 1, NRW = TR + BS, TOT = 448, COLCAP = 64;
-  constexpr int RS = TOT + 1;                  // plane stride (odd: spreads the planes over banks)
-  constexpr int VW = 16 / (int)sizeof(T);      // values per 16-byte load
-  constexpr int W = (M % VW == 0) ? VW : 1;    // values per staging load
-  typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type VT;
-  typedef typename std::conditional<(W > 1), VT, T>::type ST;
+  constexpr int RS = TOT + 1;                                // plane stride (odd: spreads the planes over banks)
+  constexpr int W = (sizeof(T) == 4 && M % 4 == 0) ? 4 : 1;  // row values per thread while staging
+  typedef typename std::conditional<W == 4, float4, T>::type ST;
   // sB: staged patch rows, one plane per patch row q (plane q, LDS row j);
   // the pad element of each plane is read (and ignored) by empty buckets.
   // sP: first LDS row of each patch column.
@@ -1376,20 +1465,30 @@ __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, c
   xcd_block(bx, slot);
   const uint32_t b = wl[slot];
   const WinMeta m = meta[b];
-  const uint32_t ntr = (nr + TR - 1) / TR, ntc = (nr + TC - 1) / TC;  // square frames
-  const uint32_t tr = bx % ntr, tc = (bx / ntr) % ntc;
-  const uint32_t k = (mode == 1) ? m.ref : bx / (ntr * ntc);
+  const uint32_t ntr = (nr + TR - 1) / TR;  // square frames
+  const uint32_t k = (mode == 1) ? m.ref : bx % nw, tile = (mode == 1) ? bx : bx / nw;
+  const uint32_t tr = tile % ntr, tc = tile / ntr;
   const int r0 = (int)(tr * TR), c0 = (int)(tc * TC);
   const int r = r0 + (int)(threadIdx.x % TR), c = c0 + (int)(threadIdx.x / TR);
   const bool live = r < (int)nr && c < (int)nr;
   const int32_t* O = off + ((size_t)b * nw + k) * ((size_t)n1 * n1 + 1);
-  const T* Bk = B0 + ((size_t)slot * nw + k) * p_sel * M;
+  const int32_t* Lk = list + ((size_t)b * nw + k) * p_sel;  // frame k's patches in bucket order
+  // bucket place j: the static w_0 row and v_0[k] at W0k + j * M, V0k[j],
+  // the patch's round record (d_0, kept ranks) at Rk + 2 j; patch id: d_r of
+  // further ranks at Ds[id * nw + r]; in the U variant's factors w_r at
+  // Ws + id * fstride + r * M (W^T) and v_r[k] at Vk + id * fstride + r * sr
+  // (V^T)
+  const uint32_t sr = (nw + 3) & ~3u;
+  const T* W0k = W0 + ((size_t)b * nw + k) * p_sel * M;
+  const T* V0k = V0 + ((size_t)b * nw + k) * p_sel;
+  const T* Rk = rec + ((size_t)slot * nw + k) * p_sel * 2;
+  const T* Ds = dr + (size_t)slot * p_sel * nw;
+  const T* Vk = factors + (size_t)b * nvar_stored * p_sel * fstride + k;
+  const T* Ws = factors + (size_t)b * nvar_stored * p_sel * fstride + nw * sr;
   // the voxel's 1/W and U are independent of the staging: load them first
   const uint32_t pix = (uint32_t)min(r, (int)nr - 1) + nr * (uint32_t)min(c, (int)nr - 1), v = k * fs + pix;
   const double iw = invw[(size_t)b * vol + v];
   const double U = (mode == 0) ? u64[(size_t)b * vol + v] : 0.0;
-  // Both staging copies have a fixed trip count per thread and issue all
-  // their loads before the first LDS write (one memory wait per copy).
   {
     constexpr int NO = (NC * NRW + 255) / 256;
     int32_t ov[NO];
@@ -1423,10 +1522,18 @@ __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, c
     if ((int)threadIdx.x == lc) sP[lc] = pre[lc];
   if (fits) {
     // patch column lc: thread -> (row lr, values q .. q + W - 1) of the
-    // column's bucket range, RPI rows per pass
+    // column's bucket range, RPI rows per pass.  The patch indices of every
+    // row are loaded first, then every row is formed.
     constexpr int QW = M / W, RPI = 256 / QW, NH = (COLCAP + RPI - 1) / RPI;
     const int lr0 = (int)threadIdx.x / QW, q = ((int)threadIdx.x % QW) * W;
-    ST sv[NC][NH];
+    // Every load of a row is at its bucket place (the patch index, the
+    // static w_0 values of the thread, v_0[k], the round record), issued for
+    // every row before the first use; further ranks are read as needed.
+    // Row value q: the FMA chain of the reconstruction over the kept ranks,
+    // fma(d_r w_r[q], v_r[k], .), rank 0 first.
+    int32_t ids[NC][NH];
+    ST w0[NC][NH];
+    T v0[NC][NH], d0[NC][NH], kn[NC][NH];
 #pragma unroll
     for (int lc = 0; lc < NC; ++lc) {
       const int32_t j0 = sT[lc * NRW], cnt = sT[lc * NRW + NRW - 1] - j0;
@@ -1434,19 +1541,37 @@ __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, c
       for (int h = 0; h < NH; ++h) {
         const int lr = lr0 + RPI * h;
         const bool use = lr0 < RPI && lr < cnt;
-        sv[lc][h] = *reinterpret_cast<const ST*>(Bk + (use ? (size_t)(j0 + lr) * M + q : 0));
+        const int32_t j = use ? j0 + lr : 0;
+        ids[lc][h] = use ? Lk[j] : -1;
+        w0[lc][h] = *reinterpret_cast<const ST*>(W0k + (size_t)j * M + q);
+        v0[lc][h] = V0k[j];
+        d0[lc][h] = Rk[2 * (size_t)j];
+        kn[lc][h] = Rk[2 * (size_t)j + 1];
       }
     }
 #pragma unroll
     for (int lc = 0; lc < NC; ++lc) {
-      const int32_t cnt = sT[lc * NRW + NRW - 1] - sT[lc * NRW];
 #pragma unroll
       for (int h = 0; h < NH; ++h) {
-        const int lr = lr0 + RPI * h;
-        if (lr0 < RPI && lr < cnt) {
-          const T* vv = reinterpret_cast<const T*>(&sv[lc][h]);
+        if (ids[lc][h] >= 0) {
+          const int n = (int)kn[lc][h];
+          const int32_t id = ids[lc][h];
+          ST row;
+          T* ac = reinterpret_cast<T*>(&row);
+          const T* ww = reinterpret_cast<const T*>(&w0[lc][h]);
 #pragma unroll
-          for (int w = 0; w < W; ++w) sB[(q + w) * RS + pre[lc] + lr] = vv[w];
+          for (int w = 0; w < W; ++w) ac[w] = (n > 0) ? fma(d0[lc][h] * ww[w], v0[lc][h], T(0)) : T(0);
+          for (int rk = 1; rk < n; ++rk) {
+            const T vk = Vk[(size_t)id * fstride + (size_t)rk * sr];
+            const T d = Ds[(size_t)id * nw + rk];
+            const ST wv = *reinterpret_cast<const ST*>(Ws + (size_t)id * fstride + (size_t)rk * M + q);
+            const T* wk = reinterpret_cast<const T*>(&wv);
+#pragma unroll
+            for (int w = 0; w < W; ++w) ac[w] = fma(d * wk[w], vk, ac[w]);
+          }
+          const int lr = lr0 + RPI * h;
+#pragma unroll
+          for (int w = 0; w < W; ++w) sB[(q + w) * RS + pre[lc] + lr] = ac[w];
         }
       }
     }
@@ -1497,8 +1622,21 @@ __global__ __launch_bounds__(256) void k_gather_tile(const T* __restrict__ B0, c
       for (int cb = 0; cb < BS; ++cb)
 #pragma unroll
         for (int ra = 0; ra < BS; ++ra) {
-          const int e = BS - 1 - ra;
-          for (int32_t j = bnd[cb][e]; j < bnd[cb][e + 1]; ++j) acc += (double)Bk[(size_t)j * M + ra + BS * cb];
+          const int e = BS - 1 - ra, qv = ra + BS * cb;
+          for (int32_t j = bnd[cb][e]; j < bnd[cb][e + 1]; ++j) {
+            // element qv of the patch's row, formed as in the staging
+            const int32_t id = Lk[j];
+            const T* R = Rk + 2 * (size_t)j;
+            const int n = (int)R[1];
+            T x = T(0);
+            for (int rk = 0; rk < n; ++rk) {
+              const T ar = rk ? Ds[(size_t)id * nw + rk] * Ws[(size_t)id * fstride + (size_t)rk * M + qv]
+                              : R[0] * W0k[(size_t)j * M + qv];
+              const T vk = rk ? Vk[(size_t)id * fstride + (size_t)rk * sr] : V0k[j];
+              x = fma(ar, vk, x);
+            }
+            acc += (double)x;
+          }
         }
     }
     const double uh = acc * iw;
@@ -1590,6 +1728,33 @@ __global__ __launch_bounds__(256) void k_risk_reduce(const double* __restrict__ 
   X(2, 3, 2) X(3, 3, 8) X(3, 5, 8) X(3, 7, 8) X(3, 9, 8) \
   X(6, 7, 32) X(6, 15, 32) X(6, 31, 64) X(8, 7, 32) X(8, 15, 64) X(8, 31, 32)
 
+// Every other shape, FP32 only: the window length is known at run time
+// (the kernels' RT form) up to NM frames.  (patch size, NM, lanes per
+// matrix in the SVD, lanes per patch in the evaluation and the divergence
+// factors; 16 for patches up to 4 x 4, whose evaluation is the fused one)
+#define PG_GENERIC(X) \
+  X(2, 16, 4, 16) X(3, 16, 4, 16) X(4, 16, 4, 16) \
+  X(5, 16, 16, 64) X(5, 32, 16, 64) X(6, 16, 16, 64) X(6, 32, 32, 64) X(7, 16, 16, 64) X(7, 32, 32, 64) \
+  X(8, 16, 32, 64) X(8, 32, 64, 64) X(16, 16, 64, 64)
+
+bool shape_exact(uint32_t bs, uint32_t nw) {
+#define PG_CHK(B, N, L, LS) \
+  if (bs == B && nw == N) return true;
+  PG_SHAPES(PG_CHK)
+#undef PG_CHK
+  return false;
+}
+
+// the generic family that runs a shape: its NM, or 0
+uint32_t shape_generic(uint32_t bs, uint32_t nw) {
+  if (nw < 1) return 0;
+#define PG_GCHK(B, NM, L, LE) \
+  if (bs == B && nw <= NM) return NM;
+  PG_GENERIC(PG_GCHK)
+#undef PG_GCHK
+  return 0;
+}
+
 uint32_t svd_lanes(uint32_t bs, uint32_t nw, int which) {
 #define PG_L1(B, N, LE, L) \
   if (which == 0 && bs == B && nw == N) return L;
@@ -1602,12 +1767,8 @@ uint32_t svd_lanes(uint32_t bs, uint32_t nw, int which) {
   return 0;
 }
 
-bool svd_shape_supported(uint32_t bs, uint32_t nw) {
-#define PG_CHK(B, N, L, LS) \
-  if (bs == B && nw == N) return true;
-  PG_SHAPES(PG_CHK)
-#undef PG_CHK
-  return false;
+bool svd_shape_supported(uint32_t bs, uint32_t nw, bool fp64) {
+  return shape_exact(bs, nw) || (!fp64 && shape_generic(bs, nw) != 0);
 }
 
 hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
@@ -1642,14 +1803,108 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
   if (sh.bs == B && sh.nw == N && mapping == 1) PG_SVD_F32(B, N, L)
   PG_SVD_ALT(PG_SVD2)
 #undef PG_SVD2
+  // an exact shape without an alternative mapping: the primary one
+#define PG_SVD3(B, N, LE, L)                                                                                     \
+  if (sh.bs == B && sh.nw == N && !sh.fp64) PG_SVD_F32(B, N, L)
+  PG_SHAPES(PG_SVD3)
+#undef PG_SVD3
 #undef PG_SVD_F32
+  if (sh.fp64) return hipErrorInvalidValue;
+  const uint32_t nm = shape_generic(sh.bs, sh.nw);
+#define PG_SVDG(B, NM, L, LE)                                                                                    \
+  if (sh.bs == B && nm == NM) {                                                                                \
+    const dim3 grid((groups * L + 255) / 256, nb);                                                             \
+    hipLaunchKernelGGL((k_svd<B, NM, L, float, true>), grid, dim3(256), 0, s, (const float*)uvar, pos, sel,    \
+                       nvar, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps,      \
+                       (float)(tol * tol), (float*)factors, stats);                                            \
+    return hipGetLastError();                                                                                  \
+  }
+  PG_GENERIC(PG_SVDG)
+#undef PG_SVDG
   return hipErrorInvalidValue;
 }
 
+bool fused_eval(const Shape& sh) { return sh.bs <= 4; }
+
+uint32_t factor_stride(uint32_t nw, uint32_t m) {
+  const uint32_t sr = (nw + 3) & ~3u;
+  return (nw * sr + nw * m + nw + 31) & ~31u;
+}
+
 uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run) {
-  (void)nvar_run;  // one group per patch either way
-  const uint32_t G = (sh.m <= 16) ? 16 : (sh.m <= 32 ? 8 : 4);
-  return (sh.p_sel + G - 1) / G;
+  (void)nvar_run;  // one group (one thread with the fused gather) per patch either way
+  if (fused_eval(sh)) return (sh.p_sel + 255) / 256;
+  return (sh.p_sel + 3) / 4;  // one 64-lane group per patch
+}
+
+hipError_t launch_evaluation(const Shape& sh, const void* factors, const void* divq, const WinMeta* meta,
+                             const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, int exp_w,
+                             const int32_t* inv, void* rec, void* rows, double* partial, const double* slot_lam,
+                             double* kept, hipStream_t s) {
+  if (fused_eval(sh))
+    return launch_round_prep(factors, nvar_run == 3 ? divq : nullptr, inv, meta, wl, na, nvar_stored, sh, exp_w,
+                             slot_lam, rec, rows, nvar_run == 3 ? partial : nullptr, kept, s);
+  return launch_eval(factors, meta, wl, na, nvar_stored, nvar_run, sh, exp_w, divq, inv, rows, partial, s, slot_lam,
+                     kept);
+}
+
+size_t round_record_bytes(const Shape& sh) { return (size_t)sh.nw * sh.p_sel * 2 * (sh.fp64 ? 8 : 4); }
+
+hipError_t launch_static_rows(const void* factors, const int32_t* list, uint32_t nb, uint32_t nvar_stored,
+                              const Shape& sh, void* W0, void* V0, hipStream_t s) {
+  const dim3 grid((sh.p_sel + 255) / 256, sh.nw, nb);
+#define PG_SR(B)                                                                                                 \
+  case B:                                                                                                      \
+    if (sh.fp64)                                                                                               \
+      hipLaunchKernelGGL((k_static_rows<B, double>), grid, dim3(256), 0, s, (const double*)factors, list,       \
+                         nvar_stored, sh.p_sel, sh.nw, sh.fstride, (double*)W0, (double*)V0);                  \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_static_rows<B, float>), grid, dim3(256), 0, s, (const float*)factors, list,         \
+                         nvar_stored, sh.p_sel, sh.nw, sh.fstride, (float*)W0, (float*)V0);                    \
+    break;
+  switch (sh.bs) {
+    PG_SR(2) PG_SR(3) PG_SR(4)
+    default: return hipErrorInvalidValue;
+  }
+#undef PG_SR
+  return hipGetLastError();
+}
+
+hipError_t launch_round_prep(const void* factors, const void* qdiv, const int32_t* inv, const WinMeta* meta,
+                             const uint32_t* wl, uint32_t na, uint32_t nvar_stored, const Shape& sh, int exp_w,
+                             const double* slot_lam, void* rec, void* dr, double* partial, double* kept,
+                             hipStream_t s) {
+  const uint32_t nblk = eval_blocks(sh, 1);
+#define PG_RP(B, N, L, LS)                                                                                       \
+  if (sh.bs == B && sh.nw == N) {                                                                              \
+    if constexpr (B <= 4) {                                                                                    \
+      if (sh.fp64)                                                                                             \
+        hipLaunchKernelGGL((k_round_prep<B, N, double>), dim3(nblk, na), dim3(256), 0, s,                      \
+                           (const double*)factors, (const double*)qdiv, inv, meta, wl, nvar_stored, sh.p_sel,   \
+                           sh.nw, sh.fstride, exp_w, slot_lam, (double*)rec, (double*)dr, partial, kept);       \
+      else                                                                                                     \
+        hipLaunchKernelGGL((k_round_prep<B, N, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors, \
+                           (const float*)qdiv, inv, meta, wl, nvar_stored, sh.p_sel, sh.nw, sh.fstride, exp_w,  \
+                           slot_lam, (float*)rec, (float*)dr, partial, kept);                                   \
+      return hipGetLastError();                                                                                \
+    }                                                                                                          \
+  }
+  PG_SHAPES(PG_RP)
+#undef PG_RP
+  if (sh.fp64) return hipErrorInvalidValue;
+  const uint32_t nm = shape_generic(sh.bs, sh.nw);
+#define PG_RPG(B, NM, L, LE)                                                                                     \
+  if constexpr (B <= 4) {                                                                                      \
+    if (sh.bs == B && nm == NM) {                                                                              \
+      hipLaunchKernelGGL((k_round_prep<B, NM, float, true>), dim3(nblk, na), dim3(256), 0, s,                  \
+                         (const float*)factors, (const float*)qdiv, inv, meta, wl, nvar_stored, sh.p_sel,       \
+                         sh.nw, sh.fstride, exp_w, slot_lam, (float*)rec, (float*)dr, partial, kept);           \
+      return hipGetLastError();                                                                                \
+    }                                                                                                          \
+  }
+  PG_GENERIC(PG_RPG)
+#undef PG_RPG
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_eval(const void* factors, const WinMeta* meta, const uint32_t* wl, uint32_t na,
@@ -1660,31 +1915,33 @@ hipError_t launch_eval(const void* factors, const WinMeta* meta, const uint32_t*
   const uint32_t nblk = eval_blocks(sh, nvar_run);
 #define PG_EV(B, N, L, LS)                                                                                       \
   if (sh.bs == B && sh.nw == N) {                                                                              \
-    if constexpr (L == 16) {                                                                                   \
-      if (nvar_run == 3) {                                                                                     \
-        if (sh.fp64)                                                                                           \
-          hipLaunchKernelGGL((k_eval_pg<B, N, double>), dim3(nblk, na), dim3(256), 0, s,                       \
-                             (const double*)factors, meta, wl, nvar_stored, sh.p_sel, sh.nw, sh.fstride, exp_w, \
-                             (const double*)divq, inv, (double*)B0, partial, slot_lam, kept);                  \
-        else                                                                                                   \
-          hipLaunchKernelGGL((k_eval_pg<B, N, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors, \
-                             meta, wl, nvar_stored, sh.p_sel, sh.nw, sh.fstride, exp_w, (const float*)divq,     \
-                             inv, (float*)B0, partial, slot_lam, kept);                                        \
-        return hipGetLastError();                                                                              \
-      }                                                                                                        \
+    if constexpr (L == 64) {                                                                                   \
+      if (sh.fp64)                                                                                             \
+        hipLaunchKernelGGL((k_eval<B, N, L, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)factors,  \
+                           meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.nw, sh.fstride, exp_w,                 \
+                           (const double*)divq, inv, (double*)B0, partial, slot_lam, kept);                    \
+      else                                                                                                     \
+        hipLaunchKernelGGL((k_eval<B, N, L, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors,    \
+                           meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.nw, sh.fstride, exp_w,                 \
+                           (const float*)divq, inv, (float*)B0, partial, slot_lam, kept);                      \
+      return hipGetLastError();                                                                                \
     }                                                                                                          \
-    if (sh.fp64)                                                                                               \
-      hipLaunchKernelGGL((k_eval<B, N, L, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)factors,    \
-                         meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.nw, sh.fstride, exp_w,                   \
-                         (const double*)divq, inv, (double*)B0, partial, slot_lam, kept);                      \
-    else                                                                                                       \
-      hipLaunchKernelGGL((k_eval<B, N, L, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors,      \
-                         meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.nw, sh.fstride, exp_w,                   \
-                         (const float*)divq, inv, (float*)B0, partial, slot_lam, kept);                        \
-    return hipGetLastError();                                                                                  \
   }
   PG_SHAPES(PG_EV)
 #undef PG_EV
+  if (sh.fp64) return hipErrorInvalidValue;
+  const uint32_t nm = shape_generic(sh.bs, sh.nw);
+#define PG_EVG(B, NM, L, LE)                                                                                     \
+  if constexpr (LE == 64) {                                                                                    \
+    if (sh.bs == B && nm == NM) {                                                                              \
+      hipLaunchKernelGGL((k_eval<B, NM, LE, float, true>), dim3(nblk, na), dim3(256), 0, s,                    \
+                         (const float*)factors, meta, wl, nvar_stored, nvar_run, sh.p_sel, sh.nw, sh.fstride,   \
+                         exp_w, (const float*)divq, inv, (float*)B0, partial, slot_lam, kept);                 \
+      return hipGetLastError();                                                                                \
+    }                                                                                                          \
+  }
+  PG_GENERIC(PG_EVG)
+#undef PG_EVG
   return hipErrorInvalidValue;
 }
 
@@ -1706,6 +1963,18 @@ hipError_t launch_divq(const void* factors, const int32_t* pos, const uint32_t* 
   }
   PG_SHAPES(PG_DQ)
 #undef PG_DQ
+  if (sh.fp64) return hipErrorInvalidValue;
+  const uint32_t nm = shape_generic(sh.bs, sh.nw);
+#define PG_DQG(B, NM, L, LE)                                                                                     \
+  if (sh.bs == B && nm == NM) {                                                                                \
+    const dim3 grid((nvar * sh.p_sel + 256 / LE - 1) / (256 / LE), nb);                                       \
+    hipLaunchKernelGGL((k_divq<B, NM, LE, float, true>), grid, dim3(256), 0, s, (const float*)factors, pos,   \
+                       sel, (const float*)dw, nvar, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim,      \
+                       sh.fstride, (float*)divq);                                                              \
+    return hipGetLastError();                                                                                  \
+  }
+  PG_GENERIC(PG_DQG)
+#undef PG_DQG
   return hipErrorInvalidValue;
 }
 
@@ -1716,25 +1985,43 @@ uint32_t gather_blocks(const Shape& sh, int mode) {
   return ((mode == 1 ? sh.fs : sh.vol) + 255) / 256;
 }
 
-hipError_t launch_gather(const void* B0, const int32_t* off, const double* invw, const double* u64,
+hipError_t launch_gather(const GatherSrc& src, const int32_t* off, const double* invw, const double* u64,
                          const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int mode, double* out,
                          double* partial, hipStream_t s) {
   const uint32_t nblk = gather_blocks(sh, mode);
-#define PG_GA(B, K)                                                                                              \
+#define PG_GT(B)                                                                                                 \
   case B:                                                                                                      \
     if (sh.fp64)                                                                                               \
-      hipLaunchKernelGGL((K<B, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)B0, off, invw, u64,    \
-                         meta, wl, sh.nr, sh.fs, sh.nw, sh.vol, sh.n1, sh.p_sel, mode, out, partial);           \
+      hipLaunchKernelGGL((k_gather_tile<B, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)src.factors, \
+                         (const double*)src.dr, (const double*)src.rec, (const double*)src.W0,                 \
+                         (const double*)src.V0, src.list, off, invw, u64, meta, wl,                            \
+                         sh.nr, sh.fs, sh.nw,                                                                  \
+                         sh.vol, sh.n1, sh.p_sel, sh.fstride, src.nvar_stored, mode, out, partial);            \
     else                                                                                                       \
-      hipLaunchKernelGGL((K<B, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)B0, off, invw, u64,      \
-                         meta, wl, sh.nr, sh.fs, sh.nw, sh.vol, sh.n1, sh.p_sel, mode, out, partial);           \
+      hipLaunchKernelGGL((k_gather_tile<B, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)src.factors,  \
+                         (const float*)src.dr, (const float*)src.rec, (const float*)src.W0,                    \
+                         (const float*)src.V0, src.list, off, invw, u64, meta, wl,                             \
+                         sh.nr, sh.fs, sh.nw,                                                                  \
+                         sh.vol, sh.n1, sh.p_sel, sh.fstride, src.nvar_stored, mode, out, partial);            \
+    break;
+#define PG_GA(B)                                                                                                 \
+  case B:                                                                                                      \
+    if (sh.fp64)                                                                                               \
+      hipLaunchKernelGGL((k_gather_direct<B, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)src.B0,   \
+                         off, invw, u64, meta, wl, sh.nr, sh.fs, sh.nw, sh.vol, sh.n1, sh.p_sel, mode, out,      \
+                         partial);                                                                             \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_gather_direct<B, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)src.B0,     \
+                         off, invw, u64, meta, wl, sh.nr, sh.fs, sh.nw, sh.vol, sh.n1, sh.p_sel, mode, out,      \
+                         partial);                                                                             \
     break;
   switch (sh.bs) {
-    PG_GA(2, k_gather_tile) PG_GA(3, k_gather_tile) PG_GA(4, k_gather_tile)
-    PG_GA(5, k_gather_direct) PG_GA(6, k_gather_direct) PG_GA(7, k_gather_direct) PG_GA(8, k_gather_direct)
+    PG_GT(2) PG_GT(3) PG_GT(4)
+    PG_GA(5) PG_GA(6) PG_GA(7) PG_GA(8) PG_GA(16)
     default: return hipErrorInvalidValue;
   }
 #undef PG_GA
+#undef PG_GT
   return hipGetLastError();
 }
 

@@ -20,11 +20,14 @@ struct Shape {
   uint32_t n1;          // nr - bs + 1 (grid positions per axis)
   uint32_t p_all;       // n1 * n1
   uint32_t p_sel;       // selected patches (src/svt.hpp:61-92)
-  uint32_t fstride;     // values per stored factor block (V^T n x sr, W^T n x m, s n)
+  uint32_t fstride;     // values per stored factor block (factor_stride)
   bool fp64 = false;    // SVD input, factors and reconstructions in FP64 (precision study)
 };
 
-bool svd_shape_supported(uint32_t bs, uint32_t nw);
+// (patch size, window length) pairs the kernels run: the instantiated table
+// in FP32 and FP64, any window up to 15 frames (patch 16) or 31 (patches
+// 5 to 8) and every window of patches up to 4 x 4 in FP32
+bool svd_shape_supported(uint32_t bs, uint32_t nw, bool fp64);
 
 hipError_t launch_to_double(const void* raw, int dtype, double* out, size_t n, hipStream_t s);
 hipError_t launch_frame_max(const double* X, uint32_t fs, uint32_t nf, double* out, hipStream_t s);
@@ -66,11 +69,53 @@ hipError_t launch_eval(const void* factors, const WinMeta* meta, const uint32_t*
 // its singular values (zero-padded) then its factors
 hipError_t launch_divq(const void* factors, const int32_t* pos, const uint32_t* sel, const void* dw, uint32_t nb,
                        uint32_t nvar, const Shape& sh, void* divq, hipStream_t s);
+// Patches up to 4 x 4 evaluate lambda in two launches: launch_round_prep
+// (per patch: its round record, d_0 and the kept-rank count, at its bucket
+// place j = inv[b][k][p] of every frame k, rec [na][nw][p_sel][2]
+// (round_record_bytes per slot), the factors d_r of further kept
+// ranks into dr [na][p_sel][nw]; with PGURE (qdiv = divq) the divergence
+// sums into partial [na][eval_blocks]), then launch_gather, which forms the
+// patch rows from them, the window's static rows (launch_static_rows, once
+// per window after the SVD: W0 [nb][nw][p_sel][m], V0 [nb][nw][p_sel] in
+// bucket order) and the factors as it gathers.  Larger patches: launch_eval
+// writes the rows to B0, launch_gather reads them.
+bool fused_eval(const Shape& sh);
+size_t round_record_bytes(const Shape& sh);
+hipError_t launch_static_rows(const void* factors, const int32_t* list, uint32_t nb, uint32_t nvar_stored,
+                              const Shape& sh, void* W0, void* V0, hipStream_t s);
+hipError_t launch_round_prep(const void* factors, const void* qdiv, const int32_t* inv, const WinMeta* meta,
+                             const uint32_t* wl, uint32_t na, uint32_t nvar_stored, const Shape& sh, int exp_w,
+                             const double* slot_lam, void* rec, void* dr, double* partial, double* kept,
+                             hipStream_t s);
+// One lambda evaluation of the listed slots, either form: `rows` is dr
+// (fused) or B0; rec is used by the fused form only.
+hipError_t launch_evaluation(const Shape& sh, const void* factors, const void* divq, const WinMeta* meta,
+                             const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, int exp_w,
+                             const int32_t* inv, void* rec, void* rows, double* partial, const double* slot_lam,
+                             double* kept, hipStream_t s);
+// what the gather reads: fused (bs <= 4) the factors, the round records,
+// dr and the static rows with the bucket lists; otherwise the rows B0
+struct GatherSrc {
+  const void* B0 = nullptr;
+  const void* factors = nullptr;
+  const void* dr = nullptr;
+  const void* rec = nullptr;
+  const void* W0 = nullptr;
+  const void* V0 = nullptr;
+  const int32_t* list = nullptr;
+  uint32_t nvar_stored = 1;
+};
+// values per stored factor block of one (patch, variant): V^T (nw rows of
+// sr = nw rounded up to 4 values), W^T (nw rows of m values), the nw
+// singular values, rounded up to a multiple of 32 values, so that every
+// block starts a 128-byte line and no line is written by two matrices
+// (which run in different waves, often on different XCDs)
+uint32_t factor_stride(uint32_t nw, uint32_t m);
 // overlap-add gather: mode 0 risk sums into partial [na][gather_blocks][5]; mode 1
 // final reference frames into out (the output cube); mode 2 the whole
 // window of slot 0 into out
 uint32_t gather_blocks(const Shape& sh, int mode);
-hipError_t launch_gather(const void* B0, const int32_t* off, const double* invw, const double* u64,
+hipError_t launch_gather(const GatherSrc& src, const int32_t* off, const double* invw, const double* u64,
                          const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int mode, double* out,
                          double* partial, hipStream_t s);
 // sums [na][6] = (sum Uhat^2, sum U, sum U^2, sum Uhat U,
