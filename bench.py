@@ -1,6 +1,6 @@
 """Benchmark of the MI355X PGURE-SVT path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-windows C]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-windows C] [--batch-mix S]
 
 Workload: 100 frames of 256x256 fp32 (blob phantom, every blob drifting by
 +-2 px per frame, Poisson-Gaussian noise alpha=1, sigma=8), ARPS motion
@@ -9,12 +9,19 @@ trajectory 15 (16x15 Casorati matrices), no median pre-filter, lambda from
 the PGURE search per frame with noise parameters estimated.
 
 One step = one pgsvt_denoise_device call over the whole sequence with the
-input already resident in HBM and the output written to HBM.  Prints one JSON
-line.  The roofline object is for the kernel with the largest share of GPU
-time in the timed steps, measured with HIP events on the library's stream.
-The CPU baseline is the CPU FP64 oracle (a restatement of the reference CPU
-path using LAPACK dgesdd for the SVDs) on a sample of output frames, one
-window per host thread.
+input already resident in HBM and the output written to HBM (`value`).  The
+same call with host buffers (pgsvt_denoise: the H2D copy of the input and the
+D2H copy of the output included) is timed separately and reported as
+`host_buffers` (median of its steps).  Prints one JSON line.  The roofline
+object is for the kernel with the largest share of GPU time in the timed
+steps, measured with HIP events on the library's stream, with the
+algorithmic work per unit of the survey's measurement plan: 14 m n^2 + 8 n^3
+flops per m x n SVD (FP32 vector work), 40 bytes per voxel per lambda
+evaluation of a window.  The CPU baseline is the CPU FP64 oracle (a
+restatement of the reference CPU path using LAPACK dgesdd for the SVDs) on a
+sample of output frames, one window per host core.  `batch_mix` times the
+batched uneven workload (independent sequences of mixed shapes, lambda by a
+64-candidate PGURE grid) on its first S sequences.
 """
 
 import argparse
@@ -42,8 +49,21 @@ NVOX = N * N * NW
 
 P_SEL = (N - PATCH + 1) ** 2  # patches per window (step 1)
 
-# Algorithmic work per unit of each timed stage (units as the library counts
-# them in its profile):
+# Algorithmic work per unit used for the roofline (units as the library
+# counts them in its profile), from the survey's measurement plan:
+#   svd_jacobi          matrix  14 m n^2 + 8 n^3 flops (Golub-Van Loan thin SVD), FP32 vector work
+#   lambda evaluation   voxel   40 bytes: the four reconstructed volumes written and read back
+#                               (2 * 4 * 4) plus U and W read (8), per voxel of the window
+# (the library's lambda evaluation materialises none of the four volumes: the
+# bytes are the algorithm's, not what the kernels move).
+SURVEY_ALGO = {
+    "svd_jacobi": ("valu", 14 * M * NW**2 + 8 * NW**3),
+    "gather_risk": ("hbm", 40.0),
+    "search_eval": ("hbm", 40.0),
+}
+
+# The design's own per-unit data movement of each stage (every datum the
+# kernels read or write moved once), kept for the traffic comparison:
 #   svd_jacobi          matrix            thin SVD of one m x n Casorati matrix with V and U S:
 #                                         14 m n^2 + 8 n^3 flops (Golub-Van Loan count)
 #   divergence_factors  matrix            every datum moved once (fp32): all V^T and W^T rows
@@ -51,23 +71,28 @@ P_SEL = (N - PATCH + 1) ** 2  # patches per window (step 1)
 #                                         (4 n), frame positions (4 n / 3) and delta2/W of the
 #                                         window's voxels (4 NVOX / (3 P)) shared by the three
 #                                         variants of a patch
-#   eval_reconstruct    patch x variant   every datum moved once (fp32 factors and rows): the U
-#                                         reconstruction of every frame written (4 m n / 3: one
-#                                         of three variants), the singular values (4 n), bucket
-#                                         places (4 n / 3); plus, per kept rank (counted by the
-#                                         library over the three variants, which keep the same
-#                                         ranks but for a few patches), its divergence factor
-#                                         (4) and, for the U variant, its V^T and W^T rows
-#                                         (4 (n + m) / 3 per counted rank)
-#   gather_risk         voxel             every patch row of the U reconstruction read once (fp32),
-#                                         U (fp64) and 1/W (fp64) of the voxel read once
-EVAL_FIXED = 4.0 * M * NW / 3 + 4.0 * NW + 4.0 * NW / 3
-EVAL_PER_RANK = 4.0 * (NW + M) / 3 + 4.0
+#   eval_reconstruct    patch x variant   every datum moved once (fp32): the round record (d_0,
+#                                         kept ranks; 8 bytes) written at the patch's bucket place
+#                                         in every frame and those places read (12 n / 3 per
+#                                         variant: one record per patch), the first singular value
+#                                         of each variant and the one that ends the kept prefix
+#                                         (8); plus per kept rank its singular value and divergence
+#                                         factor (8) and, for the U variant's further ranks, d_r
+#                                         written (4 / 3 per counted rank)
+#   static_rows         bucket place      the patch index (4), the patch's w_0 row (4 m) and v_0[k]
+#                                         (4) read, the row and v_0[k] written in bucket order
+#                                         (4 m + 4), once per window
+#   gather_risk         voxel             every staged row read once (fp32): per (patch, frame)
+#                                         its static row (4 m) and v_0[k] (4), its round record (8)
+#                                         and patch index (4), P n / NVOX of them per voxel; U
+#                                         (fp64) and 1/W (fp64) of the voxel read once
+EVAL_FIXED = 12.0 * NW / 3 + 8.0
 ALGO = {
-    "svd_jacobi": ("mfma", 14 * M * NW**2 + 8 * NW**3),
+    "svd_jacobi": ("valu", 14 * M * NW**2 + 8 * NW**3),
     "divergence_factors": ("hbm", 4.0 * NW * (NW + M) + 4.0 * NW + 4.0 * NW / 3 + 4.0 * NVOX / (3 * P_SEL)),
     "eval_reconstruct": ("hbm", EVAL_FIXED),
-    "gather_risk": ("hbm", 4.0 * P_SEL * M * NW / NVOX + 16),
+    "static_rows": ("hbm", 8.0 * M + 12.0),
+    "gather_risk": ("hbm", (4.0 * M + 16.0) * P_SEL * NW / NVOX + 16),
 }
 
 
@@ -116,6 +141,25 @@ PARAMS = dict(trajectory_length=TRAJ, patch_size=PATCH, patch_overlap=1, motion_
               noise_sigma=-1.0, tol=1e-7)
 
 
+def run_host(noisy, steps, warmup):
+    """Per-step wall times of pgsvt_denoise with host buffers (input H2D and
+    output D2H inside the call)."""
+    from pguresvt import _pguresvt as P
+
+    L = P.load()
+    X = np.asfortranarray(noisy.astype(np.float32))
+    p = P.make_params(**PARAMS)
+    Y = np.zeros(X.shape, order="F")
+    est = np.zeros((FRAMES, 4), order="F")
+    times = []
+    for k in range(warmup + steps):
+        t0 = time.perf_counter()
+        P.check(L.pgsvt_denoise(X.ctypes.data, 2, N, N, FRAMES, C.byref(p), Y.ctypes.data, est.ctypes.data))
+        if k >= warmup:
+            times.append(time.perf_counter() - t0)
+    return times
+
+
 def run_gpu(noisy, steps, warmup, model=None):
     """Time `steps` calls of the device entry point (PGURE search, or the
     learned lambda when a model is given) after `warmup` calls."""
@@ -152,10 +196,14 @@ def run_gpu(noisy, steps, warmup, model=None):
     hip.sync()
     L.pgsvt_profile_enable(1)
     t0 = time.perf_counter()
+    step_times = []
     for _ in range(steps):
-        step()
+        ts = time.perf_counter()
+        step()  # returns after the output is complete on the device
+        step_times.append(time.perf_counter() - ts)
     hip.sync()
     dt = time.perf_counter() - t0
+    run_gpu.step_times = step_times
     n = L.pgsvt_profile_read(None, 0)
     buf = C.create_string_buffer(n)
     L.pgsvt_profile_read(buf, n)
@@ -199,20 +247,14 @@ def learned_eval(noisy, steps, warmup, dt_plain, est_plain):
             "kernels_ms_per_step": {k: v["ms"] / steps for k, v in prof.items()}}
 
 
-def roofline(prof):
-    """Roofline of the stage with the largest GPU time in the timed steps."""
-    known = {k: v for k, v in prof.items() if k in ALGO and v["launches"] > 0}
-    if not known:
-        return None, None
-    name = max(known, key=lambda k: known[k]["ms"])
-    e = known[name]
-    bound, per_unit = ALGO[name]
+def roofline_of(prof, name):
+    """Roofline of one profiled stage from the survey's algorithmic work per
+    unit and the stage's average launch time."""
+    e = prof[name]
+    bound, per_unit = SURVEY_ALGO[name]
     avg_s = e["ms"] / e["launches"] / 1e3
     units_per_launch = e["units"] / e["launches"]
     work = per_unit * units_per_launch
-    kept = prof.get("eval_kept_ranks")
-    if name == "eval_reconstruct" and kept:
-        work += EVAL_PER_RANK * kept["units"] / e["launches"]
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
@@ -226,23 +268,35 @@ def roofline(prof):
              "frac": achieved / PEAK_HBM_GBS, "traffic": traffic}
     else:
         achieved = work / avg_s / 1e12
-        r = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        r = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
              "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic}
     r["kernel"] = name
+    r["algorithmic_per_unit"] = per_unit
     r["avg_launch_ms"] = e["ms"] / e["launches"]
     r["units_per_launch"] = units_per_launch
     # Launch sizes vary from run to run (windows join the lambda search as
     # their host noise estimates finish); the time per unit does not.
     r["ps_per_unit"] = e["ms"] * 1e9 / e["units"]
-    return r, name
+    return r
+
+
+def roofline(prof):
+    """Roofline of the stage with the largest GPU time in the timed steps,
+    and of every stage with an algorithmic figure."""
+    known = {k: v for k, v in prof.items() if k in SURVEY_ALGO and v["launches"] > 0}
+    if not known:
+        return None, {}
+    name = max(known, key=lambda k: known[k]["ms"])
+    return roofline_of(prof, name), {k: roofline_of(prof, k) for k in known}
 
 
 # kernels of each profiled stage, as rocprofv3 names them
 STAGE_KERNELS = {
-    "svd_jacobi": "pgsvt::k_svd<",
-    "divergence_factors": "pgsvt::k_divq<",
-    "eval_reconstruct": "pgsvt::k_eval",
-    "gather_risk": "pgsvt::k_gather_",
+    "svd_jacobi": ("pgsvt::k_svd<",),
+    "divergence_factors": ("pgsvt::k_divq<",),
+    "eval_reconstruct": ("pgsvt::k_round_prep<", "pgsvt::k_eval<"),
+    "static_rows": ("pgsvt::k_static_rows<",),
+    "gather_risk": ("pgsvt::k_gather_",),
 }
 
 
@@ -292,10 +346,12 @@ def summarize_pmc(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_byt
     fetch = _pmc_rows(fetch_dir, "FETCH_SIZE")
     write = _pmc_rows(write_dir, "WRITE_SIZE")
     res = {"copy_probe_bytes": copy_bytes, "fetch_scale": f_fetch, "write_scale": f_write, "stages": {}}
-    for st, kname in STAGE_KERNELS.items():
-        fk = sum(r[2] for r in fetch if kname in r[0])
-        wk = sum(r[2] for r in write if kname in r[0])
-        nd = sum(1 for r in fetch if kname in r[0])
+    for st, knames in STAGE_KERNELS.items():
+        def of(r):
+            return any(k in r[0] for k in knames)
+        fk = sum(r[2] for r in fetch if of(r))
+        wk = sum(r[2] for r in write if of(r))
+        nd = sum(1 for r in fetch if of(r))
         if nd == 0 or st not in units:
             continue
         u = units[st] * steps
@@ -307,6 +363,29 @@ def summarize_pmc(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_byt
         json.dump(res, f, indent=1)
         f.write("\n")
     print(json.dumps(res))
+
+
+def host_cores():
+    """CPUs this process may run on, capped by a cgroup CPU quota (the
+    library's own host thread count)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if quota != "max" and float(period) > 0:
+            n = min(n, max(1, int(np.ceil(float(quota) / float(period)))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(noisy, windows):
@@ -332,9 +411,39 @@ def cpu_baseline(noisy, windows):
     dt = time.perf_counter() - st
     O.set_svd(0)
     return {"value": windows / dt, "unit": "frames/s", "cores": windows, "kind": "port",
+            "cpu_model": cpu_model(), "host_cores": host_cores(),
             "sample": "output frames %d-%d of the same sequence (%d full windows: noise estimate, ARPS, 4 SVD sets, "
-                      "SBPLX lambda search), one window per thread, SVD %s, %.1f s wall"
-                      % (t0, t0 + windows - 1, windows, svd, dt)}, est[t0:t0 + windows, 0]
+                      "SBPLX lambda search), one window per thread on %d of the host's %d cores, SVD %s, "
+                      "%.1f s wall" % (t0, t0 + windows - 1, windows, windows, host_cores(), svd, dt)}, \
+        est[t0:t0 + windows, 0]
+
+
+def batch_mix(n_seq, group=4, n_lambda=64):
+    """The batched uneven workload: the first n_seq sequences of the seed-3
+    draw (N x N x F, N in {128, 256, 512}, F in {30, 60, 200}, patch 4/6/8,
+    trajectory 7/15/31), 64 lambda candidates per window, through
+    pgsvt_denoise_batch with `group` sequences in flight.  The sequences are
+    made before the timed call; host buffers in and out (the copies, each
+    shape's graph capture and SVD-mapping tuning are inside)."""
+    import batch_bench as bb
+    from pguresvt import batch as B
+    jobs = bb.draw(n_seq)
+    items = [bb.make(j) for j in jobs]
+    t0 = time.perf_counter()
+    outs = B.denoise_batch(items, n_lambda=n_lambda, concurrency=group)
+    dt = time.perf_counter() - t0
+    frames = sum(X.shape[2] for X, _ in items)
+    voxels = sum(X.size for X, _ in items)
+    for (X, _), (Y, est) in zip(items, outs):
+        if not (np.isfinite(Y).all() and (est[:, 0] > 0).all()):
+            raise RuntimeError("batch mix: a sequence came back with non-finite frames or no lambda")
+    return {"value": frames / dt, "unit": "frames/s", "voxels_per_s": voxels / dt, "seconds": dt,
+            "sequences": n_seq, "frames": frames, "voxels": voxels, "lambda_candidates": n_lambda,
+            "in_flight": group,
+            "jobs": ["%dx%dx%d p%d T%d" % (j["n"], j["n"], j["frames"], j["patch"], j["trajectory"]) for j in jobs],
+            "note": "first %d sequences of the seed-3 draw of the batched uneven workload (32 in the full draw), "
+                    "lambda by the least PGURE risk over %d candidates per window, one captured HIP graph per "
+                    "batch of windows; host buffers, one timed pgsvt_denoise_batch call" % (n_seq, n_lambda)}
 
 
 def main():
@@ -343,9 +452,13 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-windows", type=int, default=None,
-                    help="output frames in the CPU-baseline sample, one thread each (default 2: about 30 s)")
+                    help="output frames in the CPU-baseline sample, one thread each (default: one per host core)")
+    ap.add_argument("--host-steps", type=int, default=5,
+                    help="timed calls with host buffers (H2D and D2H included), median reported")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-lambda comparison")
+    ap.add_argument("--batch-mix", type=int, default=8, metavar="S",
+                    help="sequences of the batched uneven workload timed for the batch_mix field (0: skip)")
     ap.add_argument("--copy-probe", type=int, metavar="MIB", default=None,
                     help="only run one MIB-MiB elementwise copy (counter calibration) and exit")
     ap.add_argument("--summarize-pmc", nargs=6, default=None,
@@ -365,12 +478,21 @@ def main():
         sys.exit(2)
     clean, noisy = workload()
     dt, prof, Y, est = run_gpu(noisy, a.steps, a.warmup)
+    step_times = run_gpu.step_times
     fps = FRAMES * a.steps / dt
-    roof, _ = roofline(prof)
+    roof, stages = roofline(prof)
+    host = None
+    if a.host_steps > 0:
+        ht = run_host(noisy, a.host_steps, 1)
+        host = {"value": FRAMES / float(np.median(ht)), "unit": "frames/s", "ms_median": 1e3 * float(np.median(ht)),
+                "ms_all": [1e3 * t for t in ht],
+                "note": "pgsvt_denoise with host buffers: the H2D copy of the input and the D2H copy of the output "
+                        "are inside each timed call; median of the calls after one warm-up call"}
     learned = None if a.no_learned else learned_eval(noisy, a.steps, a.warmup, dt / a.steps, est)
+    mix = batch_mix(a.batch_mix) if a.batch_mix > 0 else None
     cpu = None
     if not a.no_cpu_baseline:
-        cw = a.cpu_windows or max(1, min(2, os.cpu_count() or 1))
+        cw = a.cpu_windows or host_cores()
         cpu, _ = cpu_baseline(noisy, cw)
     out = {
         "metric": METRIC,
@@ -380,6 +502,7 @@ def main():
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": 1e3 * dt / a.steps,
+        "ms_per_step_median": 1e3 * float(np.median(step_times)),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -390,8 +513,11 @@ def main():
                    "frames": FRAMES, "frame": [N, N], "patch": PATCH, "trajectory": TRAJ, "casorati": [M, NW],
                    "patches_per_window": (N - PATCH + 1) ** 2},
         "roofline": roof,
+        "roofline_stages": stages,
+        "host_buffers": host,
         "cpu_baseline": cpu,
         "learned_lambda": learned,
+        "batch_mix": mix,
         "kernels_ms_per_step": {k: v["ms"] / a.steps for k, v in prof.items()},
         "stage_units_per_step": {k: v["units"] / a.steps for k, v in prof.items()},
         "lambda_median": float(np.median(est[:, 0])),
