@@ -186,8 +186,15 @@ int pgsvt_learned_features(const void* X, int dtype, uint32_t n_rows,
 
 /* pgsvt_denoise with each window's lambda proposed by the model instead of
  * searched (optimize_pgure and lambda1 are ignored; the proposal is clamped
- * to the search's bounds [0, max(100, start)]).  estimates holds the
- * proposed lambda and the noise features' (alpha, mu, sigma). */
+ * to the search's bounds [0, max(100, start)]).  Each proposal is probed
+ * (pgsvt_set_learned_fallback): the window's PGURE risk at lambda / (1 +
+ * delta), lambda and lambda (1 + delta), from the perturbed decompositions
+ * of every 8th patch and the reconstruction sums of 5 of the window's
+ * frames; where the parabola through the three predicts a relative risk
+ * gap above the limit, the window is searched as by pgsvt_denoise (its
+ * full decompositions, SBPLX from the reference's start point) and its
+ * frame replaced.  estimates holds the final lambda and the noise
+ * features' (alpha, mu, sigma). */
 int pgsvt_denoise_learned(const void* X, int dtype, uint32_t n_rows,
                           uint32_t n_cols, uint32_t n_frames,
                           const pgsvt_params* p,
@@ -199,6 +206,15 @@ int pgsvt_denoise_learned_device(const void* X_dev, int dtype, uint32_t n_rows,
                                  const pgsvt_params* p,
                                  const pgsvt_lambda_model* model,
                                  double* Y_dev, double* estimates);
+
+/* Probe of pgsvt_denoise_learned: step delta in (0, 1) (default 0.1) and
+ * the predicted relative risk gap above which a window falls back to the
+ * search (default 0.01; gap <= 0 turns the probe and the fallback off).
+ * Process-wide.  No reference counterpart. */
+int pgsvt_set_learned_fallback(double delta, double gap);
+/* Windows that fell back to the search in the calling thread's last
+ * pgsvt_denoise_learned(_device) call. */
+int pgsvt_learned_fallbacks(void);
 
 /* The network alone, for training and its tests: forward of n rows of X
  * ([n][n_in], host) into out [n]; gradient of mean((y - t)^2) into grad
@@ -234,8 +250,9 @@ int pgsvt_set_precision(int bits);
 int pgsvt_set_svd_mapping(int mode);
 
 /* Device memory is cached per device between calls (buffers of one call
- * are reused by the next call of the same shape).  pgsvt_release_cache
- * waits for the current device and frees its cache.  No reference
+ * are reused by the next call of the same shape), and pinned host staging
+ * memory process-wide.  pgsvt_release_cache waits for the current device
+ * and frees its cache and the unused pinned blocks.  No reference
  * counterpart (the reference allocates per window on the host). */
 int pgsvt_release_cache(void);
 

@@ -30,7 +30,8 @@ constexpr int kVarU = 0, kVarP = 1, kVarM = 2, kMaxVariants = 3;
 struct WinMeta {
   uint32_t t;      // output frame index
   uint32_t s0;     // first input frame of the window
-  uint32_t ref;    // reference (output) frame inside the window
+  uint32_t ref;    // output frame inside the window (src/pguresvt.hpp:155-166)
+  uint32_t aref;   // ARPS reference frame inside the window (src/arps.hpp:55-118)
   uint32_t out;    // slot of the output frame in the caller's Y
   double uMax;     // max of the raw window (src/pguresvt.hpp:116)
   double wMax;     // max of the motion-estimation window (:117)

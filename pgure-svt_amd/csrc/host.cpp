@@ -57,6 +57,18 @@ std::atomic<int> g_precision{32};
 // by the grid path
 std::atomic<int> g_svd_mapping{0};
 
+// Learned-lambda fallback (pgsvt_set_learned_fallback): the probe's step
+// delta (risks at lambda / (1 + delta), lambda, lambda (1 + delta)) and the
+// predicted relative risk gap above which a window is searched; gap <= 0
+// turns the probe off.  Windows searched by the last learned call of this
+// thread: pgsvt_learned_fallbacks.
+std::mutex g_fallback_mu;
+double g_probe_delta = 0.1, g_probe_gap = 0.01;
+thread_local int g_last_fallbacks = 0;
+// probe sample: every kProbeStride-th selected patch in all three variants,
+// the data terms over kProbeFrames frames spread over the window
+constexpr uint32_t kProbeStride = 8, kProbeFrames = 5;
+
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -312,6 +324,50 @@ struct DBuf {
 
 // Pinned host staging for the small per-round copies (window metadata and
 // list up, risk sums down), which then need no staging copy by the runtime.
+// Pinned host blocks, kept for reuse across calls like the device blocks:
+// allocating pinned memory costs far more than the copies it serves (the
+// per-round staging of a call, the host copy of a device-resident input).
+// A block of at least the asked size and at most twice it is reused; take()
+// reports the block's size through `bytes`.
+class PinnedCache {
+ public:
+  static PinnedCache& get() {
+    static PinnedCache* c = new PinnedCache;  // never destroyed: outlives every caller
+    return *c;
+  }
+  void* take(size_t& bytes) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      auto it = free_.lower_bound(bytes);
+      if (it != free_.end() && it->first <= 2 * bytes) {
+        void* p = it->second;
+        bytes = it->first;
+        free_.erase(it);
+        return p;
+      }
+    }
+    void* p = nullptr;
+    hipchk(hipHostMalloc(&p, bytes, hipHostMallocDefault), "hipHostMalloc");
+    return p;
+  }
+  void give(void* p, size_t bytes) {
+    std::lock_guard<std::mutex> g(mu_);
+    free_.emplace(bytes, p);
+  }
+  void release() {
+    std::multimap<size_t, void*> fl;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      fl.swap(free_);
+    }
+    for (auto& kv : fl) (void)hipHostFree(kv.second);
+  }
+
+ private:
+  std::mutex mu_;
+  std::multimap<size_t, void*> free_;
+};
+
 template <class T>
 struct HBuf {
   T* p = nullptr;
@@ -320,15 +376,16 @@ struct HBuf {
   HBuf(const HBuf&) = delete;
   HBuf& operator=(const HBuf&) = delete;
   ~HBuf() {
-    if (p) (void)hipHostFree(p);
+    if (p) PinnedCache::get().give(p, n * sizeof(T));
   }
   void alloc(size_t count) {
     if (count <= n) return;
-    if (p) (void)hipHostFree(p);
+    if (p) PinnedCache::get().give(p, n * sizeof(T));
     p = nullptr;
     n = 0;
-    hipchk(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
-    n = count;
+    size_t bytes = count * sizeof(T);
+    p = (T*)PinnedCache::get().take(bytes);
+    n = bytes / sizeof(T);
   }
 };
 
@@ -387,16 +444,23 @@ std::vector<uint32_t> patch_indices(uint32_t N, uint32_t bs, uint32_t ov) {
 }
 
 // Window placement (src/pguresvt.hpp:100-114, 155-166).
-void window_of(uint32_t t, uint32_t nf, const Geometry& g, uint32_t& s0, uint32_t& ref) {
+// Window of output frame t: first input frame s0 (src/pguresvt.hpp:100-
+// 114), output frame ref (:155-166, with the driver's Nt) and ARPS reference
+// frame aref (src/arps.hpp:55-118, with the window's frame count Nw).  Nt
+// and Nw differ when the clamped trajectory length Nt is even (Nw = Nt + 1):
+// then, near the end of the sequence, the output frame is the one after the
+// ARPS reference frame.
+void window_of(uint32_t t, uint32_t nf, const Geometry& g, uint32_t& s0, uint32_t& ref, uint32_t& aref) {
   if (t < g.fw) {
     s0 = 0;
-    ref = t;
+    ref = aref = t;
   } else if (t >= nf - g.fw) {
     s0 = nf - 2 * g.fw - 1;
     ref = t - (nf - g.Nt);
+    aref = t - (nf - g.Nw);
   } else {
     s0 = t - g.fw;
-    ref = g.fw;
+    ref = aref = g.fw;
   }
 }
 
@@ -407,7 +471,9 @@ struct Pass {
 };
 
 bool arps_jobs(uint32_t t, uint32_t nf, const Geometry& g, std::vector<int16_t>& jobs) {
-  const uint32_t Nt = g.Nt, tw = g.fw;
+  // the motion estimator's Nt is its window's frame count (A.n_slices,
+  // src/arps.hpp:35), not the driver's clamped trajectory length
+  const uint32_t Nt = g.Nw, tw = g.fw;
   std::vector<Pass> ps;
   auto add = [&](int, int a, int b, int c) { ps.push_back({a, b, c}); };
   if (t < tw) {
@@ -585,7 +651,7 @@ struct Pipeline {
   DBuf<char> raw;
   DBuf<double> Xd, Zd, Yd;
   double* Yout = nullptr;           // final frames (Yd.p or a caller device buffer)
-  std::vector<char> host_copy;      // host copy of a device-resident input
+  HBuf<char> host_copy;             // host copy of a device-resident input (pinned)
   std::unique_ptr<Profiler> prof;
   DBuf<uint8_t> d2;
   DBuf<uint32_t> sel_d;
@@ -608,6 +674,13 @@ struct Pipeline {
   DBuf<char> w0b, v0b;  // static rows of the fused evaluation: w_0 and v_0[k] per bucket place
   DBuf<int16_t> jobs_d;
   uint32_t max_jobs = 0, gather_blocks = 0;
+  // learned-lambda probe: the patch subsample (sh_s: p_sel = its size) with
+  // its three-variant factors and divergence factors, per-slot lambdas
+  bool probe_on = false;
+  Shape sh_s;
+  DBuf<uint32_t> sel_s, wl_s;
+  DBuf<char> factors_s, divq_s;
+  DBuf<double> partial_s, lam_s;
   DBuf<uint32_t> stats;
 
   ~Pipeline() {
@@ -656,9 +729,9 @@ struct Pipeline {
     const void* src = X;
     if (on_device) {
       if (optimize || host_noise) {  // the noise estimator runs on the host
-        host_copy.resize(N * dtype_size(dtype));
-        hipchk(hipMemcpyAsync(host_copy.data(), X, host_copy.size(), hipMemcpyDeviceToHost, s), "D2H X");
-        X = host_copy.data();
+        host_copy.alloc(N * dtype_size(dtype));
+        hipchk(hipMemcpyAsync(host_copy.p, X, N * dtype_size(dtype), hipMemcpyDeviceToHost, s), "D2H X");
+        X = host_copy.p;
       }
     } else {
       raw.alloc(N * dtype_size(dtype));
@@ -695,7 +768,7 @@ struct Pipeline {
     WinMeta m{};
     m.t = t;
     m.out = t - tb;
-    window_of(t, nf, g, m.s0, m.ref);
+    window_of(t, nf, g, m.s0, m.ref, m.aref);
     double um = fmaxX[m.s0], wm = median ? fmaxZ[m.s0] : 0;
     for (uint32_t k = 1; k < g.Nw; ++k) {
       um = std::max(um, fmaxX[m.s0 + k]);
@@ -724,6 +797,7 @@ struct Pipeline {
     }
     b += es * nvar * (size_t)sh.p_sel * sh.fstride;      // factors
     if (optimize) b += es * nvar * (size_t)sh.p_sel * 2 * ((sh.nw + 3) & ~3u);  // divq
+    if (probe_on) b += es * 3 * (size_t)sh_s.p_sel * (sh.fstride + 2 * ((sh.nw + 3) & ~3u));  // probe factors
     return b;
   }
 
@@ -791,6 +865,13 @@ struct Pipeline {
     kept_d.alloc(NB);
     max_jobs = 2 * g.Nw;
     jobs_d.alloc((size_t)NB * max_jobs * 4);
+    if (probe_on) {
+      factors_s.alloc((size_t)NB * 3 * sh_s.p_sel * sh.fstride * es);
+      divq_s.alloc((size_t)NB * 3 * sh_s.p_sel * 2 * ((sh.nw + 3) & ~3u) * es);
+      partial_s.alloc((size_t)NB * eval_blocks(sh_s, 3));
+      lam_s.alloc(NB);
+      wl_s.alloc(NB);
+    }
     stats.alloc(1);
     hipchk(launch_fill32(stats.p, stats.n, 0, s), "clear stats");
   }
@@ -834,8 +915,8 @@ struct Pipeline {
     // units: matrices decomposed
     prof->run("svd_jacobi", (double)nb * nvar_run * sh.p_sel, [&] {
       // rotation threshold |c| > tol sqrt(ab): 1e-6 in FP32, 1e-12 in FP64
-      hipchk(launch_svd(uvar.p, pos.p, sel_d.p, nb, nvar_run, sh, factors.p, 30, sh.fp64 ? 1e-12 : 1e-6, stats.p,
-                        s, svd_mapping),
+      hipchk(launch_svd(uvar.p, pos.p, sel_d.p, nb, nvar_run, nvar, sh, factors.p, 30, sh.fp64 ? 1e-12 : 1e-6,
+                        stats.p, s, svd_mapping),
              "svd");
     });
     if (optimize && nvar_run == 3) {
@@ -873,6 +954,69 @@ struct Pipeline {
     hipchk(launch_evaluation(sh, factors.p, divq.p, meta_d.p, wl, na, nvar, nvar_run, p.exponential_weighting, inv.p,
                              nkb.p, B0.p, part, slot_lam, kept, s),
            "eval");
+  }
+
+  // the learned-lambda probe's patch subsample (before alloc_batch)
+  void setup_probe() {
+    std::vector<uint32_t> sub;
+    for (uint32_t q = 0; q < sh.p_sel; q += kProbeStride) sub.push_back(sel[q]);
+    sh_s = sh;
+    sh_s.p_sel = (uint32_t)sub.size();
+    sel_s.alloc(sub.size());
+    hipchk(hipMemcpyAsync(sel_s.p, sub.data(), sub.size() * 4, hipMemcpyHostToDevice, s), "H2D probe sample");
+    hipchk(hipStreamSynchronize(s), "probe sample sync");  // sub is local
+    probe_on = true;
+  }
+
+  // the subsample's three variants decomposed and their divergence factors
+  // (after decompose_kernels of the same batch)
+  void decompose_probe(uint32_t nb) {
+    prof->run("probe_decompose", (double)nb * 3 * sh_s.p_sel, [&] {
+      hipchk(launch_svd(uvar.p, pos.p, sel_s.p, nb, 3, 3, sh_s, factors_s.p, 30, 1e-6, stats.p, s,
+                        g_svd_mapping.load() == 1 ? 1 : 0),
+             "probe svd");
+      hipchk(launch_divq(factors_s.p, pos.p, sel_s.p, dw.p, nb, 3, sh_s, divq_s.p, s), "probe divergence factors");
+    });
+  }
+
+  // PGURE risks of the batch windows ms at three lambdas each, lam [nb][3],
+  // into risk [nb][3]: the data sums over kProbeFrames frames of the window
+  // (all patches reconstructed) and the divergence sum over the patch
+  // subsample, each scaled to the whole window.  The same sample for every
+  // lambda of a window, so the three risks differ by the lambda only.
+  void probe_risks(const std::vector<WinMeta>& ms, const std::vector<WindowPrep>& prep,
+                   const std::vector<double>& lam, std::vector<double>& risk) {
+    const uint32_t nb = (uint32_t)ms.size(), nslot = 3 * nb;
+    const uint32_t nk = gather_frames(sh, kProbeFrames);
+    const uint32_t gblk = pgsvt::gather_blocks(sh, 0, nk), eblk = eval_blocks(sh_s, 3);
+    const double fk = (double)sh.nw / nk, fp = (double)sh.p_sel / sh_s.p_sel;
+    hipchk(hipMemcpyAsync(meta_d.p, ms.data(), nb * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
+    risk.assign(nslot, 0.0);
+    std::vector<uint32_t> wl(NB);
+    for (uint32_t c0 = 0; c0 < nslot; c0 += NB) {
+      const uint32_t na = std::min(NB, nslot - c0);
+      for (uint32_t i = 0; i < na; ++i) wl[i] = (c0 + i) / 3;
+      hipchk(hipMemcpyAsync(wl_s.p, wl.data(), na * 4, hipMemcpyHostToDevice, s), "H2D probe list");
+      hipchk(hipMemcpyAsync(lam_s.p, lam.data() + c0, na * 8, hipMemcpyHostToDevice, s), "H2D probe lambdas");
+      prof->run("probe_eval", (double)na * sh.vol, [&] {
+        evaluate(wl_s.p, na, 1, lam_s.p, nullptr, nullptr);
+        hipchk(launch_gather(gather_src(), off.p, invw.p, u64.p, meta_d.p, wl_s.p, na, sh, 0, nullptr, partial_b.p, s,
+                             nk),
+               "probe gather");
+        hipchk(launch_round_prep(factors_s.p, divq_s.p, inv.p, meta_d.p, wl_s.p, na, 3, sh_s, p.exponential_weighting,
+                                 lam_s.p, nullptr, nullptr, partial_s.p, nullptr, s),
+               "probe divergence");
+        hipchk(launch_risk_reduce(partial_s.p, eblk, partial_b.p, gblk, na, sums.p, s), "probe reduce");
+      });
+      hipchk(hipMemcpyAsync(sums_h.p, sums.p, (size_t)na * 6 * 8, hipMemcpyDeviceToHost, s), "D2H probe sums");
+      hipchk(hipStreamSynchronize(s), "probe sync");  // wl is reused by the next chunk
+      for (uint32_t i = 0; i < na; ++i) {
+        double S[6];
+        for (int q = 0; q < 6; ++q) S[q] = sums_h.p[(size_t)i * 6 + q] * (q == 4 ? fp : fk);
+        risk[c0 + i] = this->risk(S, prep[(c0 + i) / 3]);
+      }
+    }
+    prof->flush();
   }
 
   void decompose(std::vector<WinMeta>& ms, uint32_t nvar_run) {
@@ -972,6 +1116,115 @@ struct Pipeline {
     for (size_t k = 0; k < sh.fs; ++k) out[k] = elem(X, dtype, base + k) / scale;
   }
 };
+
+// A14: lockstep SBPLX over the batch slots of ms (slot b: ms[b]).  fill(b)
+// makes prep[b] (noise estimates, start point) available, waiting if needed;
+// ready(b) says whether it would not wait.  A window's search starts once
+// its prep is there; every round evaluates the risk of the windows that
+// asked for a lambda together.  On return ms[b].lambda is each window's
+// last evaluated lambda (Q4).
+void lockstep_search(Pipeline& P, std::vector<WinMeta>& ms, std::vector<WindowPrep>& prep,
+                     const std::function<bool(uint32_t)>& ready, const std::function<void(uint32_t)>& fill,
+                     int& status, std::string& warn) {
+  const pgsvt_params& p = P.p;
+  const uint32_t nb = (uint32_t)ms.size();
+  std::vector<std::unique_ptr<Sbplx1D>> opt(nb);
+  std::vector<int> code(nb, Sbplx1D::kNeed);
+  std::vector<char> started(nb, 0);
+  uint32_t n_started = 0;
+  auto admit = [&](uint32_t b) {
+    fill(b);
+    const double start = prep[b].start;
+    const double ub = std::max(100.0, start);
+    opt[b].reset(new Sbplx1D(start, 0.0, ub, std::sqrt(start), p.tol, 1e-12, (int)p.max_iter));
+    started[b] = 1;
+    n_started++;
+  };
+  HostTimer tr("host_search_rounds");
+  std::vector<double> sums;
+  // Risks already evaluated, per window.  The search asks for the very
+  // same lambda again (a reflection clamped at the bound, a contraction
+  // onto a vertex): about a quarter of its requests.  The risk of a
+  // window is a deterministic function of lambda (fixed-order sums), so
+  // a repeat is answered from here with the identical value.
+  std::vector<std::vector<std::pair<double, double>>> seen(nb);
+  auto recall = [&](uint32_t b, double l, double& r) {
+    for (const auto& e : seen[b])
+      if (e.first == l) {
+        r = e.second;
+        return true;
+      }
+    return false;
+  };
+  double repeats = 0;
+  for (;;) {
+    for (uint32_t b = 0; b < nb; ++b)
+      if (!started[b] && ready(b)) admit(b);
+    std::vector<uint32_t> list;
+    for (uint32_t b = 0; b < nb; ++b) {
+      if (!started[b] || code[b] != Sbplx1D::kNeed) continue;
+      int rc;
+      double r;
+      while ((rc = opt[b]->run()) == Sbplx1D::kNeed && recall(b, opt[b]->pending(), r)) {
+        opt[b]->supply(r);
+        repeats += 1;
+      }
+      if (rc == Sbplx1D::kNeed) {
+        ms[b].lambda = opt[b]->pending();
+        list.push_back(b);
+      } else {
+        code[b] = rc;
+      }
+    }
+    if (list.empty()) {
+      if (n_started == nb) break;
+      HostTimer tw("host_noise_wait");
+      for (uint32_t b = 0; b < nb; ++b)
+        if (!started[b]) {
+          admit(b);
+          break;
+        }
+      continue;
+    }
+    P.eval_risk(ms, list, sums);
+    for (size_t k = 0; k < list.size(); ++k) {
+      const uint32_t b = list[k];
+      const double r = P.risk(&sums[k * 6], prep[b]);
+      seen[b].emplace_back(ms[b].lambda, r);
+      opt[b]->supply(r);
+    }
+  }
+  prof_host("search_repeats", 0.0, repeats);
+  for (uint32_t b = 0; b < nb; ++b) {
+    ms[b].lambda = opt[b]->last_x();
+    if (code[b] < 0) {
+      status = PGSVT_E_NUMERIC;
+      warn = "lambda optimisation failed (NLopt-equivalent error code " + std::to_string(code[b]) + ")";
+    } else if (code[b] == Sbplx1D::kMaxEval) {
+      std::fprintf(stderr,
+                   "WARNING: optimization terminated after max_eval (%u) was reached.\n"
+                   "Consider increasing the max_eval or increasing the convergence tolerance (tol=%g).\n",
+                   p.max_iter, p.tol);
+    }
+  }
+}
+
+// Predicted relative risk gap of the middle of three lambdas from their
+// risks: the parabola through (log lambda, risk) has its minimum
+// slope^2 / (4 curvature) below the middle risk.  Not convex, or points
+// that coincide (lambda at a bound): the best observed improvement, or
+// infinity when a neighbour is lower on a concave curve.
+double probe_gap(const double* lam, const double* r) {
+  const double xm = std::log(lam[0]), x0 = std::log(lam[1]), xp = std::log(lam[2]);
+  const double lo = std::min(r[0], r[2]), scale = std::fabs(r[1]);
+  if (!std::isfinite(xm) || !std::isfinite(xp) || !(scale > 0)) return INFINITY;
+  if (!(xp > x0) || !(x0 > xm)) return lo < r[1] ? (r[1] - lo) / scale : 0.0;
+  const double d1 = (r[1] - r[0]) / (x0 - xm), d2 = (r[2] - r[1]) / (xp - x0);
+  const double a = (d2 - d1) / (xp - xm);  // q(x) = r0 + d1 (x - xm) + a (x - xm)(x - x0)
+  const double b = d1 + a * (x0 - xm);     // q'(x0)
+  if (!(a > 0)) return lo < r[1] ? INFINITY : 0.0;
+  return b * b / (4 * a) / scale;
+}
 
 // Host threads for n_jobs (-1 = all): the CPUs this process may run on,
 // capped by a cgroup CPU quota when one is set.
@@ -1142,7 +1395,18 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
   const bool learned = rs.mode == 1 || rs.mode == 2;
   const uint32_t count = rs.te - rs.tb;
   pgsvt_params pl = *pp;
-  if (learned) pl.optimize_pgure = 0;  // no perturbed decompositions, no search
+  // learned lambda: the U decompositions only, no search, unless the probe
+  // is on (then every variant volume is prepared: the probe decomposes a
+  // patch subsample in all three, and windows that fail it are searched)
+  double probe_delta = 0, probe_gap_max = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_fallback_mu);
+    probe_delta = g_probe_delta;
+    probe_gap_max = g_probe_gap;
+  }
+  const bool probe = rs.mode == 1 && probe_gap_max > 0;
+  if (learned) pl.optimize_pgure = probe ? 1 : 0;
+  g_last_fallbacks = 0;
   if (rs.mode == 3) pl.optimize_pgure = 1;  // the grid needs the risk's perturbed decompositions
   if (rs.mode == 3 && (rs.grid < 1 || rs.grid > 1024)) return fail(PGSVT_E_ARG, "lambda grid size must be 1..1024");
   Pipeline P;
@@ -1151,6 +1415,7 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
   {
     HostTimer t("host_setup");
     P.setup(X, dtype, nr, nc, nf, pl, on_device);
+    if (probe) P.setup_probe();
   }
   const pgsvt_params& p = P.p;
   const Geometry& g = P.g;
@@ -1354,7 +1619,8 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
             hipchk(hipEventCreate(&e0), "event");
             hipchk(hipEventCreate(&e1), "event");
             hipchk(hipEventRecord(e0, P.s), "event record");
-            hipchk(launch_svd(P.uvar.p, P.pos.p, P.sel_d.p, nb, P.nvar, P.sh, P.factors.p, 30, 1e-6, P.stats.p, P.s,
+            hipchk(launch_svd(P.uvar.p, P.pos.p, P.sel_d.p, nb, P.nvar, P.nvar, P.sh, P.factors.p, 30, 1e-6, P.stats.p,
+                              P.s,
                               m),
                    "svd autotune");
             hipchk(hipEventRecord(e1, P.s), "event record");
@@ -1419,8 +1685,10 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
       }
       continue;
     }
-    P.decompose(ms, P.nvar);
+    P.decompose(ms, learned ? 1 : P.nvar);
+    if (probe) P.decompose_probe(nb);
     std::vector<WindowPrep> prep(nb);
+    std::vector<uint32_t> fallback;  // learned lambda: windows the probe sends to the search
     if (learned) {
       // log singular values of the sampled patches (standardised for the
       // network), then the noise features of each window as they arrive
@@ -1463,94 +1731,37 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
           ms[b].lambda = std::min(std::max(l, 0.0), std::max(100.0, prep[b].start));  // the search's bounds
         }
         P.prof->flush();
+        if (probe) {
+          // local optimality of each proposal: risks at lambda / f, lambda,
+          // lambda f (clamped to the search's bounds); a predicted gap above
+          // the limit sends the window to the search
+          const double f = 1.0 + probe_delta;
+          std::vector<double> lam((size_t)nb * 3), risk;
+          for (uint32_t b = 0; b < nb; ++b) {
+            const double l = ms[b].lambda, ub = std::max(100.0, prep[b].start);
+            lam[3 * b] = l / f;
+            lam[3 * b + 1] = l;
+            lam[3 * b + 2] = std::min(l * f, ub);
+          }
+          P.probe_risks(ms, prep, lam, risk);
+          for (uint32_t b = 0; b < nb; ++b)
+            if (!(probe_gap(&lam[3 * b], &risk[3 * b]) <= probe_gap_max)) fallback.push_back(b);
+        }
       }
     } else if (P.optimize) {
-      std::vector<std::unique_ptr<Sbplx1D>> opt(nb);
-      std::vector<int> code(nb, Sbplx1D::kNeed);
       {
         HostTimer t("host_decompose_sync");
         hipchk(hipStreamSynchronize(P.s), "decompose sync");
       }
       // A window joins the lockstep search as soon as its noise estimate is
       // ready; the rounds run while the host threads finish the others.
-      std::vector<char> started(nb, 0);
-      uint32_t n_started = 0;
-      auto admit = [&](uint32_t b) {
-        prep[b] = pool->wait(t0 + b - rs.tb);
-        prep[b].check();
-        const double start = prep[b].start;
-        const double ub = std::max(100.0, start);
-        opt[b].reset(new Sbplx1D(start, 0.0, ub, std::sqrt(start), p.tol, 1e-12, (int)p.max_iter));
-        started[b] = 1;
-        n_started++;
-      };
-      HostTimer tr("host_search_rounds");
-      std::vector<double> sums;
-      // Risks already evaluated, per window.  The search asks for the very
-      // same lambda again (a reflection clamped at the bound, a contraction
-      // onto a vertex): about a quarter of its requests.  The risk of a
-      // window is a deterministic function of lambda (fixed-order sums), so
-      // a repeat is answered from here with the identical value.
-      std::vector<std::vector<std::pair<double, double>>> seen(nb);
-      auto recall = [&](uint32_t b, double l, double& r) {
-        for (const auto& e : seen[b])
-          if (e.first == l) {
-            r = e.second;
-            return true;
-          }
-        return false;
-      };
-      double repeats = 0;
-      for (;;) {
-        for (uint32_t b = 0; b < nb; ++b)
-          if (!started[b] && pool->ready(t0 + b - rs.tb)) admit(b);
-        std::vector<uint32_t> list;
-        for (uint32_t b = 0; b < nb; ++b) {
-          if (!started[b] || code[b] != Sbplx1D::kNeed) continue;
-          int rc;
-          double r;
-          while ((rc = opt[b]->run()) == Sbplx1D::kNeed && recall(b, opt[b]->pending(), r)) {
-            opt[b]->supply(r);
-            repeats += 1;
-          }
-          if (rc == Sbplx1D::kNeed) {
-            ms[b].lambda = opt[b]->pending();
-            list.push_back(b);
-          } else {
-            code[b] = rc;
-          }
-        }
-        if (list.empty()) {
-          if (n_started == nb) break;
-          HostTimer tw("host_noise_wait");
-          for (uint32_t b = 0; b < nb; ++b)
-            if (!started[b]) {
-              admit(b);
-              break;
-            }
-          continue;
-        }
-        P.eval_risk(ms, list, sums);
-        for (size_t k = 0; k < list.size(); ++k) {
-          const uint32_t b = list[k];
-          const double r = P.risk(&sums[k * 6], prep[b]);
-          seen[b].emplace_back(ms[b].lambda, r);
-          opt[b]->supply(r);
-        }
-      }
-      prof_host("search_repeats", 0.0, repeats);
-      for (uint32_t b = 0; b < nb; ++b) {
-        ms[b].lambda = opt[b]->last_x();
-        if (code[b] < 0) {
-          status = PGSVT_E_NUMERIC;
-          warn = "lambda optimisation failed (NLopt-equivalent error code " + std::to_string(code[b]) + ")";
-        } else if (code[b] == Sbplx1D::kMaxEval) {
-          std::fprintf(stderr,
-                       "WARNING: optimization terminated after max_eval (%u) was reached.\n"
-                       "Consider increasing the max_eval or increasing the convergence tolerance (tol=%g).\n",
-                       p.max_iter, p.tol);
-        }
-      }
+      lockstep_search(
+          P, ms, prep, [&](uint32_t b) { return pool->ready(t0 + b - rs.tb); },
+          [&](uint32_t b) {
+            prep[b] = pool->wait(t0 + b - rs.tb);
+            prep[b].check();
+          },
+          status, warn);
     } else {
       for (uint32_t b = 0; b < nb; ++b) {
         ms[b].lambda = lambda0;
@@ -1560,6 +1771,24 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
       }
     }
     if (rs.mode != 2) P.finish(ms);
+    if (!fallback.empty()) {
+      // the windows that failed the probe: full decompositions and the
+      // reference search, then their frames again
+      HostTimer tf("learned_fallback_search");
+      std::vector<WinMeta> mf;
+      std::vector<WindowPrep> pf;
+      for (uint32_t b : fallback) {
+        mf.push_back(ms[b]);
+        pf.push_back(prep[b]);
+      }
+      P.decompose(mf, 3);
+      hipchk(hipStreamSynchronize(P.s), "fallback decompose sync");
+      lockstep_search(P, mf, pf, [](uint32_t) { return true; }, [](uint32_t) {}, status, warn);
+      P.finish(mf);
+      for (size_t i = 0; i < fallback.size(); ++i) ms[fallback[i]].lambda = mf[i].lambda;
+      g_last_fallbacks += (int)fallback.size();
+      prof_host("learned_fallbacks", 0.0, (double)fallback.size());
+    }
     for (uint32_t b = 0; b < nb; ++b) {
       const uint32_t i = t0 + b - rs.tb;
       est[i + (size_t)count * 0] = ms[b].lambda;
@@ -1758,7 +1987,7 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     m.t = t;
     m.out = t;
     uint32_t s0 = 0;
-    window_of(t, nf, g, s0, m.ref);
+    window_of(t, nf, g, s0, m.ref, m.aref);
     m.s0 = 0;  // the window's frames are staged at the start of Xw
     hipchk(launch_to_double((const char*)X + (size_t)s0 * sh.fs * dsz, dtype, L.Xw, vol, s), "to_double");
     hipchk(launch_frame_max(L.Xw, sh.fs, g.Nw, L.fmax, s), "frame max");
@@ -1789,7 +2018,7 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     hipchk(launch_buckets(L.pos, L.sel, 1, sh, L.cnt, L.off, L.list, L.inv, s), "buckets");
     hipchk(launch_weights(L.off, L.d2, 1, sh, d2neg, d2pos, L.invw, L.dw, s), "weights");
     hipchk(launch_fill32(L.stats, 1, 0, s), "clear stats");
-    hipchk(launch_svd(L.uvar, L.pos, L.sel, 1, nvar, sh, L.factors, 30, 1e-6, L.stats, s,
+    hipchk(launch_svd(L.uvar, L.pos, L.sel, 1, nvar, nvar, sh, L.factors, 30, 1e-6, L.stats, s,
                       g_svd_mapping.load() == 1 ? 1 : 0),
            "svd");
     if (optimize) hipchk(launch_divq(L.factors, L.pos, L.sel, L.dw, 1, nvar, sh, L.divq, s), "divergence factors");
@@ -1918,6 +2147,17 @@ int pgsvt_set_precision(int bits) {
   return 0;
 }
 
+int pgsvt_set_learned_fallback(double delta, double gap) {
+  if (!(delta > 0 && delta < 1)) return fail(PGSVT_E_ARG, "probe step delta must be in (0, 1)");
+  if (!std::isfinite(gap)) return fail(PGSVT_E_ARG, "probe gap limit must be finite");
+  std::lock_guard<std::mutex> lk(g_fallback_mu);
+  g_probe_delta = delta;
+  g_probe_gap = gap;
+  return 0;
+}
+
+int pgsvt_learned_fallbacks(void) { return g_last_fallbacks; }
+
 int pgsvt_set_svd_mapping(int mode) {
   if (mode < 0 || mode > 2) return fail(PGSVT_E_ARG, "SVD mapping must be 0, 1 or 2");
   g_svd_mapping.store(mode);
@@ -1930,6 +2170,7 @@ int pgsvt_release_cache(void) {
     hipchk(hipGetDevice(&dev), "hipGetDevice");
     hipchk(hipDeviceSynchronize(), "hipDeviceSynchronize");
     DeviceCache::get().release(dev);
+    PinnedCache::get().release();
     return 0;
   });
 }

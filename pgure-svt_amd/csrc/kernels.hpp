@@ -40,11 +40,13 @@ hipError_t launch_prepare(const double* X, const double* Z, const WinMeta* meta,
 hipError_t launch_fill_ref(const WinMeta* meta, uint32_t nb, const Shape& sh, int32_t* pos, hipStream_t s);
 hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs, uint32_t nb, const Shape& sh,
                        int mw, int32_t* pos, hipStream_t s);
+// SVD of the first nvar of the nvar_stored variants of every window (uvar
+// [nb][nvar_stored][vol] -> factors [nb][nvar_stored][p_sel][fstride]).
 // mapping 0: the default lanes per matrix of the shape, 1: the alternative
 // (FP32 only; see svd_lanes)
 hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
-                      const Shape& sh, void* factors, int max_sweeps, double tol, uint32_t* stats, hipStream_t s,
-                      int mapping = 0);
+                      uint32_t nvar_stored, const Shape& sh, void* factors, int max_sweeps, double tol,
+                      uint32_t* stats, hipStream_t s, int mapping = 0);
 uint32_t svd_lanes(uint32_t bs, uint32_t nw, int mapping);
 // position buckets of every (window, frame): cnt and off are [nb][nw][n1*n1
 // (+1)], list [nb][nw][p_sel]
@@ -107,17 +109,22 @@ struct GatherSrc {
 };
 // values per stored factor block of one (patch, variant): V^T (nw rows of
 // sr = nw rounded up to 4 values), W^T (nw rows of m values), the nw
-// singular values, rounded up to a multiple of 32 values, so that every
-// block starts a 128-byte line and no line is written by two matrices
-// (which run in different waves, often on different XCDs)
+// singular values, rounded up to an odd multiple of 32 values: every block
+// starts a 128-byte line, so no line is written by two matrices (which run
+// in different waves, often on different XCDs), and the stride is not a
+// power of two (2048 bytes for 16 x 15 matrices slowed the SVD's stores by
+// a tenth: they met in the same memory channels)
 uint32_t factor_stride(uint32_t nw, uint32_t m);
 // overlap-add gather: mode 0 risk sums into partial [na][gather_blocks][5]; mode 1
 // final reference frames into out (the output cube); mode 2 the whole
-// window of slot 0 into out
-uint32_t gather_blocks(const Shape& sh, int mode);
+// window of slot 0 into out.  nk (mode 0, patches up to 4 x 4): the sums
+// cover nk frames spread over the window, k_i = i (nw - 1) / (nk - 1)
+// (gather_frames: nw when nk is 0, above nw, or the patches are larger)
+uint32_t gather_frames(const Shape& sh, uint32_t nk);
+uint32_t gather_blocks(const Shape& sh, int mode, uint32_t nk = 0);
 hipError_t launch_gather(const GatherSrc& src, const int32_t* off, const double* invw, const double* u64,
                          const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int mode, double* out,
-                         double* partial, hipStream_t s);
+                         double* partial, hipStream_t s, uint32_t nk = 0);
 // sums [na][6] = (sum Uhat^2, sum U, sum U^2, sum Uhat U,
 // sum d2 (U2p - 2 Uhat + U2m), sum Uhat); with pk (the eval's kept-rank
 // counts), kept [na] = their sum per window

@@ -94,6 +94,8 @@ def _bind_pipeline(L):
         L.pgsvt_denoise_range.argtypes = [vp, C.c_int, u32, u32, u32, P, u32, u32, vp, vp]
         L.pgsvt_learned_features.argtypes = [vp, C.c_int, u32, u32, u32, P, u32, u32, vp, vp]
         L.pgsvt_denoise_learned.argtypes = [vp, C.c_int, u32, u32, u32, P, C.POINTER(_Model), vp, vp]
+        L.pgsvt_set_learned_fallback.argtypes = [C.c_double, C.c_double]
+        L.pgsvt_learned_fallbacks.argtypes = []
         L._learned_pipeline_bound = True
     return L
 
@@ -200,9 +202,27 @@ def default_model_path():
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), "lambda_model.npz")
 
 
+FALLBACK_DELTA, FALLBACK_GAP = 0.1, 0.01  # the library's defaults
+
+
+def set_fallback(delta=FALLBACK_DELTA, gap=FALLBACK_GAP):
+    """The learned path's probe (pgsvt_set_learned_fallback): risks at
+    lambda / (1 + delta), lambda and lambda (1 + delta); a window whose
+    predicted relative risk gap exceeds `gap` is searched.  gap <= 0: no
+    probe, every proposal kept.  Process-wide."""
+    L = _bind_pipeline(_lib())
+    _P.check(L.pgsvt_set_learned_fallback(float(delta), float(gap)))
+
+
+def last_fallbacks():
+    """Windows searched by this thread's last denoise_learned call."""
+    return int(_bind_pipeline(_lib()).pgsvt_learned_fallbacks())
+
+
 def denoise_learned(X, model=None, **params):
     """Denoise X (rows, cols, frames) with learned lambdas: (Y, estimates
-    (frames, 4): proposed lambda, alpha, mu, sigma)."""
+    (frames, 4): lambda (the proposal, or the search's where the probe sent
+    the window to the search), alpha, mu, sigma)."""
     if model is None:
         model = LambdaModel.load(default_model_path())
     X, dt = _cube(X)

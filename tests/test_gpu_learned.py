@@ -19,6 +19,19 @@ def seq():
     return noisy
 
 
+@pytest.fixture
+def no_probe():
+    Lr.set_fallback(Lr.FALLBACK_DELTA, 0.0)
+    yield
+    Lr.set_fallback()
+
+
+def _random_model(n_in, y_mean, seed=0):
+    rng = np.random.default_rng(seed)
+    return Lr.LambdaModel(Lr.init_params(n_in, 32, 1) * 0.05, rng.standard_normal(n_in), np.full(n_in, 3.0),
+                          y_mean, 0.3, hidden=32)
+
+
 def test_range_equals_full_call(gpu, seq):
     # windows are independent: any sub-range gives the full call's frames
     Yf, ef = Lr.denoise_range(seq, 0, 20, **PARAMS)
@@ -57,14 +70,13 @@ def test_feature_noise_equals_driver_noise_when_window_is_sequence(gpu):
     assert np.array_equal(F[0, -3:], est[0, 1:].astype(np.float32))
 
 
-def test_learned_path_applies_the_proposal(gpu, seq):
-    # a random model: each output frame equals the fixed-lambda driver run at
-    # the proposed lambda of its window
+def test_learned_path_applies_the_proposal(gpu, seq, no_probe):
+    # a random model, no probe: each output frame equals the fixed-lambda
+    # driver run at the proposed lambda of its window
     n_in = Lr.n_features(15)
-    rng = np.random.default_rng(0)
-    model = Lr.LambdaModel(Lr.init_params(n_in, 32, 1) * 0.05, rng.standard_normal(n_in), np.full(n_in, 3.0),
-                           np.log(0.5), 0.3, hidden=32)
+    model = _random_model(n_in, np.log(0.5))
     Y, est = Lr.denoise_learned(seq, model, **PARAMS)
+    assert Lr.last_fallbacks() == 0
     F, _ = Lr.features(seq, 0, 20, **PARAMS)
     lam = np.exp(model.predict_log_lambda(F))
     assert np.allclose(est[:, 0], lam, rtol=1e-5)
@@ -111,3 +123,51 @@ def test_trained_model_gains_over_no_network(gpu):
                 gap[k].append((v - r[0]) / abs(r[0]))
     g = {k: float(np.mean(v)) for k, v in gap.items()}
     assert g["net"] < 0.5 * g["median"] and g["net"] < 0.2 * g["start"], g
+
+
+def test_probe_sends_poor_proposals_to_the_search(gpu, seq, no_probe):
+    # A model that proposes lambdas 30x too large: with the probe on (the
+    # library's default), the windows it sends to the search come back with
+    # the search's lambda, whose PGURE risk (same noise estimates) is below
+    # the proposal's, and their frames are the fixed-lambda driver's at that
+    # lambda; the windows it keeps carry the proposal unchanged.
+    n_in = Lr.n_features(15)
+    _, est0 = Lr.denoise_range(seq, 0, 20, **PARAMS)
+    model = _random_model(n_in, float(np.log(30 * np.median(est0[:, 0]))))
+    _, prop = Lr.denoise_learned(seq, model, **PARAMS)  # probe off: the proposals
+    Lr.set_fallback()
+    Y, est = Lr.denoise_learned(seq, model, **PARAMS)
+    n_fb = Lr.last_fallbacks()
+    moved = np.flatnonzero(est[:, 0] != prop[:, 0])
+    assert 0 < n_fb and len(moved) <= n_fb, (n_fb, moved)
+    assert np.array_equal(est[:, 1:], prop[:, 1:])
+    for t in moved[:4]:
+        r = Lr.window_risk(seq, int(t), est[t, 1:], [est[t, 0], prop[t, 0]], **PARAMS)
+        assert r[0] < r[1], (t, r)
+        Yt, _ = Lr.denoise_range(seq, int(t), int(t) + 1, **dict(PARAMS, optimize_pgure=0, lambda1=float(est[t, 0])))
+        assert np.array_equal(Y[:, :, t], Yt[:, :, 0])
+
+
+def test_probe_keeps_near_optimal_proposals(gpu, seq, no_probe):
+    # The searched lambdas themselves as proposals (a model that returns
+    # each window's search result is emulated by a fixed lambda equal to the
+    # median of the searched ones on a stationary sequence): the probe keeps
+    # most windows, and with the limit raised to 1e9 it keeps every one.
+    n_in = Lr.n_features(15)
+    _, est0 = Lr.denoise_range(seq, 0, 20, **PARAMS)
+    model = _random_model(n_in, float(np.log(np.median(est0[:, 0]))))
+    model.params[:] = 0.0  # the network outputs 0: lambda = exp(y_mean) everywhere
+    Lr.set_fallback(0.1, 1e9)
+    Lr.denoise_learned(seq, model, **PARAMS)
+    assert Lr.last_fallbacks() == 0
+    Lr.set_fallback()
+    _, est = Lr.denoise_learned(seq, model, **PARAMS)
+    assert Lr.last_fallbacks() <= 10, Lr.last_fallbacks()
+
+
+def test_fallback_setter_rejects_bad_values(gpu):
+    with pytest.raises(RuntimeError, match="delta"):
+        Lr.set_fallback(0.0, 0.01)
+    with pytest.raises(RuntimeError, match="finite"):
+        Lr.set_fallback(0.1, float("nan"))
+    Lr.set_fallback()

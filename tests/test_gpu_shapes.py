@@ -35,8 +35,14 @@ def seq48():
 
 
 # (patch, step, trajectory, exponential weighting): windows the table has no
-# kernels for (patches 2-4 at 1, 3 and 5 frames included), patches 5, 7 and
-# 16, steps 1 and 2
+# kernels for (patches 2-4 at 1 and 7 frames included), patches 5, 7 and 16,
+# steps 1-3, and even clamped trajectory lengths Nt (trajectory_length itself
+# must be odd), whose window has Nt + 1 frames (src/pguresvt.hpp:57-58):
+# patch 3 clamps 21 to 8 (window 9), patch 5 clamps 27 and 29 to 24 (window
+# 25), patch 2 clamps 17 to 3.  With an even Nt the motion estimator runs
+# over the window's Nt + 1 frames (src/arps.hpp:35) while the driver places
+# the output frame with Nt (src/pguresvt.hpp:161): the end windows (t = 33)
+# check both.
 SHAPES = [
     (5, 1, 7, True),
     (5, 2, 25, True),
@@ -49,6 +55,10 @@ SHAPES = [
     (2, 1, 1, False),
     (16, 2, 15, True),
     (16, 3, 5, False),
+    (3, 2, 21, False),
+    (5, 1, 27, True),
+    (5, 2, 29, False),
+    (2, 1, 17, True),
 ]
 
 
