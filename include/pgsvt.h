@@ -190,10 +190,12 @@ int pgsvt_learned_features(const void* X, int dtype, uint32_t n_rows,
  * (pgsvt_set_learned_fallback): the window's PGURE risk at lambda / (1 +
  * delta), lambda and lambda (1 + delta), from the perturbed decompositions
  * of every 8th patch and the reconstruction sums of 5 of the window's
- * frames; where the parabola through the three predicts a relative risk
- * gap above the limit, the window is searched as by pgsvt_denoise (its
- * full decompositions, SBPLX from the reference's start point) and its
- * frame replaced.  estimates holds the final lambda and the noise
+ * frames.  Where the parabola through the three predicts a relative risk
+ * gap above the limit, lambda moves to the parabola's minimum and is
+ * probed again, up to `refine` times; a window still above the limit is
+ * searched (its full decompositions, the PGURE risk as pgsvt_denoise
+ * evaluates it, SBPLX from the last estimate to a relative tolerance of
+ * 1e-5) and its frame replaced.  estimates holds the final lambda and the noise
  * features' (alpha, mu, sigma). */
 int pgsvt_denoise_learned(const void* X, int dtype, uint32_t n_rows,
                           uint32_t n_cols, uint32_t n_frames,
@@ -207,11 +209,12 @@ int pgsvt_denoise_learned_device(const void* X_dev, int dtype, uint32_t n_rows,
                                  const pgsvt_lambda_model* model,
                                  double* Y_dev, double* estimates);
 
-/* Probe of pgsvt_denoise_learned: step delta in (0, 1) (default 0.1) and
- * the predicted relative risk gap above which a window falls back to the
- * search (default 0.01; gap <= 0 turns the probe and the fallback off).
- * Process-wide.  No reference counterpart. */
-int pgsvt_set_learned_fallback(double delta, double gap);
+/* Probe of pgsvt_denoise_learned: step delta in (0, 1) (default 0.1), the
+ * predicted relative risk gap above which a window's lambda is refined or
+ * searched (default 0.01; gap <= 0 turns the probe off), and the probes at
+ * refined lambdas before the search (0-16, default 2).  Process-wide.  No
+ * reference counterpart. */
+int pgsvt_set_learned_fallback(double delta, double gap, uint32_t refine);
 /* Windows that fell back to the search in the calling thread's last
  * pgsvt_denoise_learned(_device) call. */
 int pgsvt_learned_fallbacks(void);
@@ -315,6 +318,15 @@ int pgsvt_window_singular_values(const void* X, int dtype, uint32_t n_rows,
 int pgsvt_noise_estimate(const double* u, uint32_t n_rows, uint32_t n_cols,
                          uint32_t n_t, uint32_t method, double* alpha,
                          double* mu, double* sigma);
+
+/* Robust (mean, variance) of every quadtree leaf of one normalised n x n
+ * frame, in the tree's order (src/noise.hpp:55-100): on the host
+ * (on_gpu = 0, the estimator the driver uses) or on the GPU (on_gpu = 1,
+ * the learned path's; frames of at least 16 x 16).  *count receives the
+ * leaf count; mean and var hold up to cap values.  For the tests. */
+int pgsvt_frame_leaves(const double* frame, uint32_t n, int on_gpu,
+                       uint32_t cap, double* mean, double* var,
+                       uint32_t* count);
 
 /* The delta2 perturbation signs the GPU generates: PGURE<T>::
  * GenerateRandomPerturbations (src/pgure.hpp:167-186) with pcg64 seeded by

@@ -34,6 +34,7 @@ SOURCES = [
     ("kernels.hip", DEVICE),
     ("hotpixel.hip", DEVICE),
     ("mlp.hip", DEVICE),
+    ("noise_gpu.hip", DEVICE),
     ("host.cpp", DEVICE),
     ("noise.cpp", HOST),
     ("tiff.cpp", HOST),

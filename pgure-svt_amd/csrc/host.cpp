@@ -64,10 +64,11 @@ std::atomic<int> g_svd_mapping{0};
 // thread: pgsvt_learned_fallbacks.
 std::mutex g_fallback_mu;
 double g_probe_delta = 0.1, g_probe_gap = 0.01;
+uint32_t g_probe_refine = 2;
 thread_local int g_last_fallbacks = 0;
 // probe sample: every kProbeStride-th selected patch in all three variants,
 // the data terms over kProbeFrames frames spread over the window
-constexpr uint32_t kProbeStride = 8, kProbeFrames = 5;
+constexpr uint32_t kProbeStride = 16, kProbeFrames = 3;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -968,25 +969,31 @@ struct Pipeline {
     probe_on = true;
   }
 
-  // the subsample's three variants decomposed and their divergence factors
-  // (after decompose_kernels of the same batch)
+  // the subsample's three variants and their divergence factors (after
+  // decompose_kernels of the same batch): the U variant's factor blocks
+  // copied from the main decomposition, the perturbed variants decomposed
   void decompose_probe(uint32_t nb) {
-    prof->run("probe_decompose", (double)nb * 3 * sh_s.p_sel, [&] {
-      hipchk(launch_svd(uvar.p, pos.p, sel_s.p, nb, 3, 3, sh_s, factors_s.p, 30, 1e-6, stats.p, s,
+    prof->run("probe_decompose", (double)nb * 2 * sh_s.p_sel, [&] {
+      const size_t fw = sh.fstride;  // 32-bit words per factor block (FP32)
+      hipchk(launch_copy_rows(factors.p, factors_s.p, sh_s.p_sel, (uint32_t)fw, kProbeStride,
+                              (size_t)nvar * sh.p_sel * fw, (size_t)3 * sh_s.p_sel * fw, nb, s),
+             "probe U factors");
+      hipchk(launch_svd(uvar.p + sizeof(float) * sh.vol, pos.p, sel_s.p, nb, 2, 3, sh_s,
+                        factors_s.p + sizeof(float) * sh_s.p_sel * fw, 30, 1e-6, stats.p, s,
                         g_svd_mapping.load() == 1 ? 1 : 0),
              "probe svd");
       hipchk(launch_divq(factors_s.p, pos.p, sel_s.p, dw.p, nb, 3, sh_s, divq_s.p, s), "probe divergence factors");
     });
   }
 
-  // PGURE risks of the batch windows ms at three lambdas each, lam [nb][3],
-  // into risk [nb][3]: the data sums over kProbeFrames frames of the window
+  // PGURE risks of the batch windows `wins` (slots of ms) at three lambdas
+  // each, lam [wins][3], into risk [wins][3]: the data sums over kProbeFrames frames of the window
   // (all patches reconstructed) and the divergence sum over the patch
   // subsample, each scaled to the whole window.  The same sample for every
   // lambda of a window, so the three risks differ by the lambda only.
   void probe_risks(const std::vector<WinMeta>& ms, const std::vector<WindowPrep>& prep,
-                   const std::vector<double>& lam, std::vector<double>& risk) {
-    const uint32_t nb = (uint32_t)ms.size(), nslot = 3 * nb;
+                   const std::vector<uint32_t>& wins, const std::vector<double>& lam, std::vector<double>& risk) {
+    const uint32_t nb = (uint32_t)ms.size(), nslot = 3 * (uint32_t)wins.size();
     const uint32_t nk = gather_frames(sh, kProbeFrames);
     const uint32_t gblk = pgsvt::gather_blocks(sh, 0, nk), eblk = eval_blocks(sh_s, 3);
     const double fk = (double)sh.nw / nk, fp = (double)sh.p_sel / sh_s.p_sel;
@@ -995,7 +1002,7 @@ struct Pipeline {
     std::vector<uint32_t> wl(NB);
     for (uint32_t c0 = 0; c0 < nslot; c0 += NB) {
       const uint32_t na = std::min(NB, nslot - c0);
-      for (uint32_t i = 0; i < na; ++i) wl[i] = (c0 + i) / 3;
+      for (uint32_t i = 0; i < na; ++i) wl[i] = wins[(c0 + i) / 3];
       hipchk(hipMemcpyAsync(wl_s.p, wl.data(), na * 4, hipMemcpyHostToDevice, s), "H2D probe list");
       hipchk(hipMemcpyAsync(lam_s.p, lam.data() + c0, na * 8, hipMemcpyHostToDevice, s), "H2D probe lambdas");
       prof->run("probe_eval", (double)na * sh.vol, [&] {
@@ -1013,7 +1020,7 @@ struct Pipeline {
       for (uint32_t i = 0; i < na; ++i) {
         double S[6];
         for (int q = 0; q < 6; ++q) S[q] = sums_h.p[(size_t)i * 6 + q] * (q == 4 ? fp : fk);
-        risk[c0 + i] = this->risk(S, prep[(c0 + i) / 3]);
+        risk[c0 + i] = this->risk(S, prep[wins[(c0 + i) / 3]]);
       }
     }
     prof->flush();
@@ -1123,9 +1130,17 @@ struct Pipeline {
 // its prep is there; every round evaluates the risk of the windows that
 // asked for a lambda together.  On return ms[b].lambda is each window's
 // last evaluated lambda (Q4).
+// x0 (optional): per window a start point and initial step in place of the
+// reference's (prep.start, its square root), with the relative tolerance
+// tol_x0, for searches that start near the minimum.
+struct SearchStart {
+  std::vector<double> x, step;
+  double tol = 0;
+};
+
 void lockstep_search(Pipeline& P, std::vector<WinMeta>& ms, std::vector<WindowPrep>& prep,
                      const std::function<bool(uint32_t)>& ready, const std::function<void(uint32_t)>& fill,
-                     int& status, std::string& warn) {
+                     int& status, std::string& warn, const SearchStart* x0 = nullptr) {
   const pgsvt_params& p = P.p;
   const uint32_t nb = (uint32_t)ms.size();
   std::vector<std::unique_ptr<Sbplx1D>> opt(nb);
@@ -1136,7 +1151,10 @@ void lockstep_search(Pipeline& P, std::vector<WinMeta>& ms, std::vector<WindowPr
     fill(b);
     const double start = prep[b].start;
     const double ub = std::max(100.0, start);
-    opt[b].reset(new Sbplx1D(start, 0.0, ub, std::sqrt(start), p.tol, 1e-12, (int)p.max_iter));
+    if (x0)
+      opt[b].reset(new Sbplx1D(std::min(x0->x[b], ub), 0.0, ub, x0->step[b], x0->tol, 1e-12, (int)p.max_iter));
+    else
+      opt[b].reset(new Sbplx1D(start, 0.0, ub, std::sqrt(start), p.tol, 1e-12, (int)p.max_iter));
     started[b] = 1;
     n_started++;
   };
@@ -1209,21 +1227,101 @@ void lockstep_search(Pipeline& P, std::vector<WinMeta>& ms, std::vector<WindowPr
   }
 }
 
-// Predicted relative risk gap of the middle of three lambdas from their
-// risks: the parabola through (log lambda, risk) has its minimum
-// slope^2 / (4 curvature) below the middle risk.  Not convex, or points
-// that coincide (lambda at a bound): the best observed improvement, or
-// infinity when a neighbour is lower on a concave curve.
-double probe_gap(const double* lam, const double* r) {
-  const double xm = std::log(lam[0]), x0 = std::log(lam[1]), xp = std::log(lam[2]);
+// Probe of three lambdas (ascending) and their risks: the predicted
+// relative risk gap of the middle one, and where a search for the minimum
+// should start.  The parabola through (log lambda, risk) has its minimum
+// slope^2 / (4 curvature) below the middle risk, at its vertex (clamped to
+// three probe steps from the middle).  Not convex, or points that coincide
+// (lambda at a bound): the best observed improvement, or infinity when a
+// neighbour is lower on a concave curve, and the lower neighbour as start.
+struct ProbeFit {
+  double gap, start;
+};
+
+ProbeFit probe_fit(const double* lam, const double* r) {
   const double lo = std::min(r[0], r[2]), scale = std::fabs(r[1]);
-  if (!std::isfinite(xm) || !std::isfinite(xp) || !(scale > 0)) return INFINITY;
-  if (!(xp > x0) || !(x0 > xm)) return lo < r[1] ? (r[1] - lo) / scale : 0.0;
+  const double best = (r[0] <= r[2]) ? lam[0] : lam[2];
+  const double xm = std::log(lam[0]), x0 = std::log(lam[1]), xp = std::log(lam[2]);
+  if (!std::isfinite(xm) || !std::isfinite(xp) || !(scale > 0)) return {INFINITY, lam[1]};
+  if (!(xp > x0) || !(x0 > xm)) return {lo < r[1] ? (r[1] - lo) / scale : 0.0, lo < r[1] ? best : lam[1]};
   const double d1 = (r[1] - r[0]) / (x0 - xm), d2 = (r[2] - r[1]) / (xp - x0);
   const double a = (d2 - d1) / (xp - xm);  // q(x) = r0 + d1 (x - xm) + a (x - xm)(x - x0)
   const double b = d1 + a * (x0 - xm);     // q'(x0)
-  if (!(a > 0)) return lo < r[1] ? INFINITY : 0.0;
-  return b * b / (4 * a) / scale;
+  if (!(a > 0)) return {lo < r[1] ? INFINITY : 0.0, lo < r[1] ? best : lam[1]};
+  const double h = std::max(x0 - xm, xp - x0);
+  const double xs = std::min(std::max(x0 - b / (2 * a), x0 - 3 * h), x0 + 3 * h);
+  return {b * b / (4 * a) / scale, std::exp(xs)};
+}
+
+// Leaf statistics of the noise estimator (src/noise.hpp:55-100) for frames
+// `frames` of the device cube Xd (n x n frames, FP64), normalised by scale,
+// on the GPU: the split tests of every block of the complete quadtree whose
+// size the F table holds (k_split_stats), the reference's recursion on the
+// host with those decisions, then the statistics of each distinct leaf
+// (k_leaf_stats), in the tree's order per frame.  Frames of at least 16 x 16
+// (smaller ones never split and stay on the host).
+std::vector<FrameLeaves> gpu_frame_leaves(const double* Xd, uint32_t n, const std::vector<uint32_t>& frames,
+                                          double scale, hipStream_t s) {
+  std::vector<uint32_t> cand;
+  std::map<std::array<uint32_t, 3>, uint32_t> cand_at;
+  std::function<void(uint32_t, uint32_t, uint32_t)> add = [&](uint32_t i, uint32_t j, uint32_t N) {
+    if (split_threshold(N) < 0) return;
+    cand_at[{i, j, N}] = (uint32_t)(cand.size() / 3);
+    cand.insert(cand.end(), {i, j, N});
+    const uint32_t h = N / 2;
+    add(i, j, h);
+    add(i + h, j, h);
+    add(i, j + h, h);
+    add(i + h, j + h, h);
+  };
+  add(0, 0, n);
+  const uint32_t nf = (uint32_t)frames.size(), nc = (uint32_t)(cand.size() / 3);
+  std::vector<double> stat((size_t)nf * nc);
+  DBuf<uint32_t> frames_d, cand_d, leaves_d;
+  DBuf<double> stat_d, out_d;
+  if (nc > 0) {
+    frames_d.alloc(nf);
+    cand_d.alloc(cand.size());
+    stat_d.alloc(stat.size());
+    hipchk(hipMemcpyAsync(frames_d.p, frames.data(), nf * 4, hipMemcpyHostToDevice, s), "H2D leaf frames");
+    hipchk(hipMemcpyAsync(cand_d.p, cand.data(), cand.size() * 4, hipMemcpyHostToDevice, s), "H2D split blocks");
+    hipchk(launch_split_stats(Xd, n, scale, frames_d.p, nf, cand_d.p, nc, stat_d.p, s), "split stats");
+    hipchk(hipMemcpyAsync(stat.data(), stat_d.p, stat.size() * 8, hipMemcpyDeviceToHost, s), "D2H split stats");
+    hipchk(hipStreamSynchronize(s), "split stats sync");
+  }
+  // the reference's recursion per frame; each distinct leaf evaluated once
+  std::vector<uint32_t> leaves;
+  std::vector<std::vector<uint32_t>> order(nf);
+  for (uint32_t fi = 0; fi < nf; ++fi) {
+    const auto tree = quadtree_leaves(n, [&](uint32_t i, uint32_t j, uint32_t N) {
+      const double thr = split_threshold(N);
+      return thr >= 0 && stat[(size_t)fi * nc + cand_at.at({i, j, N})] > thr;
+    });
+    std::map<std::array<uint32_t, 3>, uint32_t> seen;
+    for (const auto& nd : tree) {
+      auto it = seen.find(nd);
+      if (it == seen.end()) {
+        it = seen.emplace(nd, (uint32_t)(leaves.size() / 4)).first;
+        leaves.insert(leaves.end(), {frames[fi], nd[0], nd[1], nd[2]});
+      }
+      order[fi].push_back(it->second);
+    }
+  }
+  const uint32_t nl = (uint32_t)(leaves.size() / 4);
+  std::vector<double> stats((size_t)nl * 2);
+  leaves_d.alloc(leaves.size());
+  out_d.alloc(stats.size());
+  hipchk(hipMemcpyAsync(leaves_d.p, leaves.data(), leaves.size() * 4, hipMemcpyHostToDevice, s), "H2D leaves");
+  hipchk(launch_leaf_stats(Xd, n, scale, leaves_d.p, nl, out_d.p, s), "leaf stats");
+  hipchk(hipMemcpyAsync(stats.data(), out_d.p, stats.size() * 8, hipMemcpyDeviceToHost, s), "D2H leaf stats");
+  hipchk(hipStreamSynchronize(s), "leaf stats sync");
+  std::vector<FrameLeaves> out(nf);
+  for (uint32_t fi = 0; fi < nf; ++fi)
+    for (uint32_t k : order[fi]) {
+      out[fi].mean.push_back(stats[2 * (size_t)k]);
+      out[fi].var.push_back(stats[2 * (size_t)k + 1]);
+    }
+  return out;
 }
 
 // Host threads for n_jobs (-1 = all): the CPUs this process may run on,
@@ -1399,10 +1497,12 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
   // is on (then every variant volume is prepared: the probe decomposes a
   // patch subsample in all three, and windows that fail it are searched)
   double probe_delta = 0, probe_gap_max = 0;
+  uint32_t probe_refine = 0;
   {
     std::lock_guard<std::mutex> lk(g_fallback_mu);
     probe_delta = g_probe_delta;
     probe_gap_max = g_probe_gap;
+    probe_refine = g_probe_refine;
   }
   const bool probe = rs.mode == 1 && probe_gap_max > 0;
   if (learned) pl.optimize_pgure = probe ? 1 : 0;
@@ -1502,8 +1602,20 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
   };
   std::vector<std::unique_ptr<LeafJob>> leaf_jobs;
   std::unique_ptr<PrepPool> pool;
+  // learned path: the GPU leaf statistics on a thread of their own (joined
+  // before the pool and the jobs it serves go away)
+  std::thread leaf_thread;
+  struct Joiner {
+    std::thread& t;
+    ~Joiner() {
+      if (t.joinable()) t.join();
+    }
+  } leaf_joiner{leaf_thread};
   const double gmax = *std::max_element(P.fmaxX.begin(), P.fmaxX.end());
   if (P.optimize || learned) {
+    // learned lambda: the leaf statistics of every frame on the GPU (one
+    // scale for all); the reference path keeps the host's bit-exact ones
+    const bool leaves_on_gpu = learned && nr >= 16;
     std::map<std::pair<uint32_t, uint64_t>, uint32_t> key_of;
     std::vector<std::function<void(PrepPool&)>> jobs;
     for (uint32_t t = rs.tb; t < rs.te; ++t) {
@@ -1520,15 +1632,16 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
           leaf_jobs.emplace_back(new LeafJob{m.s0 + k, scale, {}, {}});
           LeafJob* J = leaf_jobs.back().get();
           J->fu = J->pr.get_future().share();
-          jobs.push_back([J, &P, nr](PrepPool&) {
-            try {
-              std::vector<double> buf((size_t)nr * nr);
-              P.host_frame(J->frame, J->scale, buf.data());
-              J->pr.set_value(std::make_shared<const FrameLeaves>(frame_leaves(buf.data(), nr)));
-            } catch (...) {
-              J->pr.set_exception(std::current_exception());
-            }
-          });
+          if (!leaves_on_gpu)
+            jobs.push_back([J, &P, nr](PrepPool&) {
+              try {
+                std::vector<double> buf((size_t)nr * nr);
+                P.host_frame(J->frame, J->scale, buf.data());
+                J->pr.set_value(std::make_shared<const FrameLeaves>(frame_leaves(buf.data(), nr)));
+              } catch (...) {
+                J->pr.set_exception(std::current_exception());
+              }
+            });
           it = key_of.emplace(key, id).first;
         }
         need.push_back(it->second);
@@ -1565,6 +1678,36 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
           w.err = "noise estimate failed";
         }
         pool.publish(t - rs.tb, std::move(w));
+      });
+    }
+    if (leaves_on_gpu) {
+      // on a stream of their own, beside the decomposition; the window jobs
+      // wait for the frames they need
+      int dev = 0;
+      hipchk(hipGetDevice(&dev), "hipGetDevice");
+      std::vector<uint32_t> frames;
+      for (const auto& J : leaf_jobs) frames.push_back(J->frame);
+      const double* Xd = P.Xd.p;
+      leaf_thread = std::thread([&leaf_jobs, frames, dev, nr, gmax, Xd]() {
+        size_t done = 0;
+        try {
+          hipchk(hipSetDevice(dev), "hipSetDevice");
+          hipStream_t s2 = nullptr;
+          hipchk(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking), "hipStreamCreate");
+          struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() {
+              (void)hipStreamSynchronize(s);
+              (void)hipStreamDestroy(s);
+            }
+          } guard{s2};
+          HostTimer tl("noise_leaves_gpu");
+          std::vector<FrameLeaves> lv = gpu_frame_leaves(Xd, nr, frames, gmax, s2);
+          for (; done < leaf_jobs.size(); ++done)
+            leaf_jobs[done]->pr.set_value(std::make_shared<const FrameLeaves>(std::move(lv[done])));
+        } catch (...) {
+          for (; done < leaf_jobs.size(); ++done) leaf_jobs[done]->pr.set_exception(std::current_exception());
+        }
       });
     }
     pool.reset(new PrepPool(std::move(jobs), count, thread_count(p.n_jobs)));
@@ -1689,6 +1832,7 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
     if (probe) P.decompose_probe(nb);
     std::vector<WindowPrep> prep(nb);
     std::vector<uint32_t> fallback;  // learned lambda: windows the probe sends to the search
+    SearchStart restart;             // and where their searches start
     if (learned) {
       // log singular values of the sampled patches (standardised for the
       // network), then the noise features of each window as they arrive
@@ -1733,19 +1877,45 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
         P.prof->flush();
         if (probe) {
           // local optimality of each proposal: risks at lambda / f, lambda,
-          // lambda f (clamped to the search's bounds); a predicted gap above
-          // the limit sends the window to the search
+          // lambda f (clamped to the search's bounds).  A window whose
+          // parabola predicts a gap above the limit moves to the parabola's
+          // minimum and is probed again there, up to probe_refine times; a
+          // window still above the limit then goes to the search, which
+          // starts at the last estimate.
           const double f = 1.0 + probe_delta;
-          std::vector<double> lam((size_t)nb * 3), risk;
-          for (uint32_t b = 0; b < nb; ++b) {
-            const double l = ms[b].lambda, ub = std::max(100.0, prep[b].start);
-            lam[3 * b] = l / f;
-            lam[3 * b + 1] = l;
-            lam[3 * b + 2] = std::min(l * f, ub);
+          std::vector<uint32_t> active(nb);
+          for (uint32_t b = 0; b < nb; ++b) active[b] = b;
+          double refined = 0;
+          for (uint32_t round = 0; !active.empty(); ++round) {
+            std::vector<double> lam(active.size() * 3), risk;
+            for (size_t i = 0; i < active.size(); ++i) {
+              const uint32_t b = active[i];
+              const double l = ms[b].lambda, ub = std::max(100.0, prep[b].start);
+              lam[3 * i] = l / f;
+              lam[3 * i + 1] = l;
+              lam[3 * i + 2] = std::min(l * f, ub);
+            }
+            P.probe_risks(ms, prep, active, lam, risk);
+            std::vector<uint32_t> again;
+            for (size_t i = 0; i < active.size(); ++i) {
+              const uint32_t b = active[i];
+              const ProbeFit pf = probe_fit(&lam[3 * i], &risk[3 * i]);
+              if (pf.gap <= probe_gap_max) continue;  // accepted at its current lambda
+              const double ub = std::max(100.0, prep[b].start);
+              const double next = std::min(pf.start, ub);
+              if (round < probe_refine) {
+                ms[b].lambda = next;
+                again.push_back(b);
+                refined += 1;
+              } else {
+                fallback.push_back(b);
+                restart.x.push_back(next);
+                restart.step.push_back(std::max(std::fabs(next - ms[b].lambda), probe_delta * ms[b].lambda));
+              }
+            }
+            active.swap(again);
           }
-          P.probe_risks(ms, prep, lam, risk);
-          for (uint32_t b = 0; b < nb; ++b)
-            if (!(probe_gap(&lam[3 * b], &risk[3 * b]) <= probe_gap_max)) fallback.push_back(b);
+          prof_host("probe_refinements", 0.0, refined);
         }
       }
     } else if (P.optimize) {
@@ -1783,7 +1953,11 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
       }
       P.decompose(mf, 3);
       hipchk(hipStreamSynchronize(P.s), "fallback decompose sync");
-      lockstep_search(P, mf, pf, [](uint32_t) { return true; }, [](uint32_t) {}, status, warn);
+      // SBPLX from the probe's estimate of the minimum, steps of the size of
+      // the correction, to a relative tolerance far below the risk gaps the
+      // probe is there to remove
+      restart.tol = std::max(p.tol, 1e-5);
+      lockstep_search(P, mf, pf, [](uint32_t) { return true; }, [](uint32_t) {}, status, warn, &restart);
       P.finish(mf);
       for (size_t i = 0; i < fallback.size(); ++i) ms[fallback[i]].lambda = mf[i].lambda;
       g_last_fallbacks += (int)fallback.size();
@@ -2147,12 +2321,14 @@ int pgsvt_set_precision(int bits) {
   return 0;
 }
 
-int pgsvt_set_learned_fallback(double delta, double gap) {
+int pgsvt_set_learned_fallback(double delta, double gap, uint32_t refine) {
   if (!(delta > 0 && delta < 1)) return fail(PGSVT_E_ARG, "probe step delta must be in (0, 1)");
   if (!std::isfinite(gap)) return fail(PGSVT_E_ARG, "probe gap limit must be finite");
+  if (refine > 16) return fail(PGSVT_E_ARG, "at most 16 probe refinements");
   std::lock_guard<std::mutex> lk(g_fallback_mu);
   g_probe_delta = delta;
   g_probe_gap = gap;
+  g_probe_refine = refine;
   return 0;
 }
 
@@ -2418,6 +2594,40 @@ int pgsvt_noise_estimate(const double* u, uint32_t n_rows, uint32_t n_cols, uint
   if (n_rows != n_cols) return fail(PGSVT_E_ARG, "quadtree noise estimation requires square frames");
   return guarded([&] {
     estimate_noise(u, n_rows, n_t, method, *alpha, *mu, *sigma);
+    return 0;
+  });
+}
+
+int pgsvt_frame_leaves(const double* frame, uint32_t n, int on_gpu, uint32_t cap, double* mean, double* var,
+                       uint32_t* count) {
+  if (!frame || !count || (cap > 0 && (!mean || !var))) return fail(PGSVT_E_ARG, "null pointer");
+  if (n < 1) return fail(PGSVT_E_ARG, "empty frame");
+  return guarded([&] {
+    FrameLeaves fl;
+    if (on_gpu) {
+      if (n < 16) return fail(PGSVT_E_ARG, "leaf statistics on the GPU need frames of at least 16 x 16");
+      if (int rc = check_device()) return rc;
+      hipStream_t s = nullptr;
+      hipchk(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+      struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() {
+          (void)hipStreamSynchronize(s);
+          (void)hipStreamDestroy(s);
+        }
+      } guard{s};
+      DBuf<double> d;
+      d.alloc((size_t)n * n);
+      hipchk(hipMemcpyAsync(d.p, frame, (size_t)n * n * 8, hipMemcpyHostToDevice, s), "H2D frame");
+      fl = std::move(gpu_frame_leaves(d.p, n, {0u}, 1.0, s)[0]);
+    } else {
+      fl = frame_leaves(frame, n);
+    }
+    *count = (uint32_t)fl.mean.size();
+    if (fl.mean.size() > cap) return fail(PGSVT_E_ARG, "the frame has " + std::to_string(fl.mean.size()) +
+                                                            " leaves, more than cap");
+    std::copy(fl.mean.begin(), fl.mean.end(), mean);
+    std::copy(fl.var.begin(), fl.var.end(), var);
     return 0;
   });
 }

@@ -15,10 +15,7 @@
 #include <cmath>
 #include <cstdint>
 #include <limits>
-#include <map>
-#include <mutex>
 #include <type_traits>
-#include <utility>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -783,6 +780,24 @@ __global__ void k_fill32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
     p[i] = v;
 }
 
+// rows of `width` 32-bit words: row r of batch b from src row r * step of
+// that batch (batch strides src_batch, dst_batch words)
+__global__ void k_copy_rows(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, uint32_t width,
+                            uint32_t step, size_t src_batch, size_t dst_batch) {
+  const uint32_t r = blockIdx.x, b = blockIdx.y;
+  const uint32_t* S = src + b * src_batch + (size_t)r * step * width;
+  uint32_t* D = dst + b * dst_batch + (size_t)r * width;
+  for (uint32_t w = threadIdx.x; w < width; w += blockDim.x) D[w] = S[w];
+}
+
+hipError_t launch_copy_rows(const void* src, void* dst, uint32_t rows, uint32_t width, uint32_t step,
+                            size_t src_batch, size_t dst_batch, uint32_t nb, hipStream_t s) {
+  if (rows == 0 || nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_rows, dim3(rows, nb), dim3(256), 0, s, (const uint32_t*)src, (uint32_t*)dst, width, step,
+                     src_batch, dst_batch);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill32(void* p, uint64_t n, uint32_t v, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
@@ -1425,10 +1440,10 @@ __global__ __launch_bounds__(256) void k_gather_direct(const T* __restrict__ B0,
 }
 
 // Tiled gather for patches up to 4 x 4, fused with the reconstruction.  A
-// tile is TR x TC voxels of one frame k; the patches that can cover it sit
-// in one contiguous bucket range per patch column.  The block forms those
-// patches' rows for frame k, B[q][k] = sum over kept ranks r of
-// fma(d_r w_r[q], v_r[k], .) (d_r from k_round_prep's records, w_0 and
+// block covers a tile of TR x TC voxels of one frame k; the patches that can
+// cover the tile sit in one contiguous bucket range per patch column.  The
+// block forms those patches' rows for frame k, B[q][k] = sum over kept ranks
+// r of fma(d_r w_r[q], v_r[k], .) (d_r from k_round_prep's records, w_0 and
 // v_0[k] from k_static_rows, further ranks from the factors; the FMA chain of
 // the reconstruction, in rank order), and stores them in LDS, the column ranges packed one after another (TOT rows
 // per tile), one plane per patch row (lanes on neighbouring voxels read
@@ -1438,11 +1453,8 @@ __global__ __launch_bounds__(256) void k_gather_direct(const T* __restrict__ B0,
 // remaining ones.  The order is fixed, so the sums are reproducible.  A tile
 // with more than TOT patch rows, or a column range longer than COLCAP
 // (trajectories piling up), forms each contribution where it is summed, in
-// the same order.  Logical tiles of one spatial tile run frame after frame,
-// so that the rows the tile's patches use in every frame meet in one L2.
-// The blocks are persistent (as many as are resident): each walks its
-// tiles, loading the next tile's bucket offsets while it stages and sums
-// the current one.
+// the same order.  Blocks of one tile run frame after frame, so that the
+// factor rows the tile's patches use in every frame meet in one L2.
 //   mode 0: risk sums per block: sum Uhat^2, sum Uhat, sum Uhat U, sum U, sum U^2
 //   mode 1: final frame: Y(:,:,out) = uMax * Uhat(:,:,ref)
 //   mode 2: whole window volume of slot 0 into out
@@ -1453,7 +1465,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(
     const int32_t* __restrict__ list, const int32_t* __restrict__ off, const double* __restrict__ invw,
     const double* __restrict__ u64, const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl, uint32_t nr,
     uint32_t fs, uint32_t nw, uint32_t vol, uint32_t n1, uint32_t p_sel, uint32_t fstride, uint32_t nvar_stored,
-    int mode, uint32_t nk, uint32_t nbx, uint32_t total, double* __restrict__ out, double* __restrict__ partial) {
+    int mode, uint32_t nk, double* __restrict__ out, double* __restrict__ partial) {
   // TOT: staged patch rows per tile, the bucket ranges of its NC patch
   // columns packed one after another; COLCAP: rows per patch column
   constexpr int M = BS * BS, TR = 32, TC = 8, NC = TC + BS - 1, NRW = TR + BS, TOT = 448, COLCAP = 64;
@@ -1471,266 +1483,242 @@ __global__ __launch_bounds__(256) void k_gather_tile(
   __shared__ int32_t sT[NC * NRW];
   __shared__ int32_t sP[NC];
   __shared__ double sm[5][4];
-  constexpr int NO = (NC * NRW + 255) / 256;
+  uint32_t bx, slot;
+  xcd_block(bx, slot);
+  const uint32_t b = wl[slot];
+  const WinMeta m = meta[b];
   const uint32_t ntr = (nr + TR - 1) / TR;  // square frames
-  // Logical tile L = slot * nbx + bx, total of them.  The blocks are
-  // persistent: XCD x (blocks p with p % 8 == x) takes a contiguous range of
-  // logical tiles, so neighbouring tiles, which read the same patches, meet
-  // in one L2; block p / 8 of the XCD takes every count-th tile of it.
-  constexpr uint32_t kXcd = 8;
-  const uint32_t xcd = blockIdx.x % kXcd, idx = blockIdx.x / kXcd;
-  const uint32_t count = gridDim.x / kXcd + (xcd < gridDim.x % kXcd ? 1u : 0u);
-  const uint32_t start = xcd * (total / kXcd) + min(xcd, total % kXcd);
-  const uint32_t len = total / kXcd + (xcd < total % kXcd ? 1u : 0u);
   // mode 0 covers nk frames of the window: all (nk = nw), or nk spread over
   // it (k_i = i (nw - 1) / (nk - 1)); mode 1 the reference frame
-  auto frame_of = [&](uint32_t bx_, uint32_t ref) {
-    const uint32_t i_ = bx_ % nk;
-    return (mode == 1) ? ref : (nk == nw ? i_ : (nk > 1 ? i_ * (nw - 1) / (nk - 1) : nw / 2));
-  };
-  // the bucket offsets of a tile's halo, loaded into registers ahead of the
-  // tile: cell e = (column e / NRW, row e % NRW), 0 for columns outside
-  auto prefetch = [&](uint32_t L, int32_t (&ov)[NO]) {
-    const uint32_t bx_ = L % nbx, slot_ = L / nbx, b_ = wl[slot_];
-    const uint32_t k_ = frame_of(bx_, mode == 1 ? meta[b_].ref : 0u);
-    const uint32_t tile_ = (mode == 1) ? bx_ : bx_ / nk;
-    const int r0_ = (int)((tile_ % ntr) * TR), c0_ = (int)((tile_ / ntr) * TC);
-    const int32_t* O_ = off + ((size_t)b_ * nw + k_) * ((size_t)n1 * n1 + 1);
+  const uint32_t i = bx % nk;
+  const uint32_t k = (mode == 1) ? m.ref : (nk == nw ? i : (nk > 1 ? i * (nw - 1) / (nk - 1) : nw / 2));
+  const uint32_t tile = (mode == 1) ? bx : bx / nk;
+  const uint32_t tr = tile % ntr, tc = tile / ntr;
+  const int r0 = (int)(tr * TR), c0 = (int)(tc * TC);
+  const int r = r0 + (int)(threadIdx.x % TR), c = c0 + (int)(threadIdx.x / TR);
+  const bool live = r < (int)nr && c < (int)nr;
+  const int32_t* O = off + ((size_t)b * nw + k) * ((size_t)n1 * n1 + 1);
+  const int32_t* Lk = list + ((size_t)b * nw + k) * p_sel;  // frame k's patches in bucket order
+  // bucket place j: the static w_0 row and v_0[k] at W0k + j * M, V0k[j],
+  // the patch's round record (d_0, kept ranks) at Rk + 2 j; patch id: d_r of
+  // further ranks at Ds[id * nw + r]; in the U variant's factors w_r at
+  // Ws + id * fstride + r * M (W^T) and v_r[k] at Vk + id * fstride + r * sr
+  // (V^T)
+  const uint32_t sr = (nw + 3) & ~3u;
+  const T* W0k = W0 + ((size_t)b * nw + k) * p_sel * M;
+  const T* V0k = V0 + ((size_t)b * nw + k) * p_sel;
+  const T* Rk = rec + ((size_t)slot * nw + k) * p_sel * 2;
+  const T* Ds = dr + (size_t)slot * p_sel * nw;
+  const T* Vk = factors + (size_t)b * nvar_stored * p_sel * fstride + k;
+  const T* Ws = factors + (size_t)b * nvar_stored * p_sel * fstride + nw * sr;
+  // the voxel's 1/W and U are independent of the staging: load them first
+  const uint32_t pix = (uint32_t)min(r, (int)nr - 1) + nr * (uint32_t)min(c, (int)nr - 1), v = k * fs + pix;
+  const double iw = invw[(size_t)b * vol + v];
+  const double U = (mode == 0) ? u64[(size_t)b * vol + v] : 0.0;
+  {
+    constexpr int NO = (NC * NRW + 255) / 256;
+    int32_t ov[NO];
 #pragma unroll
     for (int s = 0; s < NO; ++s) {
       const int e = min((int)threadIdx.x + 256 * s, NC * NRW - 1);
-      const int col = c0_ - (BS - 1) + e / NRW, row = min(max(r0_ - (BS - 1) + e % NRW, 0), (int)n1);
-      const int32_t o = O_[(size_t)n1 * min(max(col, 0), (int)n1 - 1) + row];
+      const int col = c0 - (BS - 1) + e / NRW, row = min(max(r0 - (BS - 1) + e % NRW, 0), (int)n1);
+      const int32_t o = O[(size_t)n1 * min(max(col, 0), (int)n1 - 1) + row];
       ov[s] = (col >= 0 && col < (int)n1) ? o : 0;
     }
-  };
-  int32_t ov[NO];
-  if (idx < len) prefetch(start + idx, ov);
-  for (uint32_t l = idx; l < len; l += count) {
-    const uint32_t L = start + l, bx = L % nbx, slot = L / nbx;
-    const uint32_t b = wl[slot];
-    const WinMeta m = meta[b];
-    const uint32_t k = frame_of(bx, m.ref);
-    const uint32_t tile = (mode == 1) ? bx : bx / nk;
-    const uint32_t tr = tile % ntr, tc = tile / ntr;
-    const int r0 = (int)(tr * TR), c0 = (int)(tc * TC);
-    const int r = r0 + (int)(threadIdx.x % TR), c = c0 + (int)(threadIdx.x / TR);
-    const bool live = r < (int)nr && c < (int)nr;
-    const int32_t* Lk = list + ((size_t)b * nw + k) * p_sel;  // frame k's patches in bucket order
-    // bucket place j: the static w_0 row and v_0[k] at W0k + j * M, V0k[j],
-    // the patch's round record (d_0, kept ranks) at Rk + 2 j; patch id: d_r of
-    // further ranks at Ds[id * nw + r]; in the U variant's factors w_r at
-    // Ws + id * fstride + r * M (W^T) and v_r[k] at Vk + id * fstride + r * sr
-    // (V^T)
-    const uint32_t sr = (nw + 3) & ~3u;
-    const T* W0k = W0 + ((size_t)b * nw + k) * p_sel * M;
-    const T* V0k = V0 + ((size_t)b * nw + k) * p_sel;
-    const T* Rk = rec + ((size_t)slot * nw + k) * p_sel * 2;
-    const T* Ds = dr + (size_t)slot * p_sel * nw;
-    const T* Vk = factors + (size_t)b * nvar_stored * p_sel * fstride + k;
-    const T* Ws = factors + (size_t)b * nvar_stored * p_sel * fstride + nw * sr;
-    // the voxel's 1/W and U are independent of the staging: load them first
-    const uint32_t pix = (uint32_t)min(r, (int)nr - 1) + nr * (uint32_t)min(c, (int)nr - 1), v = k * fs + pix;
-    const double iw = invw[(size_t)b * vol + v];
-    const double U = (mode == 0) ? u64[(size_t)b * vol + v] : 0.0;
 #pragma unroll
     for (int s = 0; s < NO; ++s) {
       const int e = (int)threadIdx.x + 256 * s;
       if (e < NC * NRW) sT[e] = ov[s];
     }
-    __syncthreads();
-    // the next tile's bucket offsets, in flight with this tile's staging
-    if (l + count < len) prefetch(start + l + count, ov);
-    // column ranges, packed: column lc's rows start at LDS row pre[lc]
-    int32_t pre[NC + 1];
-    bool fits = true;
-    pre[0] = 0;
+  }
+  __syncthreads();
+  // column ranges, packed: column lc's rows start at LDS row pre[lc]
+  int32_t pre[NC + 1];
+  bool fits = true;
+  pre[0] = 0;
+#pragma unroll
+  for (int lc = 0; lc < NC; ++lc) {
+    const int32_t cnt = sT[lc * NRW + NRW - 1] - sT[lc * NRW];
+    fits = fits && cnt <= COLCAP;
+    pre[lc + 1] = pre[lc] + cnt;
+  }
+  fits = fits && pre[NC] <= TOT;
+#pragma unroll
+  for (int lc = 0; lc < NC; ++lc)
+    if ((int)threadIdx.x == lc) sP[lc] = pre[lc];
+  if (fits) {
+    // patch column lc: thread -> (row lr, values q .. q + W - 1) of the
+    // column's bucket range, RPI rows per pass.  The patch indices of every
+    // row are loaded first, then every row is formed.
+    constexpr int QW = M / W, RPI = 256 / QW, NH = (COLCAP + RPI - 1) / RPI;
+    const int lr0 = (int)threadIdx.x / QW, q = ((int)threadIdx.x % QW) * W;
+    // Every load of a row is at its bucket place (the patch index, the
+    // static w_0 values of the thread, v_0[k], the round record), issued for
+    // every row before the first use; further ranks are read as needed.
+    // Row value q: the FMA chain of the reconstruction over the kept ranks,
+    // fma(d_r w_r[q], v_r[k], .), rank 0 first.
+    int32_t ids[NC][NH];
+    ST w0[NC][NH];
+    T v0[NC][NH], d0[NC][NH], kn[NC][NH];
+    // 4 x 4 patches in FP32: the 4 lanes of a row are one quad; lane s of
+    // the quad loads the row's s-th scalar (patch index, v_0[k], d_0, kept
+    // ranks) and the quad shares them by DPP broadcasts after the wait, so
+    // the loads in flight hold one register per row instead of four
+    constexpr bool QUAD = (QW == 4 && sizeof(T) == 4);
+    uint32_t sc[NC][NH];
+    const int sub = (int)threadIdx.x & 3;
 #pragma unroll
     for (int lc = 0; lc < NC; ++lc) {
-      const int32_t cnt = sT[lc * NRW + NRW - 1] - sT[lc * NRW];
-      fits = fits && cnt <= COLCAP;
-      pre[lc + 1] = pre[lc] + cnt;
+      const int32_t j0 = sT[lc * NRW], cnt = sT[lc * NRW + NRW - 1] - j0;
+#pragma unroll
+      for (int h = 0; h < NH; ++h) {
+        const int lr = lr0 + RPI * h;
+        const bool use = lr0 < RPI && lr < cnt;
+        const int32_t j = use ? j0 + lr : 0;
+        w0[lc][h] = *reinterpret_cast<const ST*>(W0k + (size_t)j * M + q);
+        if constexpr (QUAD) {
+          const void* a = sub == 0 ? (const void*)(Lk + j)
+                        : sub == 1 ? (const void*)(V0k + j)
+                                   : (const void*)(Rk + 2 * (size_t)j + (sub - 2));
+          sc[lc][h] = use ? *reinterpret_cast<const uint32_t*>(a) : (sub == 0 ? 0xffffffffu : 0u);
+        } else {
+          ids[lc][h] = use ? Lk[j] : -1;
+          v0[lc][h] = V0k[j];
+          d0[lc][h] = Rk[2 * (size_t)j];
+          kn[lc][h] = Rk[2 * (size_t)j + 1];
+        }
+      }
     }
-    fits = fits && pre[NC] <= TOT;
 #pragma unroll
-    for (int lc = 0; lc < NC; ++lc)
-      if ((int)threadIdx.x == lc) sP[lc] = pre[lc];
-    if (fits) {
-      // patch column lc: thread -> (row lr, values q .. q + W - 1) of the
-      // column's bucket range, RPI rows per pass.  The patch indices of every
-      // row are loaded first, then every row is formed.
-      constexpr int QW = M / W, RPI = 256 / QW, NH = (COLCAP + RPI - 1) / RPI;
-      const int lr0 = (int)threadIdx.x / QW, q = ((int)threadIdx.x % QW) * W;
-      // Every load of a row is at its bucket place (the patch index, the
-      // static w_0 values of the thread, v_0[k], the round record), issued for
-      // every row before the first use; further ranks are read as needed.
-      // Row value q: the FMA chain of the reconstruction over the kept ranks,
-      // fma(d_r w_r[q], v_r[k], .), rank 0 first.
-      int32_t ids[NC][NH];
-      ST w0[NC][NH];
-      T v0[NC][NH], d0[NC][NH], kn[NC][NH];
-      // 4 x 4 patches in FP32: the 4 lanes of a row are one quad; lane s of
-      // the quad loads the row's s-th scalar (patch index, v_0[k], d_0, kept
-      // ranks) and the quad shares them by DPP broadcasts after the wait, so
-      // the loads in flight hold one register per row instead of four
-      constexpr bool QUAD = (QW == 4 && sizeof(T) == 4);
-      uint32_t sc[NC][NH];
-      const int sub = (int)threadIdx.x & 3;
+    for (int lc = 0; lc < NC; ++lc) {
 #pragma unroll
-      for (int lc = 0; lc < NC; ++lc) {
-        const int32_t j0 = sT[lc * NRW], cnt = sT[lc * NRW + NRW - 1] - j0;
+      for (int h = 0; h < NH; ++h) {
+        if constexpr (QUAD) {  // every lane of the quad is active here
+          const int x = (int)sc[lc][h];
+          ids[lc][h] = __builtin_amdgcn_mov_dpp(x, 0x00, 0xF, 0xF, false);  // quad_perm [0,0,0,0]
+          v0[lc][h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0x55, 0xF, 0xF, false));
+          d0[lc][h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0xAA, 0xF, 0xF, false));
+          kn[lc][h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0xFF, 0xF, 0xF, false));
+        }
+        if (ids[lc][h] >= 0) {
+          const int n = (int)kn[lc][h];
+          const int32_t id = ids[lc][h];
+          ST row;
+          T* ac = reinterpret_cast<T*>(&row);
+          const T* ww = reinterpret_cast<const T*>(&w0[lc][h]);
 #pragma unroll
-        for (int h = 0; h < NH; ++h) {
+          for (int w = 0; w < W; ++w) ac[w] = (n > 0) ? fma(d0[lc][h] * ww[w], v0[lc][h], T(0)) : T(0);
+          for (int rk = 1; rk < n; ++rk) {
+            const T vk = Vk[(size_t)id * fstride + (size_t)rk * sr];
+            const T d = Ds[(size_t)id * nw + rk];
+            const ST wv = *reinterpret_cast<const ST*>(Ws + (size_t)id * fstride + (size_t)rk * M + q);
+            const T* wk = reinterpret_cast<const T*>(&wv);
+#pragma unroll
+            for (int w = 0; w < W; ++w) ac[w] = fma(d * wk[w], vk, ac[w]);
+          }
           const int lr = lr0 + RPI * h;
-          const bool use = lr0 < RPI && lr < cnt;
-          const int32_t j = use ? j0 + lr : 0;
-          w0[lc][h] = *reinterpret_cast<const ST*>(W0k + (size_t)j * M + q);
-          if constexpr (QUAD) {
-            const void* a = sub == 0 ? (const void*)(Lk + j)
-                          : sub == 1 ? (const void*)(V0k + j)
-                                     : (const void*)(Rk + 2 * (size_t)j + (sub - 2));
-            sc[lc][h] = use ? *reinterpret_cast<const uint32_t*>(a) : (sub == 0 ? 0xffffffffu : 0u);
-          } else {
-            ids[lc][h] = use ? Lk[j] : -1;
-            v0[lc][h] = V0k[j];
-            d0[lc][h] = Rk[2 * (size_t)j];
-            kn[lc][h] = Rk[2 * (size_t)j + 1];
-          }
-        }
-      }
 #pragma unroll
-      for (int lc = 0; lc < NC; ++lc) {
-#pragma unroll
-        for (int h = 0; h < NH; ++h) {
-          if constexpr (QUAD) {  // every lane of the quad is active here
-            const int x = (int)sc[lc][h];
-            ids[lc][h] = __builtin_amdgcn_mov_dpp(x, 0x00, 0xF, 0xF, false);  // quad_perm [0,0,0,0]
-            v0[lc][h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0x55, 0xF, 0xF, false));
-            d0[lc][h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0xAA, 0xF, 0xF, false));
-            kn[lc][h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0xFF, 0xF, 0xF, false));
-          }
-          if (ids[lc][h] >= 0) {
-            const int n = (int)kn[lc][h];
-            const int32_t id = ids[lc][h];
-            ST row;
-            T* ac = reinterpret_cast<T*>(&row);
-            const T* ww = reinterpret_cast<const T*>(&w0[lc][h]);
-#pragma unroll
-            for (int w = 0; w < W; ++w) ac[w] = (n > 0) ? fma(d0[lc][h] * ww[w], v0[lc][h], T(0)) : T(0);
-            for (int rk = 1; rk < n; ++rk) {
-              const T vk = Vk[(size_t)id * fstride + (size_t)rk * sr];
-              const T d = Ds[(size_t)id * nw + rk];
-              const ST wv = *reinterpret_cast<const ST*>(Ws + (size_t)id * fstride + (size_t)rk * M + q);
-              const T* wk = reinterpret_cast<const T*>(&wv);
-#pragma unroll
-              for (int w = 0; w < W; ++w) ac[w] = fma(d * wk[w], vk, ac[w]);
-            }
-            const int lr = lr0 + RPI * h;
-#pragma unroll
-            for (int w = 0; w < W; ++w) sB[(q + w) * RS + pre[lc] + lr] = ac[w];
-          }
+          for (int w = 0; w < W; ++w) sB[(q + w) * RS + pre[lc] + lr] = ac[w];
         }
       }
     }
-    __syncthreads();
-    double a_q = 0, a_h = 0, a_hu = 0, a_u = 0, a_u2 = 0;
-    if (live) {
-      double acc = 0.0;
-      // bucket bounds bnd[cb][e] of positions (r - BS + 1 + e, c - cb); e = BS
-      // is the end of row r.  cb0: the voxel's column base cell.
-      const int tb = ((c - c0) + BS - 1) * NRW + (r - r0);
-      int32_t bnd[BS][BS + 1];
-      int base[BS];  // bucket index j -> LDS row j + base[cb]
+  }
+  __syncthreads();
+  double a_q = 0, a_h = 0, a_hu = 0, a_u = 0, a_u2 = 0;
+  if (live) {
+    double acc = 0.0;
+    // bucket bounds bnd[cb][e] of positions (r - BS + 1 + e, c - cb); e = BS
+    // is the end of row r.  cb0: the voxel's column base cell.
+    const int tb = ((c - c0) + BS - 1) * NRW + (r - r0);
+    int32_t bnd[BS][BS + 1];
+    int base[BS];  // bucket index j -> LDS row j + base[cb]
+#pragma unroll
+    for (int cb = 0; cb < BS; ++cb) {
+#pragma unroll
+      for (int e = 0; e <= BS; ++e) bnd[cb][e] = sT[tb - cb * NRW + e];
+      base[cb] = sP[(c - c0) + BS - 1 - cb] - sT[tb - (r - r0) - cb * NRW];
+    }
+    if (fits) {
+      bool multi = false;
 #pragma unroll
       for (int cb = 0; cb < BS; ++cb) {
 #pragma unroll
-        for (int e = 0; e <= BS; ++e) bnd[cb][e] = sT[tb - cb * NRW + e];
-        base[cb] = sP[(c - c0) + BS - 1 - cb] - sT[tb - (r - r0) - cb * NRW];
+        for (int ra = 0; ra < BS; ++ra) {
+          const int e = BS - 1 - ra;  // position r - ra
+          const int32_t j0 = bnd[cb][e], n = bnd[cb][e + 1] - j0;
+          // an empty bucket reads the first row of the next bucket or the
+          // plane's pad element: in bounds, and not added
+          const T f = sB[(ra + BS * cb) * RS + j0 + base[cb]];
+          acc += (double)((n > 0) ? f : T(0));
+          multi = multi || n > 1;
+        }
       }
-      if (fits) {
-        bool multi = false;
+      if (__any(multi)) {
 #pragma unroll
         for (int cb = 0; cb < BS; ++cb) {
 #pragma unroll
           for (int ra = 0; ra < BS; ++ra) {
-            const int e = BS - 1 - ra;  // position r - ra
-            const int32_t j0 = bnd[cb][e], n = bnd[cb][e + 1] - j0;
-            // an empty bucket reads the first row of the next bucket or the
-            // plane's pad element: in bounds, and not added
-            const T f = sB[(ra + BS * cb) * RS + j0 + base[cb]];
-            acc += (double)((n > 0) ? f : T(0));
-            multi = multi || n > 1;
+            const int e = BS - 1 - ra;
+            for (int32_t j = bnd[cb][e] + 1; j < bnd[cb][e + 1]; ++j)
+              acc += (double)sB[(ra + BS * cb) * RS + j + base[cb]];
           }
         }
-        if (__any(multi)) {
+      }
+    } else {
 #pragma unroll
-          for (int cb = 0; cb < BS; ++cb) {
+      for (int cb = 0; cb < BS; ++cb)
 #pragma unroll
-            for (int ra = 0; ra < BS; ++ra) {
-              const int e = BS - 1 - ra;
-              for (int32_t j = bnd[cb][e] + 1; j < bnd[cb][e + 1]; ++j)
-                acc += (double)sB[(ra + BS * cb) * RS + j + base[cb]];
+        for (int ra = 0; ra < BS; ++ra) {
+          const int e = BS - 1 - ra, qv = ra + BS * cb;
+          for (int32_t j = bnd[cb][e]; j < bnd[cb][e + 1]; ++j) {
+            // element qv of the patch's row, formed as in the staging
+            const int32_t id = Lk[j];
+            const T* R = Rk + 2 * (size_t)j;
+            const int n = (int)R[1];
+            T x = T(0);
+            for (int rk = 0; rk < n; ++rk) {
+              const T ar = rk ? Ds[(size_t)id * nw + rk] * Ws[(size_t)id * fstride + (size_t)rk * M + qv]
+                              : R[0] * W0k[(size_t)j * M + qv];
+              const T vk = rk ? Vk[(size_t)id * fstride + (size_t)rk * sr] : V0k[j];
+              x = fma(ar, vk, x);
             }
+            acc += (double)x;
           }
         }
-      } else {
-#pragma unroll
-        for (int cb = 0; cb < BS; ++cb)
-#pragma unroll
-          for (int ra = 0; ra < BS; ++ra) {
-            const int e = BS - 1 - ra, qv = ra + BS * cb;
-            for (int32_t j = bnd[cb][e]; j < bnd[cb][e + 1]; ++j) {
-              // element qv of the patch's row, formed as in the staging
-              const int32_t id = Lk[j];
-              const T* R = Rk + 2 * (size_t)j;
-              const int n = (int)R[1];
-              T x = T(0);
-              for (int rk = 0; rk < n; ++rk) {
-                const T ar = rk ? Ds[(size_t)id * nw + rk] * Ws[(size_t)id * fstride + (size_t)rk * M + qv]
-                                : R[0] * W0k[(size_t)j * M + qv];
-                const T vk = rk ? Vk[(size_t)id * fstride + (size_t)rk * sr] : V0k[j];
-                x = fma(ar, vk, x);
-              }
-              acc += (double)x;
-            }
-          }
-      }
-      const double uh = acc * iw;
-      if (mode == 0) {
-        a_q = uh * uh;
-        a_h = uh;
-        a_hu = uh * U;
-        a_u = U;
-        a_u2 = U * U;
-      } else if (mode == 1) {
-        out[(size_t)m.out * fs + pix] = uh * m.uMax;
-      } else {
-        out[v] = uh;
-      }
     }
+    const double uh = acc * iw;
     if (mode == 0) {
-      for (int o = 32; o > 0; o >>= 1) {
-        a_q += __shfl_xor(a_q, o);
-        a_h += __shfl_xor(a_h, o);
-        a_hu += __shfl_xor(a_hu, o);
-        a_u += __shfl_xor(a_u, o);
-        a_u2 += __shfl_xor(a_u2, o);
-      }
-      const int w = threadIdx.x / 64;
-      if ((threadIdx.x & 63) == 0) {
-        sm[0][w] = a_q;
-        sm[1][w] = a_h;
-        sm[2][w] = a_hu;
-        sm[3][w] = a_u;
-        sm[4][w] = a_u2;
-      }
-      __syncthreads();
-      if (threadIdx.x < 5)
-        partial[((size_t)slot * nbx + bx) * 5 + threadIdx.x] =
-            ((sm[threadIdx.x][0] + sm[threadIdx.x][1]) + sm[threadIdx.x][2]) + sm[threadIdx.x][3];
+      a_q = uh * uh;
+      a_h = uh;
+      a_hu = uh * U;
+      a_u = U;
+      a_u2 = U * U;
+    } else if (mode == 1) {
+      out[(size_t)m.out * fs + pix] = uh * m.uMax;
+    } else {
+      out[v] = uh;
     }
-    __syncthreads();  // sT, sP, sB and sm are the next tile's
   }
+  if (mode != 0) return;
+  for (int o = 32; o > 0; o >>= 1) {
+    a_q += __shfl_xor(a_q, o);
+    a_h += __shfl_xor(a_h, o);
+    a_hu += __shfl_xor(a_hu, o);
+    a_u += __shfl_xor(a_u, o);
+    a_u2 += __shfl_xor(a_u2, o);
+  }
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) {
+    sm[0][w] = a_q;
+    sm[1][w] = a_h;
+    sm[2][w] = a_hu;
+    sm[3][w] = a_u;
+    sm[4][w] = a_u2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5)
+    partial[((size_t)slot * gridDim.x + bx) * 5 + threadIdx.x] =
+        ((sm[threadIdx.x][0] + sm[threadIdx.x][1]) + sm[threadIdx.x][2]) + sm[threadIdx.x][3];
 }
 
 // per slot: (sum Uhat^2, sum U, sum U^2, sum Uhat U, sum d2 (U2p - 2 Uhat + U2m), sum Uhat)
@@ -2051,48 +2039,25 @@ uint32_t gather_blocks(const Shape& sh, int mode, uint32_t nk) {
   return ((mode == 1 ? sh.fs : sh.vol) + 255) / 256;
 }
 
-// Blocks of `kernel` (256 threads) resident on the current device at once,
-// from the occupancy query, once per kernel and device.
-template <class K>
-uint32_t resident_blocks(K kernel) {
-  static std::mutex mu;
-  static std::map<std::pair<const void*, int>, uint32_t> cache;
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 1024;
-  const auto key = std::make_pair((const void*)kernel, dev);
-  std::lock_guard<std::mutex> g(mu);
-  auto it = cache.find(key);
-  if (it != cache.end()) return it->second;
-  int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
-  const uint32_t n = (uint32_t)(per_cu * cus);
-  cache.emplace(key, n);
-  return n;
-}
-
 hipError_t launch_gather(const GatherSrc& src, const int32_t* off, const double* invw, const double* u64,
                          const WinMeta* meta, const uint32_t* wl, uint32_t na, const Shape& sh, int mode, double* out,
                          double* partial, hipStream_t s, uint32_t nk) {
   const uint32_t nblk = gather_blocks(sh, mode, nk);
   const uint32_t nkk = (mode == 1) ? 1 : gather_frames(sh, nk);
-  const uint32_t total = nblk * na;  // logical tiles of the persistent tiled gather
 #define PG_GT(B)                                                                                                 \
   case B:                                                                                                      \
     if (sh.fp64)                                                                                               \
-      hipLaunchKernelGGL((k_gather_tile<B, double>),                                                           \
-                         dim3(std::min(total, resident_blocks(k_gather_tile<B, double>))), dim3(256), 0, s,      \
-                         (const double*)src.factors, (const double*)src.dr, (const double*)src.rec,            \
-                         (const double*)src.W0, (const double*)src.V0, src.list, off, invw, u64, meta, wl,     \
-                         sh.nr, sh.fs, sh.nw, sh.vol, sh.n1, sh.p_sel, sh.fstride, src.nvar_stored, mode, nkk, \
-                         nblk, total, out, partial);                                                           \
+      hipLaunchKernelGGL((k_gather_tile<B, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)src.factors, \
+                         (const double*)src.dr, (const double*)src.rec, (const double*)src.W0,                 \
+                         (const double*)src.V0, src.list, off, invw, u64, meta, wl,                            \
+                         sh.nr, sh.fs, sh.nw,                                                                  \
+                         sh.vol, sh.n1, sh.p_sel, sh.fstride, src.nvar_stored, mode, nkk, out, partial);       \
     else                                                                                                       \
-      hipLaunchKernelGGL((k_gather_tile<B, float>),                                                            \
-                         dim3(std::min(total, resident_blocks(k_gather_tile<B, float>))), dim3(256), 0, s,       \
-                         (const float*)src.factors, (const float*)src.dr, (const float*)src.rec,               \
-                         (const float*)src.W0, (const float*)src.V0, src.list, off, invw, u64, meta, wl,       \
-                         sh.nr, sh.fs, sh.nw, sh.vol, sh.n1, sh.p_sel, sh.fstride, src.nvar_stored, mode, nkk, \
-                         nblk, total, out, partial);                                                           \
+      hipLaunchKernelGGL((k_gather_tile<B, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)src.factors,  \
+                         (const float*)src.dr, (const float*)src.rec, (const float*)src.W0,                    \
+                         (const float*)src.V0, src.list, off, invw, u64, meta, wl,                             \
+                         sh.nr, sh.fs, sh.nw,                                                                  \
+                         sh.vol, sh.n1, sh.p_sel, sh.fstride, src.nvar_stored, mode, nkk, out, partial);       \
     break;
 #define PG_GA(B)                                                                                                 \
   case B:                                                                                                      \

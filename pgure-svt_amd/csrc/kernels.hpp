@@ -156,6 +156,20 @@ hipError_t launch_features(const float* factors, uint32_t nvar_stored, uint32_t 
 
 // fill n 32-bit words with v (graph-capturable, unlike a memset node)
 hipError_t launch_fill32(void* p, uint64_t n, uint32_t v, hipStream_t s);
+// rows of `width` 32-bit words, batch b: dst row r = src row r * step
+// (batch strides src_batch and dst_batch in words)
+hipError_t launch_copy_rows(const void* src, void* dst, uint32_t rows, uint32_t width, uint32_t step,
+                            size_t src_batch, size_t dst_batch, uint32_t nb, hipStream_t s);
+
+// Noise estimator's quadtree on the GPU (noise_gpu.hip).  Split tests of
+// blocks [nblocks][3] (row, column, size) of frames frames [nf] of the FP64
+// cube X (n x n frames) normalised by scale: stat [nf][nblocks] = the F
+// statistic of SplitBlockQ.  Leaf statistics of leaves [nleaves][4] (frame,
+// row, column, size): out [nleaves][2] = (robust mean, robust variance).
+hipError_t launch_split_stats(const double* X, uint32_t n, double scale, const uint32_t* frames, uint32_t nf,
+                              const uint32_t* blocks, uint32_t nblocks, double* stat, hipStream_t s);
+hipError_t launch_leaf_stats(const double* X, uint32_t n, double scale, const uint32_t* leaves, uint32_t nleaves,
+                             double* out, hipStream_t s);
 // lambda grid: G candidates per window from the sampled patches' median
 // singular value; then risk of every candidate from the sums [nb*G][6] and
 // the argmin into meta[b].lambda (risk [nb*G] kept)

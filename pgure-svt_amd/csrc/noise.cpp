@@ -305,12 +305,12 @@ bool split_block(const double* F, uint32_t n, uint32_t i0, uint32_t j0, uint32_t
 
 // QuadTree (src/noise.hpp:419-458): after a split the recursion starts at the
 // previous last node, not at the first new child.
-void quadtree(const double* F, uint32_t n, std::vector<std::array<uint32_t, 3>>& tree, std::vector<uint32_t>& del,
+void quadtree(const SplitTest& split, std::vector<std::array<uint32_t, 3>>& tree, std::vector<uint32_t>& del,
               uint32_t part) {
   const auto node = tree[part];
   const uint32_t i = node[0], j = node[1];
   uint32_t s = node[2];
-  if (!split_block(F, n, i, j, s)) return;
+  if (!split(i, j, s)) return;
   s /= 2;
   const uint32_t last = (uint32_t)tree.size() - 1;
   tree.push_back({i, j, s});
@@ -318,7 +318,7 @@ void quadtree(const double* F, uint32_t n, std::vector<std::array<uint32_t, 3>>&
   tree.push_back({i, j + s, s});
   tree.push_back({i + s, j + s, s});
   del.push_back(part);
-  for (uint32_t it = last; it < last + 4; ++it) quadtree(F, n, tree, del, it);
+  for (uint32_t it = last; it < last + 4; ++it) quadtree(split, tree, del, it);
 }
 
 // ConvolveFIR (src/noise.hpp:395-417) values, in any order (only their
@@ -390,14 +390,27 @@ double compute_mode(const std::vector<double>& A) {
 
 }  // namespace
 
-FrameLeaves frame_leaves(const double* F, uint32_t n) {
+double split_threshold(uint32_t N) {
+  if (N <= 8) return -1.0;
+  for (int k = 0; k < 12; ++k)
+    if (kDof[k] == N) return kF0025[k];
+  return -1.0;
+}
+
+std::vector<std::array<uint32_t, 3>> quadtree_leaves(uint32_t n, const SplitTest& split) {
   std::vector<std::array<uint32_t, 3>> tree(1, {0u, 0u, n});
   std::vector<uint32_t> del;
-  quadtree(F, n, tree, del, 0);
+  quadtree(split, tree, del, 0);
   std::sort(del.begin(), del.end());
   del.erase(std::unique(del.begin(), del.end()), del.end());
   if (!del.empty())
     for (size_t k = del.size() - 1; k > 0; --k) tree.erase(tree.begin() + del[k]);  // del[0] is never shed
+  return tree;
+}
+
+FrameLeaves frame_leaves(const double* F, uint32_t n) {
+  const std::vector<std::array<uint32_t, 3>> tree =
+      quadtree_leaves(n, [&](uint32_t i, uint32_t j, uint32_t N) { return split_block(F, n, i, j, N); });
   FrameLeaves out;
   out.mean.reserve(tree.size());
   out.var.reserve(tree.size());

@@ -1,7 +1,9 @@
 // noise.hpp -- host Poisson-Gaussian noise estimator of the PGURE-SVT path.
 #pragma once
 
+#include <array>
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <vector>
@@ -30,6 +32,19 @@ void estimate_noise(const double* u, uint32_t n, uint32_t nt, uint32_t method, d
 
 // Leaf statistics of one normalised n x n frame.
 FrameLeaves frame_leaves(const double* f, uint32_t n);
+
+// Split test of the quadtree: split(i, j, N) for the N x N block at row i,
+// column j of the frame.
+using SplitTest = std::function<bool(uint32_t, uint32_t, uint32_t)>;
+
+// QuadTree (src/noise.hpp:55-100, 419-458): the leaves (row, column, size)
+// in the reference's order, its repeated nodes included, for a split test.
+std::vector<std::array<uint32_t, 3>> quadtree_leaves(uint32_t n, const SplitTest& split);
+
+// F-test threshold of SplitBlockQ for N x N blocks (src/noise.hpp:173-221),
+// or a negative value when such a block is never split (N <= 8, or N not in
+// the table).
+double split_threshold(uint32_t N);
 
 // The window-level part of the estimate (robust line fit and the noise
 // method) from the leaf statistics of its frames, in frame order.

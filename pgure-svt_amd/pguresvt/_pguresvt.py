@@ -71,6 +71,7 @@ def load():
         L.pgsvt_window_singular_values.argtypes = [vp, C.c_int, u32, u32, u32, P, u32, vp, C.POINTER(u32)]
         L.pgsvt_noise_estimate.argtypes = [vp, u32, u32, u32, u32, dp, dp, dp]
         L.pgsvt_delta2_signs.argtypes = [i64, C.c_uint64, vp]
+        L.pgsvt_frame_leaves.argtypes = [vp, u32, C.c_int, u32, vp, vp, C.POINTER(u32)]
         _lib = L
     return _lib
 

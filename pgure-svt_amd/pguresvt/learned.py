@@ -94,7 +94,7 @@ def _bind_pipeline(L):
         L.pgsvt_denoise_range.argtypes = [vp, C.c_int, u32, u32, u32, P, u32, u32, vp, vp]
         L.pgsvt_learned_features.argtypes = [vp, C.c_int, u32, u32, u32, P, u32, u32, vp, vp]
         L.pgsvt_denoise_learned.argtypes = [vp, C.c_int, u32, u32, u32, P, C.POINTER(_Model), vp, vp]
-        L.pgsvt_set_learned_fallback.argtypes = [C.c_double, C.c_double]
+        L.pgsvt_set_learned_fallback.argtypes = [C.c_double, C.c_double, C.c_uint32]
         L.pgsvt_learned_fallbacks.argtypes = []
         L._learned_pipeline_bound = True
     return L
@@ -202,16 +202,17 @@ def default_model_path():
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), "lambda_model.npz")
 
 
-FALLBACK_DELTA, FALLBACK_GAP = 0.1, 0.01  # the library's defaults
+FALLBACK_DELTA, FALLBACK_GAP, FALLBACK_REFINE = 0.1, 0.01, 2  # the library's defaults
 
 
-def set_fallback(delta=FALLBACK_DELTA, gap=FALLBACK_GAP):
+def set_fallback(delta=FALLBACK_DELTA, gap=FALLBACK_GAP, refine=FALLBACK_REFINE):
     """The learned path's probe (pgsvt_set_learned_fallback): risks at
     lambda / (1 + delta), lambda and lambda (1 + delta); a window whose
-    predicted relative risk gap exceeds `gap` is searched.  gap <= 0: no
-    probe, every proposal kept.  Process-wide."""
+    predicted relative risk gap exceeds `gap` moves to the probe's estimate
+    of the minimum and is probed again, up to `refine` times, then
+    searched.  gap <= 0: no probe, every proposal kept.  Process-wide."""
     L = _bind_pipeline(_lib())
-    _P.check(L.pgsvt_set_learned_fallback(float(delta), float(gap)))
+    _P.check(L.pgsvt_set_learned_fallback(float(delta), float(gap), int(refine)))
 
 
 def last_fallbacks():

@@ -88,6 +88,19 @@ def noise_estimate(u, method=4, alpha=-1.0, mu=-1.0, sigma=-1.0):
     return a.value, m.value, s.value
 
 
+def frame_leaves(frame, on_gpu):
+    """Robust (mean, variance) of the quadtree leaves of one normalised
+    square frame, on the host or the GPU (pgsvt_frame_leaves)."""
+    f = np.asfortranarray(frame, dtype=np.float64)
+    n = f.shape[0]
+    cnt = C.c_uint32(0)
+    cap = (n // 8) ** 2 * 2 + 64
+    mean, var = np.zeros(cap), np.zeros(cap)
+    P.check(P.load().pgsvt_frame_leaves(f.ctypes.data, n, int(on_gpu), cap, mean.ctypes.data, var.ctypes.data,
+                                        C.byref(cnt)))
+    return mean[:cnt.value].copy(), var[:cnt.value].copy()
+
+
 def delta2_signs(seed, n):
     out = np.zeros(n, dtype=np.uint8)
     P.check(P.load().pgsvt_delta2_signs(seed, n, out.ctypes.data))

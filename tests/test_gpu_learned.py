@@ -135,7 +135,7 @@ def test_probe_sends_poor_proposals_to_the_search(gpu, seq, no_probe):
     _, est0 = Lr.denoise_range(seq, 0, 20, **PARAMS)
     model = _random_model(n_in, float(np.log(30 * np.median(est0[:, 0]))))
     _, prop = Lr.denoise_learned(seq, model, **PARAMS)  # probe off: the proposals
-    Lr.set_fallback()
+    Lr.set_fallback(refine=0)  # no refinement: every failing window is searched
     Y, est = Lr.denoise_learned(seq, model, **PARAMS)
     n_fb = Lr.last_fallbacks()
     moved = np.flatnonzero(est[:, 0] != prop[:, 0])
@@ -165,9 +165,32 @@ def test_probe_keeps_near_optimal_proposals(gpu, seq, no_probe):
     assert Lr.last_fallbacks() <= 10, Lr.last_fallbacks()
 
 
+def test_probe_refines_poor_proposals(gpu, seq, no_probe):
+    # With refinement (the default), a window the probe fails moves to the
+    # probe's estimate of the minimum: every window whose lambda changed ends
+    # with a lower PGURE risk than its proposal's, and its frame is the
+    # fixed-lambda driver's at its final lambda.
+    n_in = Lr.n_features(15)
+    _, est0 = Lr.denoise_range(seq, 0, 20, **PARAMS)
+    model = _random_model(n_in, float(np.log(30 * np.median(est0[:, 0]))))
+    _, prop = Lr.denoise_learned(seq, model, **PARAMS)
+    Lr.set_fallback()
+    Y, est = Lr.denoise_learned(seq, model, **PARAMS)
+    moved = np.flatnonzero(est[:, 0] != prop[:, 0])
+    assert len(moved) > 0
+    for t in moved[:6]:
+        r = Lr.window_risk(seq, int(t), est[t, 1:], [est[t, 0], prop[t, 0]], **PARAMS)
+        assert r[0] < r[1], (t, r)
+    t = int(moved[0])
+    Yt, _ = Lr.denoise_range(seq, t, t + 1, **dict(PARAMS, optimize_pgure=0, lambda1=float(est[t, 0])))
+    assert np.array_equal(Y[:, :, t], Yt[:, :, 0])
+
+
 def test_fallback_setter_rejects_bad_values(gpu):
     with pytest.raises(RuntimeError, match="delta"):
         Lr.set_fallback(0.0, 0.01)
     with pytest.raises(RuntimeError, match="finite"):
         Lr.set_fallback(0.1, float("nan"))
+    with pytest.raises(RuntimeError, match="refinements"):
+        Lr.set_fallback(0.1, 0.01, 17)
     Lr.set_fallback()
