@@ -79,9 +79,6 @@ SURVEY_ALGO = {
 #                                         (8); plus per kept rank its singular value and divergence
 #                                         factor (8) and, for the U variant's further ranks, d_r
 #                                         written (4 / 3 per counted rank)
-#   static_rows         bucket place      the patch index (4), the patch's w_0 row (4 m) and v_0[k]
-#                                         (4) read, the row and v_0[k] written in bucket order
-#                                         (4 m + 4), once per window
 #   gather_risk         voxel             every staged row read once (fp32): per (patch, frame)
 #                                         its static row (4 m) and v_0[k] (4), its round record (8)
 #                                         and patch index (4), P n / NVOX of them per voxel; U
@@ -91,7 +88,6 @@ ALGO = {
     "svd_jacobi": ("valu", 14 * M * NW**2 + 8 * NW**3),
     "divergence_factors": ("hbm", 4.0 * NW * (NW + M) + 4.0 * NW + 4.0 * NW / 3 + 4.0 * NVOX / (3 * P_SEL)),
     "eval_reconstruct": ("hbm", EVAL_FIXED),
-    "static_rows": ("hbm", 8.0 * M + 12.0),
     "gather_risk": ("hbm", (4.0 * M + 16.0) * P_SEL * NW / NVOX + 16),
 }
 
@@ -321,7 +317,6 @@ STAGE_KERNELS = {
     "svd_jacobi": ("pgsvt::k_svd<",),
     "divergence_factors": ("pgsvt::k_divq<",),
     "eval_reconstruct": ("pgsvt::k_round_prep<", "pgsvt::k_eval<"),
-    "static_rows": ("pgsvt::k_static_rows<",),
     "gather_risk": ("pgsvt::k_gather_",),
 }
 

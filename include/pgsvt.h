@@ -189,7 +189,7 @@ int pgsvt_learned_features(const void* X, int dtype, uint32_t n_rows,
  * to the search's bounds [0, max(100, start)]).  Each proposal is probed
  * (pgsvt_set_learned_fallback): the window's PGURE risk at lambda / (1 +
  * delta), lambda and lambda (1 + delta), from the perturbed decompositions
- * of every 8th patch and the reconstruction sums of 5 of the window's
+ * of every 16th patch and the reconstruction sums of 5 of the window's
  * frames.  Where the parabola through the three predicts a relative risk
  * gap above the limit, lambda moves to the parabola's minimum and is
  * probed again, up to `refine` times; a window still above the limit is

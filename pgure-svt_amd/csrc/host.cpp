@@ -68,7 +68,7 @@ uint32_t g_probe_refine = 2;
 thread_local int g_last_fallbacks = 0;
 // probe sample: every kProbeStride-th selected patch in all three variants,
 // the data terms over kProbeFrames frames spread over the window
-constexpr uint32_t kProbeStride = 16, kProbeFrames = 3;
+constexpr uint32_t kProbeStride = 16, kProbeFrames = 5;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -915,21 +915,18 @@ struct Pipeline {
     });
     // units: matrices decomposed
     prof->run("svd_jacobi", (double)nb * nvar_run * sh.p_sel, [&] {
-      // rotation threshold |c| > tol sqrt(ab): 1e-6 in FP32, 1e-12 in FP64
+      // rotation threshold |c| > tol sqrt(ab): 1e-6 in FP32, 1e-12 in FP64;
+      // the U variant also lays out the fused evaluation's static rows
+      const bool fused = fused_eval(sh);
       hipchk(launch_svd(uvar.p, pos.p, sel_d.p, nb, nvar_run, nvar, sh, factors.p, 30, sh.fp64 ? 1e-12 : 1e-6,
-                        stats.p, s, svd_mapping),
+                        stats.p, s, svd_mapping, fused ? inv.p : nullptr, fused ? w0b.p : nullptr,
+                        fused ? v0b.p : nullptr),
              "svd");
     });
     if (optimize && nvar_run == 3) {
       // units: matrices
       prof->run("divergence_factors", (double)nb * nvar_run * sh.p_sel, [&] {
         hipchk(launch_divq(factors.p, pos.p, sel_d.p, dw.p, nb, nvar_run, sh, divq.p, s), "divergence factors");
-      });
-    }
-    if (fused_eval(sh)) {
-      // units: bucket places
-      prof->run("static_rows", (double)nb * sh.nw * sh.p_sel, [&] {
-        hipchk(launch_static_rows(factors.p, list.p, nb, nvar, sh, w0b.p, v0b.p, s), "static rows");
       });
     }
   }
@@ -2193,10 +2190,9 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     hipchk(launch_weights(L.off, L.d2, 1, sh, d2neg, d2pos, L.invw, L.dw, s), "weights");
     hipchk(launch_fill32(L.stats, 1, 0, s), "clear stats");
     hipchk(launch_svd(L.uvar, L.pos, L.sel, 1, nvar, nvar, sh, L.factors, 30, 1e-6, L.stats, s,
-                      g_svd_mapping.load() == 1 ? 1 : 0),
+                      g_svd_mapping.load() == 1 ? 1 : 0, L.inv, L.w0, L.v0),
            "svd");
     if (optimize) hipchk(launch_divq(L.factors, L.pos, L.sel, L.dw, 1, nvar, sh, L.divq, s), "divergence factors");
-    if (fused_eval(sh)) hipchk(launch_static_rows(L.factors, L.list, 1, nvar, sh, L.w0, L.v0, s), "static rows");
     double alpha = alpha0, mu = mu0, sigma = sigma0, lambda = lambda0;
     if (optimize) {
       // noise estimate and start point from the normalised window on the host

@@ -523,7 +523,9 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
                                              const uint32_t* __restrict__ sel, uint32_t nvar, uint32_t nvar_stored,
                                              uint32_t p_sel, uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw,
                                              uint32_t vol, uint32_t lim, uint32_t fstride, int max_sweeps, T tol2,
-                                             T* __restrict__ factors, uint32_t* __restrict__ stats) {
+                                             T* __restrict__ factors, uint32_t* __restrict__ stats,
+                                             const int32_t* __restrict__ place, T* __restrict__ W0,
+                                             T* __restrict__ V0) {
   constexpr int M = BS * BS, R = (M + L - 1) / L, SRN = (N + 3) & ~3;
   const int n = RT ? (int)nw : N;
   const int SR = RT ? ((n + 3) & ~3) : SRN;
@@ -618,6 +620,11 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
   T* F = factors + (((size_t)b * nvar_stored + var) * p_sel + p) * fstride;
   T* FW = F + n * SR;
   T* FS = FW + n * M;
+  // The U variant also writes the static rows of the fused evaluation: at
+  // the patch's bucket place j_k = place[b][k][p] of every frame k, W0 its
+  // rank-0 row of W^T and V0 v_0[k] (the values stored in its factors)
+  const bool rows0 = W0 != nullptr && var == 0;
+  T w0[R];
   int rank[N];
   T inv[N];
   T amax = a[0];
@@ -636,8 +643,20 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
     for (int h = 0; h < R; ++h) {
       const int q = lane + L * h;
       if (q < M) FW[rank[l] * M + q] = x[h][l];
+      if (rank[l] == 0) w0[h] = x[h][l];
     }
     if (lane == 0) FS[rank[l]] = sqrt(a[l]);
+  }
+  if (rows0) {
+    for (int k = 0; k < n; ++k) {
+      const int32_t j = place[((size_t)b * nw + k) * p_sel + p];
+      T* o = W0 + (((size_t)b * nw + k) * p_sel + j) * M;
+#pragma unroll
+      for (int h = 0; h < R; ++h) {
+        const int q = lane + L * h;
+        if (q < M) o[q] = w0[h];
+      }
+    }
   }
   // V^T, column by column: element k of v_l = X^T w_l / s_l^2 from column
   // k of X (reloaded), stored by lane k % L
@@ -647,6 +666,7 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
     if (k < n) {
       const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
       const uint32_t row = min((uint32_t)(pk & 0xffff), lim), col = min((uint32_t)(pk >> 16), lim);
+      const int32_t jk = rows0 ? place[((size_t)b * nw + k) * p_sel + p] : 0;
       T xk[R];
 #pragma unroll
       for (int h = 0; h < R; ++h) {
@@ -660,7 +680,10 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
 #pragma unroll
         for (int h = 1; h < R; ++h) pv = fma(xk[h], x[h][l], pv);
         const T vk = group_sum<L>(pv) * inv[l];
-        if (lane == k % L) F[rank[l] * SR + k] = vk;
+        if (lane == k % L) {
+          F[rank[l] * SR + k] = vk;
+          if (rows0 && rank[l] == 0) V0[((size_t)b * nw + k) * p_sel + jk] = vk;
+        }
       }
     } else {
 #pragma unroll
@@ -1220,7 +1243,7 @@ __global__ __launch_bounds__(256) void k_divq(const T* __restrict__ factors, con
 // (d_0, kept-rank count) at the patch's bucket place in every frame, rec
 // [slot][k][inv[b][k][p]], and the factors d_r of further kept ranks to dr
 // [slot][p][r].  The gather forms the patch's rows from them, the window's
-// static rows (k_static_rows: w_0 and v_0[k] at every bucket place) and, for
+// static rows (w_0 and v_0[k] at every bucket place, written by k_svd) and, for
 // further ranks, w_r and v_r from the factors, as it stages them, so the
 // reconstruction itself is never written.  With PGURE (qdiv set) the thread
 // also adds the patch's divergence sum sum_r (d1_r q1_r - 2 d0_r q0_r +
@@ -1299,31 +1322,6 @@ __global__ __launch_bounds__(256) void k_round_prep(const T* __restrict__ factor
     if (threadIdx.x == 0) partial[(size_t)slot * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
   }
   if (kept) block_count(kept + (size_t)slot * gridDim.x + blockIdx.x, kr);
-}
-
-// The lambda-independent part of every patch row the fused gather forms,
-// once per window after the SVD, in the gather's access order: at bucket
-// place j of frame k (list[b][k][j] = patch id), W0 [b][k][j][m] = w_0 of
-// the patch (row 0 of its W^T) and V0 [b][k][j] = v_0[k].  A lambda
-// evaluation then forms the rank-0 row as fma(d_0 w_0[q], v_0[k], 0), the
-// FMA chain of the reconstruction, with d_0 from the patch's round record.
-template <int BS, class T>
-__global__ __launch_bounds__(256) void k_static_rows(const T* __restrict__ factors, const int32_t* __restrict__ list,
-                                                     uint32_t nvar_stored, uint32_t p_sel, uint32_t nw,
-                                                     uint32_t fstride, T* __restrict__ W0, T* __restrict__ V0) {
-  constexpr int M = BS * BS;
-  const uint32_t b = blockIdx.z, k = blockIdx.y;
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= p_sel) return;
-  const size_t at = ((size_t)b * nw + k) * p_sel + j;
-  const int32_t id = list[at];
-  const uint32_t sr = (nw + 3) & ~3u;
-  const T* F = factors + ((size_t)b * nvar_stored * p_sel + id) * fstride;  // the U variant's factors
-  const T* w = F + nw * sr;                                                 // row 0 of W^T
-  T* o = W0 + at * M;
-#pragma unroll
-  for (int q = 0; q < M; ++q) o[q] = w[q];
-  V0[at] = F[k];
 }
 
 // ----------------------------------------------------------- gather --
@@ -1444,9 +1442,10 @@ __global__ __launch_bounds__(256) void k_gather_direct(const T* __restrict__ B0,
 // cover the tile sit in one contiguous bucket range per patch column.  The
 // block forms those patches' rows for frame k, B[q][k] = sum over kept ranks
 // r of fma(d_r w_r[q], v_r[k], .) (d_r from k_round_prep's records, w_0 and
-// v_0[k] from k_static_rows, further ranks from the factors; the FMA chain of
-// the reconstruction, in rank order), and stores them in LDS, the column ranges packed one after another (TOT rows
-// per tile), one plane per patch row (lanes on neighbouring voxels read
+// v_0[k] from the SVD's static rows, further ranks from the factors; the
+// FMA chain of the reconstruction, in rank order), and stores them in LDS,
+// the column ranges packed one after another (TOT rows per tile), one plane
+// per patch row (lanes on neighbouring voxels read
 // neighbouring words).  Each thread then sums its voxel: first the leading
 // patch of each of its bs x bs buckets (one LDS read and a select, no
 // branch), then, only in waves where a bucket holds several patches, the
@@ -1821,7 +1820,7 @@ bool svd_shape_supported(uint32_t bs, uint32_t nw, bool fp64) {
 
 hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
                       uint32_t nvar_stored, const Shape& sh, void* factors, int max_sweeps, double tol,
-                      uint32_t* stats, hipStream_t s, int mapping) {
+                      uint32_t* stats, hipStream_t s, int mapping, const int32_t* place, void* W0, void* V0) {
   if (nvar > nvar_stored) return hipErrorInvalidValue;
   const uint32_t lim = sh.nr - sh.bs;
   const uint32_t groups = nvar * sh.p_sel;
@@ -1830,7 +1829,7 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
     const dim3 grid((groups * L + 255) / 256, nb);                                                             \
     hipLaunchKernelGGL((k_svd<B, N, L, float>), grid, dim3(256), 0, s, (const float*)uvar, pos, sel, nvar,     \
                        nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps,            \
-                       (float)(tol * tol), (float*)factors, stats);                                            \
+                       (float)(tol * tol), (float*)factors, stats, place, (float*)W0, (float*)V0);             \
     return hipGetLastError();                                                                                  \
   }
 #define PG_SVD(B, N, LE, L)                                                                                      \
@@ -1841,7 +1840,7 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
       const dim3 grid((groups * LD + 255) / 256, nb);                                                          \
       hipLaunchKernelGGL((k_svd<B, N, LD, double>), grid, dim3(256), 0, s, (const double*)uvar, pos, sel, nvar, \
                          nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps, tol * tol, \
-                         (double*)factors, stats);                                                             \
+                         (double*)factors, stats, place, (double*)W0, (double*)V0);                            \
       return hipGetLastError();                                                                                \
     }                                                                                                          \
     PG_SVD_F32(B, N, L)                                                                                        \
@@ -1865,7 +1864,7 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
     const dim3 grid((groups * L + 255) / 256, nb);                                                             \
     hipLaunchKernelGGL((k_svd<B, NM, L, float, true>), grid, dim3(256), 0, s, (const float*)uvar, pos, sel,    \
                        nvar, nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps,      \
-                       (float)(tol * tol), (float*)factors, stats);                                            \
+                       (float)(tol * tol), (float*)factors, stats, place, (float*)W0, (float*)V0);             \
     return hipGetLastError();                                                                                  \
   }
   PG_GENERIC(PG_SVDG)
@@ -1898,26 +1897,6 @@ hipError_t launch_evaluation(const Shape& sh, const void* factors, const void* d
 }
 
 size_t round_record_bytes(const Shape& sh) { return (size_t)sh.nw * sh.p_sel * 2 * (sh.fp64 ? 8 : 4); }
-
-hipError_t launch_static_rows(const void* factors, const int32_t* list, uint32_t nb, uint32_t nvar_stored,
-                              const Shape& sh, void* W0, void* V0, hipStream_t s) {
-  const dim3 grid((sh.p_sel + 255) / 256, sh.nw, nb);
-#define PG_SR(B)                                                                                                 \
-  case B:                                                                                                      \
-    if (sh.fp64)                                                                                               \
-      hipLaunchKernelGGL((k_static_rows<B, double>), grid, dim3(256), 0, s, (const double*)factors, list,       \
-                         nvar_stored, sh.p_sel, sh.nw, sh.fstride, (double*)W0, (double*)V0);                  \
-    else                                                                                                       \
-      hipLaunchKernelGGL((k_static_rows<B, float>), grid, dim3(256), 0, s, (const float*)factors, list,         \
-                         nvar_stored, sh.p_sel, sh.nw, sh.fstride, (float*)W0, (float*)V0);                    \
-    break;
-  switch (sh.bs) {
-    PG_SR(2) PG_SR(3) PG_SR(4)
-    default: return hipErrorInvalidValue;
-  }
-#undef PG_SR
-  return hipGetLastError();
-}
 
 hipError_t launch_round_prep(const void* factors, const void* qdiv, const int32_t* inv, const WinMeta* meta,
                              const uint32_t* wl, uint32_t na, uint32_t nvar_stored, const Shape& sh, int exp_w,

@@ -43,10 +43,14 @@ hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs
 // SVD of the first nvar of the nvar_stored variants of every window (uvar
 // [nb][nvar_stored][vol] -> factors [nb][nvar_stored][p_sel][fstride]).
 // mapping 0: the default lanes per matrix of the shape, 1: the alternative
-// (FP32 only; see svd_lanes)
+// (FP32 only; see svd_lanes).  W0 set (patches up to 4 x 4): the U variant
+// also writes the static rows of the fused evaluation, W0 [nb][nw][p_sel][m]
+// and V0 [nb][nw][p_sel] at the bucket places place [nb][nw][p_sel] (the
+// buckets' inverse).
 hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
                       uint32_t nvar_stored, const Shape& sh, void* factors, int max_sweeps, double tol,
-                      uint32_t* stats, hipStream_t s, int mapping = 0);
+                      uint32_t* stats, hipStream_t s, int mapping = 0, const int32_t* place = nullptr,
+                      void* W0 = nullptr, void* V0 = nullptr);
 uint32_t svd_lanes(uint32_t bs, uint32_t nw, int mapping);
 // position buckets of every (window, frame): cnt and off are [nb][nw][n1*n1
 // (+1)], list [nb][nw][p_sel]
@@ -77,14 +81,12 @@ hipError_t launch_divq(const void* factors, const int32_t* pos, const uint32_t* 
 // (round_record_bytes per slot), the factors d_r of further kept
 // ranks into dr [na][p_sel][nw]; with PGURE (qdiv = divq) the divergence
 // sums into partial [na][eval_blocks]), then launch_gather, which forms the
-// patch rows from them, the window's static rows (launch_static_rows, once
-// per window after the SVD: W0 [nb][nw][p_sel][m], V0 [nb][nw][p_sel] in
-// bucket order) and the factors as it gathers.  Larger patches: launch_eval
+// patch rows from them, the window's static rows (written by launch_svd: W0
+// [nb][nw][p_sel][m], V0 [nb][nw][p_sel] in bucket order) and the factors
+// as it gathers.  Larger patches: launch_eval
 // writes the rows to B0, launch_gather reads them.
 bool fused_eval(const Shape& sh);
 size_t round_record_bytes(const Shape& sh);
-hipError_t launch_static_rows(const void* factors, const int32_t* list, uint32_t nb, uint32_t nvar_stored,
-                              const Shape& sh, void* W0, void* V0, hipStream_t s);
 hipError_t launch_round_prep(const void* factors, const void* qdiv, const int32_t* inv, const WinMeta* meta,
                              const uint32_t* wl, uint32_t na, uint32_t nvar_stored, const Shape& sh, int exp_w,
                              const double* slot_lam, void* rec, void* dr, double* partial, double* kept,
