@@ -80,7 +80,7 @@ SURVEY_ALGO = {
 #                                         factor (8) and, for the U variant's further ranks, d_r
 #                                         written (4 / 3 per counted rank)
 #   gather_risk         voxel             every staged row read once (fp32): per (patch, frame)
-#                                         its static row (4 m) and v_0[k] (4), its round record (8)
+#                                         its static row w_0 v_0[k] (4 m), its round record (8)
 #                                         and patch index (4), P n / NVOX of them per voxel; U
 #                                         (fp64) and 1/W (fp64) of the voxel read once
 EVAL_FIXED = 12.0 * NW / 3 + 8.0
@@ -88,7 +88,7 @@ ALGO = {
     "svd_jacobi": ("valu", 14 * M * NW**2 + 8 * NW**3),
     "divergence_factors": ("hbm", 4.0 * NW * (NW + M) + 4.0 * NW + 4.0 * NW / 3 + 4.0 * NVOX / (3 * P_SEL)),
     "eval_reconstruct": ("hbm", EVAL_FIXED),
-    "gather_risk": ("hbm", (4.0 * M + 16.0) * P_SEL * NW / NVOX + 16),
+    "gather_risk": ("hbm", (4.0 * M + 12.0) * P_SEL * NW / NVOX + 16),
 }
 
 

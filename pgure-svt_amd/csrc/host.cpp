@@ -672,7 +672,7 @@ struct Pipeline {
   DBuf<char> uvar, factors, B0;  // B0: reconstructed rows (patches above 4 x 4) or the factors d_r (dr)
   DBuf<int32_t> pos, cnt, off, list, inv;
   DBuf<char> nkb;  // round records per (slot, frame, bucket place) of the fused evaluation (patches up to 4 x 4)
-  DBuf<char> w0b, v0b;  // static rows of the fused evaluation: w_0 and v_0[k] per bucket place
+  DBuf<char> w0b;  // static rows of the fused evaluation: w_0 v_0[k] per bucket place
   DBuf<int16_t> jobs_d;
   uint32_t max_jobs = 0, gather_blocks = 0;
   // learned-lambda probe: the patch subsample (sh_s: p_sel = its size) with
@@ -854,7 +854,6 @@ struct Pipeline {
     if (fused_eval(sh)) {
       nkb.alloc((size_t)NB * round_record_bytes(sh));
       w0b.alloc((size_t)NB * sh.nw * sh.p_sel * sh.m * es);
-      v0b.alloc((size_t)NB * sh.nw * sh.p_sel * es);
     }
     factors.alloc((size_t)NB * nvar * sh.p_sel * sh.fstride * es);
     if (optimize) divq.alloc((size_t)NB * nvar * sh.p_sel * 2 * ((sh.nw + 3) & ~3u) * es);
@@ -919,8 +918,7 @@ struct Pipeline {
       // the U variant also lays out the fused evaluation's static rows
       const bool fused = fused_eval(sh);
       hipchk(launch_svd(uvar.p, pos.p, sel_d.p, nb, nvar_run, nvar, sh, factors.p, 30, sh.fp64 ? 1e-12 : 1e-6,
-                        stats.p, s, svd_mapping, fused ? inv.p : nullptr, fused ? w0b.p : nullptr,
-                        fused ? v0b.p : nullptr),
+                        stats.p, s, svd_mapping, fused ? inv.p : nullptr, fused ? w0b.p : nullptr),
              "svd");
     });
     if (optimize && nvar_run == 3) {
@@ -939,7 +937,6 @@ struct Pipeline {
     g.dr = B0.p;
     g.rec = nkb.p;
     g.W0 = w0b.p;
-    g.V0 = v0b.p;
     g.list = list.p;
     g.nvar_stored = nvar;
     return g;
@@ -2086,7 +2083,7 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     double *fmax, *sums, *Xw, *Zw, *u64, *w64, *invw, *partial, *partial_b;
     WinMeta* meta;
     int16_t* jobs;
-    float *uvar, *dw, *B0, *factors, *divq, *w0, *v0;
+    float *uvar, *dw, *B0, *factors, *divq, *w0;
     int32_t *pos, *cnt, *off, *list, *inv;
     char* rec;
   } L{};
@@ -2114,7 +2111,6 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     L.B0 = c.take<float>((size_t)g.Nw * sh.p_sel * (fused_eval(sh) ? 1 : sh.m));
     L.rec = fused_eval(sh) ? c.take<char>(round_record_bytes(sh)) : nullptr;
     L.w0 = fused_eval(sh) ? c.take<float>((size_t)g.Nw * sh.p_sel * sh.m) : nullptr;
-    L.v0 = fused_eval(sh) ? c.take<float>((size_t)g.Nw * sh.p_sel) : nullptr;
     L.factors = c.take<float>((size_t)nvar * sh.p_sel * sh.fstride);
     L.divq = optimize ? c.take<float>((size_t)nvar * sh.p_sel * 2 * sr) : nullptr;
     L.partial = c.take<double>(eblk);
@@ -2147,7 +2143,6 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
   gsrc.dr = L.B0;
   gsrc.rec = L.rec;
   gsrc.W0 = L.w0;
-  gsrc.V0 = L.v0;
   gsrc.list = L.list;
   gsrc.nvar_stored = nvar;
   std::vector<double> fmax(2 * g.Nw), sums(6), hw;
@@ -2190,7 +2185,7 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     hipchk(launch_weights(L.off, L.d2, 1, sh, d2neg, d2pos, L.invw, L.dw, s), "weights");
     hipchk(launch_fill32(L.stats, 1, 0, s), "clear stats");
     hipchk(launch_svd(L.uvar, L.pos, L.sel, 1, nvar, nvar, sh, L.factors, 30, 1e-6, L.stats, s,
-                      g_svd_mapping.load() == 1 ? 1 : 0, L.inv, L.w0, L.v0),
+                      g_svd_mapping.load() == 1 ? 1 : 0, L.inv, L.w0),
            "svd");
     if (optimize) hipchk(launch_divq(L.factors, L.pos, L.sel, L.dw, 1, nvar, sh, L.divq, s), "divergence factors");
     double alpha = alpha0, mu = mu0, sigma = sigma0, lambda = lambda0;

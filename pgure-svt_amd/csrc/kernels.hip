@@ -524,8 +524,7 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
                                              uint32_t p_sel, uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw,
                                              uint32_t vol, uint32_t lim, uint32_t fstride, int max_sweeps, T tol2,
                                              T* __restrict__ factors, uint32_t* __restrict__ stats,
-                                             const int32_t* __restrict__ place, T* __restrict__ W0,
-                                             T* __restrict__ V0) {
+                                             const int32_t* __restrict__ place, T* __restrict__ W0) {
   constexpr int M = BS * BS, R = (M + L - 1) / L, SRN = (N + 3) & ~3;
   const int n = RT ? (int)nw : N;
   const int SR = RT ? ((n + 3) & ~3) : SRN;
@@ -621,8 +620,8 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
   T* FW = F + n * SR;
   T* FS = FW + n * M;
   // The U variant also writes the static rows of the fused evaluation: at
-  // the patch's bucket place j_k = place[b][k][p] of every frame k, W0 its
-  // rank-0 row of W^T and V0 v_0[k] (the values stored in its factors)
+  // the patch's bucket place j_k = place[b][k][p] of every frame k, its
+  // rank-0 row w_0 v_0[k] (from the values stored in its factors)
   const bool rows0 = W0 != nullptr && var == 0;
   T w0[R];
   int rank[N];
@@ -647,17 +646,7 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
     }
     if (lane == 0) FS[rank[l]] = sqrt(a[l]);
   }
-  if (rows0) {
-    for (int k = 0; k < n; ++k) {
-      const int32_t j = place[((size_t)b * nw + k) * p_sel + p];
-      T* o = W0 + (((size_t)b * nw + k) * p_sel + j) * M;
-#pragma unroll
-      for (int h = 0; h < R; ++h) {
-        const int q = lane + L * h;
-        if (q < M) o[q] = w0[h];
-      }
-    }
-  }
+
   // V^T, column by column: element k of v_l = X^T w_l / s_l^2 from column
   // k of X (reloaded), stored by lane k % L
 #pragma unroll
@@ -680,9 +669,14 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
 #pragma unroll
         for (int h = 1; h < R; ++h) pv = fma(xk[h], x[h][l], pv);
         const T vk = group_sum<L>(pv) * inv[l];
-        if (lane == k % L) {
-          F[rank[l] * SR + k] = vk;
-          if (rows0 && rank[l] == 0) V0[((size_t)b * nw + k) * p_sel + jk] = vk;
+        if (lane == k % L) F[rank[l] * SR + k] = vk;
+        if (rows0 && rank[l] == 0) {  // every lane holds vk
+          T* o = W0 + (((size_t)b * nw + k) * p_sel + jk) * M;
+#pragma unroll
+          for (int h = 0; h < R; ++h) {
+            const int q = lane + L * h;
+            if (q < M) o[q] = w0[h] * vk;
+          }
         }
       }
     } else {
@@ -1243,7 +1237,7 @@ __global__ __launch_bounds__(256) void k_divq(const T* __restrict__ factors, con
 // (d_0, kept-rank count) at the patch's bucket place in every frame, rec
 // [slot][k][inv[b][k][p]], and the factors d_r of further kept ranks to dr
 // [slot][p][r].  The gather forms the patch's rows from them, the window's
-// static rows (w_0 and v_0[k] at every bucket place, written by k_svd) and, for
+// static rows (w_0 v_0[k] at every bucket place, written by k_svd) and, for
 // further ranks, w_r and v_r from the factors, as it stages them, so the
 // reconstruction itself is never written.  With PGURE (qdiv set) the thread
 // also adds the patch's divergence sum sum_r (d1_r q1_r - 2 d0_r q0_r +
@@ -1440,10 +1434,10 @@ __global__ __launch_bounds__(256) void k_gather_direct(const T* __restrict__ B0,
 // Tiled gather for patches up to 4 x 4, fused with the reconstruction.  A
 // block covers a tile of TR x TC voxels of one frame k; the patches that can
 // cover the tile sit in one contiguous bucket range per patch column.  The
-// block forms those patches' rows for frame k, B[q][k] = sum over kept ranks
-// r of fma(d_r w_r[q], v_r[k], .) (d_r from k_round_prep's records, w_0 and
-// v_0[k] from the SVD's static rows, further ranks from the factors; the
-// FMA chain of the reconstruction, in rank order), and stores them in LDS,
+// block forms those patches' rows for frame k, B[q][k] = d_0 (w_0[q] v_0[k])
+// followed by fma(d_r w_r[q], v_r[k], .) over further kept ranks r (d_r from
+// k_round_prep's records, w_0 v_0[k] from the SVD's static rows, further
+// ranks from the factors; in rank order), and stores them in LDS,
 // the column ranges packed one after another (TOT rows per tile), one plane
 // per patch row (lanes on neighbouring voxels read
 // neighbouring words).  Each thread then sums its voxel: first the leading
@@ -1460,7 +1454,6 @@ __global__ __launch_bounds__(256) void k_gather_direct(const T* __restrict__ B0,
 template <int BS, class T>
 __global__ __launch_bounds__(256) void k_gather_tile(
     const T* __restrict__ factors, const T* __restrict__ dr, const T* __restrict__ rec, const T* __restrict__ W0,
-    const T* __restrict__ V0,
     const int32_t* __restrict__ list, const int32_t* __restrict__ off, const double* __restrict__ invw,
     const double* __restrict__ u64, const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl, uint32_t nr,
     uint32_t fs, uint32_t nw, uint32_t vol, uint32_t n1, uint32_t p_sel, uint32_t fstride, uint32_t nvar_stored,
@@ -1498,14 +1491,13 @@ __global__ __launch_bounds__(256) void k_gather_tile(
   const bool live = r < (int)nr && c < (int)nr;
   const int32_t* O = off + ((size_t)b * nw + k) * ((size_t)n1 * n1 + 1);
   const int32_t* Lk = list + ((size_t)b * nw + k) * p_sel;  // frame k's patches in bucket order
-  // bucket place j: the static w_0 row and v_0[k] at W0k + j * M, V0k[j],
-  // the patch's round record (d_0, kept ranks) at Rk + 2 j; patch id: d_r of
+  // bucket place j: the static row w_0 v_0[k] at W0k + j * M, the patch's
+  // round record (d_0, kept ranks) at Rk + 2 j; patch id: d_r of
   // further ranks at Ds[id * nw + r]; in the U variant's factors w_r at
   // Ws + id * fstride + r * M (W^T) and v_r[k] at Vk + id * fstride + r * sr
   // (V^T)
   const uint32_t sr = (nw + 3) & ~3u;
   const T* W0k = W0 + ((size_t)b * nw + k) * p_sel * M;
-  const T* V0k = V0 + ((size_t)b * nw + k) * p_sel;
   const T* Rk = rec + ((size_t)slot * nw + k) * p_sel * 2;
   const T* Ds = dr + (size_t)slot * p_sel * nw;
   const T* Vk = factors + (size_t)b * nvar_stored * p_sel * fstride + k;
@@ -1552,17 +1544,17 @@ __global__ __launch_bounds__(256) void k_gather_tile(
     constexpr int QW = M / W, RPI = 256 / QW, NH = (COLCAP + RPI - 1) / RPI;
     const int lr0 = (int)threadIdx.x / QW, q = ((int)threadIdx.x % QW) * W;
     // Every load of a row is at its bucket place (the patch index, the
-    // static w_0 values of the thread, v_0[k], the round record), issued for
+    // thread's values of the static row, the round record), issued for
     // every row before the first use; further ranks are read as needed.
-    // Row value q: the FMA chain of the reconstruction over the kept ranks,
-    // fma(d_r w_r[q], v_r[k], .), rank 0 first.
+    // Row value q: d_0 (w_0[q] v_0[k]), then the FMA chain of the
+    // reconstruction over the further kept ranks, fma(d_r w_r[q], v_r[k], .).
     int32_t ids[NC][NH];
     ST w0[NC][NH];
-    T v0[NC][NH], d0[NC][NH], kn[NC][NH];
-    // 4 x 4 patches in FP32: the 4 lanes of a row are one quad; lane s of
-    // the quad loads the row's s-th scalar (patch index, v_0[k], d_0, kept
-    // ranks) and the quad shares them by DPP broadcasts after the wait, so
-    // the loads in flight hold one register per row instead of four
+    T d0[NC][NH], kn[NC][NH];
+    // 4 x 4 patches in FP32: the 4 lanes of a row are one quad; lanes 0, 2
+    // and 3 of the quad load the row's patch index, d_0 and kept ranks, and
+    // the quad shares them by DPP broadcasts after the wait, so the loads in
+    // flight hold one register per row instead of three
     constexpr bool QUAD = (QW == 4 && sizeof(T) == 4);
     uint32_t sc[NC][NH];
     const int sub = (int)threadIdx.x & 3;
@@ -1576,13 +1568,10 @@ __global__ __launch_bounds__(256) void k_gather_tile(
         const int32_t j = use ? j0 + lr : 0;
         w0[lc][h] = *reinterpret_cast<const ST*>(W0k + (size_t)j * M + q);
         if constexpr (QUAD) {
-          const void* a = sub == 0 ? (const void*)(Lk + j)
-                        : sub == 1 ? (const void*)(V0k + j)
-                                   : (const void*)(Rk + 2 * (size_t)j + (sub - 2));
-          sc[lc][h] = use ? *reinterpret_cast<const uint32_t*>(a) : (sub == 0 ? 0xffffffffu : 0u);
+          const void* a = sub == 0 ? (const void*)(Lk + j) : (const void*)(Rk + 2 * (size_t)j + (sub - 2));
+          sc[lc][h] = (use && sub != 1) ? *reinterpret_cast<const uint32_t*>(a) : (sub == 0 ? 0xffffffffu : 0u);
         } else {
           ids[lc][h] = use ? Lk[j] : -1;
-          v0[lc][h] = V0k[j];
           d0[lc][h] = Rk[2 * (size_t)j];
           kn[lc][h] = Rk[2 * (size_t)j + 1];
         }
@@ -1595,7 +1584,6 @@ __global__ __launch_bounds__(256) void k_gather_tile(
         if constexpr (QUAD) {  // every lane of the quad is active here
           const int x = (int)sc[lc][h];
           ids[lc][h] = __builtin_amdgcn_mov_dpp(x, 0x00, 0xF, 0xF, false);  // quad_perm [0,0,0,0]
-          v0[lc][h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0x55, 0xF, 0xF, false));
           d0[lc][h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0xAA, 0xF, 0xF, false));
           kn[lc][h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0xFF, 0xF, 0xF, false));
         }
@@ -1606,7 +1594,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(
           T* ac = reinterpret_cast<T*>(&row);
           const T* ww = reinterpret_cast<const T*>(&w0[lc][h]);
 #pragma unroll
-          for (int w = 0; w < W; ++w) ac[w] = (n > 0) ? fma(d0[lc][h] * ww[w], v0[lc][h], T(0)) : T(0);
+          for (int w = 0; w < W; ++w) ac[w] = (n > 0) ? d0[lc][h] * ww[w] : T(0);
           for (int rk = 1; rk < n; ++rk) {
             const T vk = Vk[(size_t)id * fstride + (size_t)rk * sr];
             const T d = Ds[(size_t)id * nw + rk];
@@ -1674,13 +1662,10 @@ __global__ __launch_bounds__(256) void k_gather_tile(
             const int32_t id = Lk[j];
             const T* R = Rk + 2 * (size_t)j;
             const int n = (int)R[1];
-            T x = T(0);
-            for (int rk = 0; rk < n; ++rk) {
-              const T ar = rk ? Ds[(size_t)id * nw + rk] * Ws[(size_t)id * fstride + (size_t)rk * M + qv]
-                              : R[0] * W0k[(size_t)j * M + qv];
-              const T vk = rk ? Vk[(size_t)id * fstride + (size_t)rk * sr] : V0k[j];
-              x = fma(ar, vk, x);
-            }
+            T x = (n > 0) ? R[0] * W0k[(size_t)j * M + qv] : T(0);
+            for (int rk = 1; rk < n; ++rk)
+              x = fma(Ds[(size_t)id * nw + rk] * Ws[(size_t)id * fstride + (size_t)rk * M + qv],
+                      Vk[(size_t)id * fstride + (size_t)rk * sr], x);
             acc += (double)x;
           }
         }
@@ -1820,7 +1805,7 @@ bool svd_shape_supported(uint32_t bs, uint32_t nw, bool fp64) {
 
 hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
                       uint32_t nvar_stored, const Shape& sh, void* factors, int max_sweeps, double tol,
-                      uint32_t* stats, hipStream_t s, int mapping, const int32_t* place, void* W0, void* V0) {
+                      uint32_t* stats, hipStream_t s, int mapping, const int32_t* place, void* W0) {
   if (nvar > nvar_stored) return hipErrorInvalidValue;
   const uint32_t lim = sh.nr - sh.bs;
   const uint32_t groups = nvar * sh.p_sel;
@@ -1829,7 +1814,7 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
     const dim3 grid((groups * L + 255) / 256, nb);                                                             \
     hipLaunchKernelGGL((k_svd<B, N, L, float>), grid, dim3(256), 0, s, (const float*)uvar, pos, sel, nvar,     \
                        nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps,            \
-                       (float)(tol * tol), (float*)factors, stats, place, (float*)W0, (float*)V0);             \
+                       (float)(tol * tol), (float*)factors, stats, place, (float*)W0);             \
     return hipGetLastError();                                                                                  \
   }
 #define PG_SVD(B, N, LE, L)                                                                                      \
@@ -1840,7 +1825,7 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
       const dim3 grid((groups * LD + 255) / 256, nb);                                                          \
       hipLaunchKernelGGL((k_svd<B, N, LD, double>), grid, dim3(256), 0, s, (const double*)uvar, pos, sel, nvar, \
                          nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps, tol * tol, \
-                         (double*)factors, stats, place, (double*)W0, (double*)V0);                            \
+                         (double*)factors, stats, place, (double*)W0);                            \
       return hipGetLastError();                                                                                \
     }                                                                                                          \
     PG_SVD_F32(B, N, L)                                                                                        \
@@ -1864,7 +1849,7 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
     const dim3 grid((groups * L + 255) / 256, nb);                                                             \
     hipLaunchKernelGGL((k_svd<B, NM, L, float, true>), grid, dim3(256), 0, s, (const float*)uvar, pos, sel,    \
                        nvar, nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps,      \
-                       (float)(tol * tol), (float*)factors, stats, place, (float*)W0, (float*)V0);             \
+                       (float)(tol * tol), (float*)factors, stats, place, (float*)W0);             \
     return hipGetLastError();                                                                                  \
   }
   PG_GENERIC(PG_SVDG)
@@ -2028,13 +2013,13 @@ hipError_t launch_gather(const GatherSrc& src, const int32_t* off, const double*
     if (sh.fp64)                                                                                               \
       hipLaunchKernelGGL((k_gather_tile<B, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)src.factors, \
                          (const double*)src.dr, (const double*)src.rec, (const double*)src.W0,                 \
-                         (const double*)src.V0, src.list, off, invw, u64, meta, wl,                            \
+                         src.list, off, invw, u64, meta, wl,                            \
                          sh.nr, sh.fs, sh.nw,                                                                  \
                          sh.vol, sh.n1, sh.p_sel, sh.fstride, src.nvar_stored, mode, nkk, out, partial);       \
     else                                                                                                       \
       hipLaunchKernelGGL((k_gather_tile<B, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)src.factors,  \
                          (const float*)src.dr, (const float*)src.rec, (const float*)src.W0,                    \
-                         (const float*)src.V0, src.list, off, invw, u64, meta, wl,                             \
+                         src.list, off, invw, u64, meta, wl,                             \
                          sh.nr, sh.fs, sh.nw,                                                                  \
                          sh.vol, sh.n1, sh.p_sel, sh.fstride, src.nvar_stored, mode, nkk, out, partial);       \
     break;

@@ -44,13 +44,13 @@ hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs
 // [nb][nvar_stored][vol] -> factors [nb][nvar_stored][p_sel][fstride]).
 // mapping 0: the default lanes per matrix of the shape, 1: the alternative
 // (FP32 only; see svd_lanes).  W0 set (patches up to 4 x 4): the U variant
-// also writes the static rows of the fused evaluation, W0 [nb][nw][p_sel][m]
-// and V0 [nb][nw][p_sel] at the bucket places place [nb][nw][p_sel] (the
+// also writes the static rows of the fused evaluation, w_0 v_0[k] into W0
+// [nb][nw][p_sel][m] at the bucket places place [nb][nw][p_sel] (the
 // buckets' inverse).
 hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
                       uint32_t nvar_stored, const Shape& sh, void* factors, int max_sweeps, double tol,
                       uint32_t* stats, hipStream_t s, int mapping = 0, const int32_t* place = nullptr,
-                      void* W0 = nullptr, void* V0 = nullptr);
+                      void* W0 = nullptr);
 uint32_t svd_lanes(uint32_t bs, uint32_t nw, int mapping);
 // position buckets of every (window, frame): cnt and off are [nb][nw][n1*n1
 // (+1)], list [nb][nw][p_sel]
@@ -82,7 +82,7 @@ hipError_t launch_divq(const void* factors, const int32_t* pos, const uint32_t* 
 // ranks into dr [na][p_sel][nw]; with PGURE (qdiv = divq) the divergence
 // sums into partial [na][eval_blocks]), then launch_gather, which forms the
 // patch rows from them, the window's static rows (written by launch_svd: W0
-// [nb][nw][p_sel][m], V0 [nb][nw][p_sel] in bucket order) and the factors
+// [nb][nw][p_sel][m] in bucket order) and the factors
 // as it gathers.  Larger patches: launch_eval
 // writes the rows to B0, launch_gather reads them.
 bool fused_eval(const Shape& sh);
@@ -105,7 +105,6 @@ struct GatherSrc {
   const void* dr = nullptr;
   const void* rec = nullptr;
   const void* W0 = nullptr;
-  const void* V0 = nullptr;
   const int32_t* list = nullptr;
   uint32_t nvar_stored = 1;
 };
