@@ -945,9 +945,9 @@ struct Pipeline {
   // one lambda evaluation of the slots listed in wl (device), their lambdas
   // from meta or slot_lam
   void evaluate(const uint32_t* wl, uint32_t na, uint32_t nvar_run, const double* slot_lam, double* part,
-                double* kept) {
+                double* kept, uint32_t nk = 0) {
     hipchk(launch_evaluation(sh, factors.p, divq.p, meta_d.p, wl, na, nvar, nvar_run, p.exponential_weighting, inv.p,
-                             nkb.p, B0.p, part, slot_lam, kept, s),
+                             nkb.p, B0.p, part, slot_lam, kept, s, nk),
            "eval");
   }
 
@@ -1000,7 +1000,7 @@ struct Pipeline {
       hipchk(hipMemcpyAsync(wl_s.p, wl.data(), na * 4, hipMemcpyHostToDevice, s), "H2D probe list");
       hipchk(hipMemcpyAsync(lam_s.p, lam.data() + c0, na * 8, hipMemcpyHostToDevice, s), "H2D probe lambdas");
       prof->run("probe_eval", (double)na * sh.vol, [&] {
-        evaluate(wl_s.p, na, 1, lam_s.p, nullptr, nullptr);
+        evaluate(wl_s.p, na, 1, lam_s.p, nullptr, nullptr, nk);
         hipchk(launch_gather(gather_src(), off.p, invw.p, u64.p, meta_d.p, wl_s.p, na, sh, 0, nullptr, partial_b.p, s,
                              nk),
                "probe gather");
@@ -1303,10 +1303,37 @@ std::vector<FrameLeaves> gpu_frame_leaves(const double* Xd, uint32_t n, const st
   }
   const uint32_t nl = (uint32_t)(leaves.size() / 4);
   std::vector<double> stats((size_t)nl * 2);
+  // leaves above the kernel's register cache: values and Laplacians formed
+  // once, up to kLeafScratch elements in all (the rest form them per pass)
+  constexpr uint64_t kLeafScratch = (uint64_t)1 << 25;
+  std::vector<uint64_t> boff(nl, ~0ull);
+  std::vector<uint32_t> big;
+  uint64_t nbe = 0;
+  for (uint32_t k = 0; k < nl; ++k) {
+    const uint64_t N = (uint64_t)leaves[4 * k + 3] * leaves[4 * k + 3];
+    if (N > leaf_cache_elements() && nbe + N <= kLeafScratch) {
+      boff[k] = nbe;
+      big.push_back(k);
+      nbe += N;
+    }
+  }
+  DBuf<uint32_t> big_d;
+  DBuf<uint64_t> boff_d;
+  DBuf<double> vals_d, laps_d;
   leaves_d.alloc(leaves.size());
   out_d.alloc(stats.size());
+  boff_d.alloc(nl);
   hipchk(hipMemcpyAsync(leaves_d.p, leaves.data(), leaves.size() * 4, hipMemcpyHostToDevice, s), "H2D leaves");
-  hipchk(launch_leaf_stats(Xd, n, scale, leaves_d.p, nl, out_d.p, s), "leaf stats");
+  hipchk(hipMemcpyAsync(boff_d.p, boff.data(), (size_t)nl * 8, hipMemcpyHostToDevice, s), "H2D leaf offsets");
+  if (!big.empty()) {
+    big_d.alloc(big.size());
+    vals_d.alloc(nbe);
+    laps_d.alloc(nbe);
+    hipchk(hipMemcpyAsync(big_d.p, big.data(), big.size() * 4, hipMemcpyHostToDevice, s), "H2D large leaves");
+  }
+  hipchk(launch_leaf_stats(Xd, n, scale, leaves_d.p, nl, big_d.p, (uint32_t)big.size(), boff_d.p, vals_d.p, laps_d.p,
+                           out_d.p, s),
+         "leaf stats");
   hipchk(hipMemcpyAsync(stats.data(), out_d.p, stats.size() * 8, hipMemcpyDeviceToHost, s), "D2H leaf stats");
   hipchk(hipStreamSynchronize(s), "leaf stats sync");
   std::vector<FrameLeaves> out(nf);
@@ -1535,13 +1562,14 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
   // learned lambda: fixed patch sample, feature buffer, network
   std::unique_ptr<Mlp> net;
   DBuf<uint32_t> sample_d;
-  DBuf<float> feat_d, mean_d, istd_d, yout_d;
+  DBuf<float> feat_d, nz_d, mean_d, istd_d, yout_d;
   const uint32_t nlog = kFeatureSamples * g.Nw;
   if (learned || rs.mode == 3) {
     const std::vector<uint32_t> smp = feature_sample(P.sh.p_sel, kFeatureSamples);
     sample_d.alloc(smp.size());
     hipchk(hipMemcpyAsync(sample_d.p, smp.data(), smp.size() * 4, hipMemcpyHostToDevice, P.s), "H2D sample");
     feat_d.alloc((size_t)count * n_in);
+    nz_d.alloc((size_t)count * 3);
     yout_d.alloc(count);
     if (rs.mode == 1) {
       net.reset(new Mlp(n_in, rs.model->hidden, P.s));
@@ -1851,8 +1879,10 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
           }
         }
       }
-      hipchk(hipMemcpy2DAsync(feat_d.p + nlog, (size_t)n_in * 4, nz.data(), 12, 12, nb, hipMemcpyHostToDevice, P.s),
-             "H2D noise features");
+      // one contiguous upload, then into the feature rows on the device (a
+      // strided upload from pageable memory is split into a copy per row)
+      hipchk(hipMemcpyAsync(nz_d.p, nz.data(), nz.size() * 4, hipMemcpyHostToDevice, P.s), "H2D noise features");
+      hipchk(launch_copy_rows(nz_d.p, feat_d.p + nlog, 1, 3, 1, 3, n_in, nb, P.s), "noise features");
       if (rs.mode == 2) {
         hipchk(hipMemcpyAsync(rs.features + (size_t)(t0 - rs.tb) * n_in, feat_d.p, (size_t)nb * n_in * 4,
                               hipMemcpyDeviceToHost, P.s),

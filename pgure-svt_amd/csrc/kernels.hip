@@ -1254,7 +1254,7 @@ __global__ __launch_bounds__(256) void k_round_prep(const T* __restrict__ factor
                                                     uint32_t p_sel, uint32_t nw, uint32_t fstride, int exp_w,
                                                     const double* __restrict__ slot_lam, T* __restrict__ rec,
                                                     T* __restrict__ dr, double* __restrict__ partial,
-                                                    double* __restrict__ kept) {
+                                                    double* __restrict__ kept, uint32_t nk) {
   constexpr int M = BS * BS;
   __shared__ double red[4];
   const int n = RT ? (int)nw : N;
@@ -1297,14 +1297,22 @@ __global__ __launch_bounds__(256) void k_round_prep(const T* __restrict__ factor
     }
     if (v == 0 && rec) {
       // d_0 and the kept-rank count (a small integer, exact in T) at the
-      // patch's place in every frame
+      // patch's place in every frame, or in the nk < nw frames the gather
+      // reads (frame i (nw - 1) / (nk - 1), i < nk)
       typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type T2;
       T2 rv;
       rv.x = d0;
       rv.y = (T)r;
       const int32_t* I = inv + (size_t)b * nw * p_sel + p;
       T2* Rb = reinterpret_cast<T2*>(rec) + (size_t)slot * nw * p_sel;
-      for (int k = 0; k < n; ++k) Rb[(size_t)k * p_sel + I[(size_t)k * p_sel]] = rv;
+      if (nk >= (uint32_t)n) {
+        for (int k = 0; k < n; ++k) Rb[(size_t)k * p_sel + I[(size_t)k * p_sel]] = rv;
+      } else {
+        for (uint32_t i = 0; i < nk; ++i) {
+          const uint32_t k = nk > 1 ? i * (nw - 1) / (nk - 1) : nw / 2;
+          Rb[(size_t)k * p_sel + I[(size_t)k * p_sel]] = rv;
+        }
+      }
     }
     kr += r;
   }
@@ -1873,10 +1881,10 @@ uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run) {
 hipError_t launch_evaluation(const Shape& sh, const void* factors, const void* divq, const WinMeta* meta,
                              const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, int exp_w,
                              const int32_t* inv, void* rec, void* rows, double* partial, const double* slot_lam,
-                             double* kept, hipStream_t s) {
+                             double* kept, hipStream_t s, uint32_t nk) {
   if (fused_eval(sh))
     return launch_round_prep(factors, nvar_run == 3 ? divq : nullptr, inv, meta, wl, na, nvar_stored, sh, exp_w,
-                             slot_lam, rec, rows, nvar_run == 3 ? partial : nullptr, kept, s);
+                             slot_lam, rec, rows, nvar_run == 3 ? partial : nullptr, kept, s, gather_frames(sh, nk));
   return launch_eval(factors, meta, wl, na, nvar_stored, nvar_run, sh, exp_w, divq, inv, rows, partial, s, slot_lam,
                      kept);
 }
@@ -1886,19 +1894,20 @@ size_t round_record_bytes(const Shape& sh) { return (size_t)sh.nw * sh.p_sel * 2
 hipError_t launch_round_prep(const void* factors, const void* qdiv, const int32_t* inv, const WinMeta* meta,
                              const uint32_t* wl, uint32_t na, uint32_t nvar_stored, const Shape& sh, int exp_w,
                              const double* slot_lam, void* rec, void* dr, double* partial, double* kept,
-                             hipStream_t s) {
+                             hipStream_t s, uint32_t nk) {
   const uint32_t nblk = eval_blocks(sh, 1);
+  if (nk == 0 || nk > sh.nw) nk = sh.nw;
 #define PG_RP(B, N, L, LS)                                                                                       \
   if (sh.bs == B && sh.nw == N) {                                                                              \
     if constexpr (B <= 4) {                                                                                    \
       if (sh.fp64)                                                                                             \
         hipLaunchKernelGGL((k_round_prep<B, N, double>), dim3(nblk, na), dim3(256), 0, s,                      \
                            (const double*)factors, (const double*)qdiv, inv, meta, wl, nvar_stored, sh.p_sel,   \
-                           sh.nw, sh.fstride, exp_w, slot_lam, (double*)rec, (double*)dr, partial, kept);       \
+                           sh.nw, sh.fstride, exp_w, slot_lam, (double*)rec, (double*)dr, partial, kept, nk);   \
       else                                                                                                     \
         hipLaunchKernelGGL((k_round_prep<B, N, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors, \
                            (const float*)qdiv, inv, meta, wl, nvar_stored, sh.p_sel, sh.nw, sh.fstride, exp_w,  \
-                           slot_lam, (float*)rec, (float*)dr, partial, kept);                                   \
+                           slot_lam, (float*)rec, (float*)dr, partial, kept, nk);                               \
       return hipGetLastError();                                                                                \
     }                                                                                                          \
   }
@@ -1911,7 +1920,7 @@ hipError_t launch_round_prep(const void* factors, const void* qdiv, const int32_
     if (sh.bs == B && nm == NM) {                                                                              \
       hipLaunchKernelGGL((k_round_prep<B, NM, float, true>), dim3(nblk, na), dim3(256), 0, s,                  \
                          (const float*)factors, (const float*)qdiv, inv, meta, wl, nvar_stored, sh.p_sel,       \
-                         sh.nw, sh.fstride, exp_w, slot_lam, (float*)rec, (float*)dr, partial, kept);           \
+                         sh.nw, sh.fstride, exp_w, slot_lam, (float*)rec, (float*)dr, partial, kept, nk);       \
       return hipGetLastError();                                                                                \
     }                                                                                                          \
   }

@@ -90,13 +90,14 @@ size_t round_record_bytes(const Shape& sh);
 hipError_t launch_round_prep(const void* factors, const void* qdiv, const int32_t* inv, const WinMeta* meta,
                              const uint32_t* wl, uint32_t na, uint32_t nvar_stored, const Shape& sh, int exp_w,
                              const double* slot_lam, void* rec, void* dr, double* partial, double* kept,
-                             hipStream_t s);
+                             hipStream_t s, uint32_t nk = 0);
 // One lambda evaluation of the listed slots, either form: `rows` is dr
-// (fused) or B0; rec is used by the fused form only.
+// (fused) or B0; rec is used by the fused form only, and holds records of
+// the frames a gather with the same nk reads (every frame when nk is 0).
 hipError_t launch_evaluation(const Shape& sh, const void* factors, const void* divq, const WinMeta* meta,
                              const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, int exp_w,
                              const int32_t* inv, void* rec, void* rows, double* partial, const double* slot_lam,
-                             double* kept, hipStream_t s);
+                             double* kept, hipStream_t s, uint32_t nk = 0);
 // what the gather reads: fused (bs <= 4) the factors, the round records,
 // dr and the static rows with the bucket lists; otherwise the rows B0
 struct GatherSrc {
@@ -167,9 +168,15 @@ hipError_t launch_copy_rows(const void* src, void* dst, uint32_t rows, uint32_t 
 // cube X (n x n frames) normalised by scale: stat [nf][nblocks] = the F
 // statistic of SplitBlockQ.  Leaf statistics of leaves [nleaves][4] (frame,
 // row, column, size): out [nleaves][2] = (robust mean, robust variance).
+// Leaves of more than leaf_cache_elements() elements: those listed in big
+// [nbig] have their values and Laplacians formed once into vals and laps
+// from element boff[leaf] (boff [nleaves]); the others (boff ~0) form them
+// again at every pass.
 hipError_t launch_split_stats(const double* X, uint32_t n, double scale, const uint32_t* frames, uint32_t nf,
                               const uint32_t* blocks, uint32_t nblocks, double* stat, hipStream_t s);
+uint32_t leaf_cache_elements();
 hipError_t launch_leaf_stats(const double* X, uint32_t n, double scale, const uint32_t* leaves, uint32_t nleaves,
+                             const uint32_t* big, uint32_t nbig, const uint64_t* boff, double* vals, double* laps,
                              double* out, hipStream_t s);
 // lambda grid: G candidates per window from the sampled patches' median
 // singular value; then risk of every candidate from the sums [nb*G][6] and

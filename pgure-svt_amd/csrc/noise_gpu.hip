@@ -16,6 +16,7 @@
 // rounding of its threshold.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "kernels.hpp"
@@ -167,27 +168,20 @@ __global__ __launch_bounds__(256) void k_split_stats(const double* __restrict__ 
 // values per thread held in registers: leaves up to 64 x 64
 constexpr uint32_t kLeafCache = 16;
 
-// RobustMeanEstimate (src/noise.hpp:238-271) and RobustVarianceEstimate of
-// the 9-point Laplacian (src/noise.hpp:232-236, 395-417) of leaf
-// (frame, i0, j0, s): out [leaf][2] = (mean, variance).  Leaf element
-// e = r + s c (column-major), value X(i0 + r, j0 + c) / scale.  Leaves of at
-// most 256 kLeafCache elements hold their values and Laplacians in
-// registers (element threadIdx.x + 256 i in slot i); larger ones recompute
-// them from the frame at every pass.
-__global__ __launch_bounds__(256) void k_leaf_stats(const double* __restrict__ X, uint32_t n, double scale,
-                                                    const uint32_t* __restrict__ leaves, double* __restrict__ out) {
+// Leaf element e = r + s c (column-major) of leaf (frame, i0, j0, s): its
+// value X(i0 + r, j0 + c) / scale and its 9-point Laplacian from the
+// wrapped 3 x 3 neighbourhood within the leaf, in the host's order (even
+// terms, odd terms, then their sum).
+struct LeafView {
+  const double* F;
+  uint32_t n, i0, j0, s;
+  double scale;
+  __device__ LeafView(const double* X, uint32_t n_, double scale_, const uint32_t* Lf)
+      : F(X + (size_t)Lf[0] * n_ * n_), n(n_), i0(Lf[1]), j0(Lf[2]), s(Lf[3]), scale(scale_) {}
+  __device__ double val(uint32_t e) const { return F[(i0 + e % s) + (size_t)n * (j0 + e / s)] / scale; }
+  __device__ double lap(uint32_t e) const {
 #pragma clang fp contract(off)
-  __shared__ double red[12];
-  __shared__ SelectLds sel;
-  const uint32_t* Lf = leaves + 4 * (size_t)blockIdx.x;
-  const double* F = X + (size_t)Lf[0] * n * n;
-  const uint32_t i0 = Lf[1], j0 = Lf[2], s = Lf[3], N = s * s;
-  const bool cached = N <= 256 * kLeafCache;
-  auto val = [&](uint32_t e) { return F[(i0 + e % s) + (size_t)n * (j0 + e / s)] / scale; };
-  // lap element y + s x from the wrapped 3 x 3 neighbourhood, in the host's
-  // order (even terms, odd terms, then their sum)
-  const double ne = (1.0 * 0.125) * -1.0, nm = -1.0 * -1.0;
-  auto lap = [&](uint32_t e) {
+    const double ne = (1.0 * 0.125) * -1.0, nm = -1.0 * -1.0;
     const uint32_t x = e / s, y = e % s;
     const uint32_t xp = (x + 1 == s) ? 1 : x + 1, xm = (x == 0) ? s - 2 : x - 1;
     const uint32_t yp = (y + 1 == s) ? 1 : y + 1, ym = (y == 0) ? s - 2 : y - 1;
@@ -203,14 +197,56 @@ __global__ __launch_bounds__(256) void k_leaf_stats(const double* __restrict__ X
     a2 += I(xp, y) * ne;
     a1 += I(xp, yp) * ne;
     return a1 + a2;
-  };
+  }
+};
+
+// Values and Laplacians of the leaves too large for the register cache,
+// computed once: leaf big[q] into vals / laps from element boff[big[q]].
+__global__ __launch_bounds__(256) void k_leaf_prep(const double* __restrict__ X, uint32_t n, double scale,
+                                                   const uint32_t* __restrict__ leaves,
+                                                   const uint32_t* __restrict__ big,
+                                                   const uint64_t* __restrict__ boff, double* __restrict__ vals,
+                                                   double* __restrict__ laps) {
+#pragma clang fp contract(off)
+  const uint32_t leaf = big[blockIdx.y];
+  const LeafView v(X, n, scale, leaves + 4 * (size_t)leaf);
+  const uint32_t N = v.s * v.s;
+  const uint64_t o = boff[leaf];
+  for (uint32_t e = blockIdx.x * 256 + threadIdx.x; e < N; e += gridDim.x * 256) {
+    vals[o + e] = v.val(e);
+    laps[o + e] = v.lap(e);
+  }
+}
+
+// RobustMeanEstimate (src/noise.hpp:238-271) and RobustVarianceEstimate of
+// the 9-point Laplacian (src/noise.hpp:232-236, 395-417) of leaf
+// (frame, i0, j0, s): out [leaf][2] = (mean, variance).  Leaves of at most
+// 256 kLeafCache elements hold their values and Laplacians in registers
+// (element threadIdx.x + 256 i in slot i); larger ones read them from vals /
+// laps at boff[leaf] (k_leaf_prep), or recompute them from the frame at
+// every pass where boff[leaf] is kNoOffset.
+constexpr uint64_t kNoOffset = ~0ull;
+__global__ __launch_bounds__(256) void k_leaf_stats(const double* __restrict__ X, uint32_t n, double scale,
+                                                    const uint32_t* __restrict__ leaves,
+                                                    const uint64_t* __restrict__ boff,
+                                                    const double* __restrict__ vals,
+                                                    const double* __restrict__ laps, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  __shared__ double red[12];
+  __shared__ SelectLds sel;
+  const LeafView lv(X, n, scale, leaves + 4 * (size_t)blockIdx.x);
+  const uint32_t N = lv.s * lv.s;
+  const bool cached = N <= 256 * kLeafCache;
+  const uint64_t o = cached ? kNoOffset : boff[blockIdx.x];
+  const double* V = vals + (o == kNoOffset ? 0 : o);
+  const double* Lp = laps + (o == kNoOffset ? 0 : o);
   double cv[kLeafCache], cl[kLeafCache];
   if (cached) {
 #pragma unroll
     for (uint32_t i = 0; i < kLeafCache; ++i) {
       const uint32_t e = threadIdx.x + 256 * i;
-      cv[i] = e < N ? val(e) : 0.0;
-      cl[i] = e < N ? lap(e) : 0.0;
+      cv[i] = e < N ? lv.val(e) : 0.0;
+      cl[i] = e < N ? lv.lap(e) : 0.0;
     }
   }
   // visit the thread's values (or Laplacians, or a function g of them)
@@ -220,8 +256,11 @@ __global__ __launch_bounds__(256) void k_leaf_stats(const double* __restrict__ X
 #pragma unroll
         for (uint32_t i = 0; i < kLeafCache; ++i)
           if (threadIdx.x + 256 * i < N) f(g(of_lap ? cl[i] : cv[i]));
+      } else if (o != kNoOffset) {
+        const double* P = of_lap ? Lp : V;
+        for (uint32_t e = threadIdx.x; e < N; e += 256) f(g(P[e]));
       } else {
-        for (uint32_t e = threadIdx.x; e < N; e += 256) f(g(of_lap ? lap(e) : val(e)));
+        for (uint32_t e = threadIdx.x; e < N; e += 256) f(g(of_lap ? lv.lap(e) : lv.val(e)));
       }
     };
   };
@@ -284,10 +323,18 @@ hipError_t launch_split_stats(const double* X, uint32_t n, double scale, const u
   return hipGetLastError();
 }
 
+uint32_t leaf_cache_elements() { return 256 * kLeafCache; }
+
 hipError_t launch_leaf_stats(const double* X, uint32_t n, double scale, const uint32_t* leaves, uint32_t nleaves,
+                             const uint32_t* big, uint32_t nbig, const uint64_t* boff, double* vals, double* laps,
                              double* out, hipStream_t s) {
   if (nleaves == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_leaf_stats, dim3(nleaves), dim3(256), 0, s, X, n, scale, leaves, out);
+  if (nbig > 0) {
+    // elements per leaf above 256 kLeafCache: at least 17 chunks of 256
+    const uint32_t chunks = std::min<uint32_t>((n * n + 255) / 256, 64);
+    hipLaunchKernelGGL(k_leaf_prep, dim3(chunks, nbig), dim3(256), 0, s, X, n, scale, leaves, big, boff, vals, laps);
+  }
+  hipLaunchKernelGGL(k_leaf_stats, dim3(nleaves), dim3(256), 0, s, X, n, scale, leaves, boff, vals, laps, out);
   return hipGetLastError();
 }
 
