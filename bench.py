@@ -71,10 +71,7 @@ SURVEY_ALGO = {
 #                                         (4 n), frame positions (4 n / 3) and delta2/W of the
 #                                         window's voxels (4 NVOX / (3 P)) shared by the three
 #                                         variants of a patch
-#   eval_reconstruct    patch x variant   every datum moved once (fp32): the round record (d_0,
-#                                         kept ranks; 8 bytes) written at the patch's bucket place
-#                                         in every frame and those places read (12 n / 3 per
-#                                         variant: one record per patch), the first singular value
+#   eval_reconstruct    patch x variant   every datum moved once (fp32): the first singular value
 #                                         of each variant and the one that ends the kept prefix
 #                                         (8); plus per kept rank its singular value and divergence
 #                                         factor (8) and, for the U variant's further ranks, d_r
@@ -83,7 +80,7 @@ SURVEY_ALGO = {
 #                                         its static row w_0 v_0[k] (4 m), its round record (8)
 #                                         and patch index (4), P n / NVOX of them per voxel; U
 #                                         (fp64) and 1/W (fp64) of the voxel read once
-EVAL_FIXED = 12.0 * NW / 3 + 8.0
+EVAL_FIXED = 8.0
 ALGO = {
     "svd_jacobi": ("valu", 14 * M * NW**2 + 8 * NW**3),
     "divergence_factors": ("hbm", 4.0 * NW * (NW + M) + 4.0 * NW + 4.0 * NW / 3 + 4.0 * NVOX / (3 * P_SEL)),

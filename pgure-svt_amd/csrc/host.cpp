@@ -671,8 +671,9 @@ struct Pipeline {
   // the precision study (sh.fp64); sized in bytes
   DBuf<char> uvar, factors, B0;  // B0: reconstructed rows (patches above 4 x 4) or the factors d_r (dr)
   DBuf<int32_t> pos, cnt, off, list, inv;
-  DBuf<char> nkb;  // round records per (slot, frame, bucket place) of the fused evaluation (patches up to 4 x 4)
-  DBuf<char> w0b;  // static rows of the fused evaluation: w_0 v_0[k] per bucket place
+  // the fused evaluation's static data per window (patches up to 4 x 4):
+  // rows w_0 v_0[k] and singular pairs (s_0, s_1) per (frame, bucket place)
+  DBuf<char> w0b, s01b;
   DBuf<int16_t> jobs_d;
   uint32_t max_jobs = 0, gather_blocks = 0;
   // learned-lambda probe: the patch subsample (sh_s: p_sel = its size) with
@@ -793,8 +794,8 @@ struct Pipeline {
     b += 8ull * sh.vol * (optimize ? 2 : 1);             // 1/W, delta2/W
     b += es * sh.nw * (size_t)sh.p_sel * (fused_eval(sh) ? 1 : sh.m);  // B0 (or dr)
     if (fused_eval(sh)) {
-      b += round_record_bytes(sh);                           // round records
-      b += es * sh.nw * (size_t)sh.p_sel * (sh.m + 1);       // static rows
+      b += singular_pair_bytes(sh);                          // static singular pairs
+      b += es * sh.nw * (size_t)sh.p_sel * sh.m;             // static rows
     }
     b += es * nvar * (size_t)sh.p_sel * sh.fstride;      // factors
     if (optimize) b += es * nvar * (size_t)sh.p_sel * 2 * ((sh.nw + 3) & ~3u);  // divq
@@ -852,8 +853,8 @@ struct Pipeline {
     if (optimize) dw.alloc((size_t)NB * sh.vol * (sh.fp64 ? sizeof(double) : sizeof(float)));
     B0.alloc((size_t)NB * sh.nw * sh.p_sel * (fused_eval(sh) ? 1 : sh.m) * es);
     if (fused_eval(sh)) {
-      nkb.alloc((size_t)NB * round_record_bytes(sh));
       w0b.alloc((size_t)NB * sh.nw * sh.p_sel * sh.m * es);
+      s01b.alloc((size_t)NB * singular_pair_bytes(sh));
     }
     factors.alloc((size_t)NB * nvar * sh.p_sel * sh.fstride * es);
     if (optimize) divq.alloc((size_t)NB * nvar * sh.p_sel * 2 * ((sh.nw + 3) & ~3u) * es);
@@ -915,10 +916,12 @@ struct Pipeline {
     // units: matrices decomposed
     prof->run("svd_jacobi", (double)nb * nvar_run * sh.p_sel, [&] {
       // rotation threshold |c| > tol sqrt(ab): 1e-6 in FP32, 1e-12 in FP64;
-      // the U variant also lays out the fused evaluation's static rows
+      // the U variant also lays out the fused evaluation's static rows and
+      // singular pairs
       const bool fused = fused_eval(sh);
       hipchk(launch_svd(uvar.p, pos.p, sel_d.p, nb, nvar_run, nvar, sh, factors.p, 30, sh.fp64 ? 1e-12 : 1e-6,
-                        stats.p, s, svd_mapping, fused ? inv.p : nullptr, fused ? w0b.p : nullptr),
+                        stats.p, s, svd_mapping, fused ? inv.p : nullptr, fused ? w0b.p : nullptr,
+                        fused ? s01b.p : nullptr),
              "svd");
     });
     if (optimize && nvar_run == 3) {
@@ -935,19 +938,20 @@ struct Pipeline {
     g.B0 = B0.p;
     g.factors = factors.p;
     g.dr = B0.p;
-    g.rec = nkb.p;
+    g.S01 = s01b.p;
     g.W0 = w0b.p;
     g.list = list.p;
     g.nvar_stored = nvar;
+    g.exp_w = p.exponential_weighting;
     return g;
   }
 
   // one lambda evaluation of the slots listed in wl (device), their lambdas
   // from meta or slot_lam
   void evaluate(const uint32_t* wl, uint32_t na, uint32_t nvar_run, const double* slot_lam, double* part,
-                double* kept, uint32_t nk = 0) {
+                double* kept) {
     hipchk(launch_evaluation(sh, factors.p, divq.p, meta_d.p, wl, na, nvar, nvar_run, p.exponential_weighting, inv.p,
-                             nkb.p, B0.p, part, slot_lam, kept, s, nk),
+                             B0.p, part, slot_lam, kept, s),
            "eval");
   }
 
@@ -1000,12 +1004,13 @@ struct Pipeline {
       hipchk(hipMemcpyAsync(wl_s.p, wl.data(), na * 4, hipMemcpyHostToDevice, s), "H2D probe list");
       hipchk(hipMemcpyAsync(lam_s.p, lam.data() + c0, na * 8, hipMemcpyHostToDevice, s), "H2D probe lambdas");
       prof->run("probe_eval", (double)na * sh.vol, [&] {
-        evaluate(wl_s.p, na, 1, lam_s.p, nullptr, nullptr, nk);
-        hipchk(launch_gather(gather_src(), off.p, invw.p, u64.p, meta_d.p, wl_s.p, na, sh, 0, nullptr, partial_b.p, s,
-                             nk),
+        evaluate(wl_s.p, na, 1, lam_s.p, nullptr, nullptr);
+        GatherSrc gs = gather_src();
+        gs.slot_lam = lam_s.p;
+        hipchk(launch_gather(gs, off.p, invw.p, u64.p, meta_d.p, wl_s.p, na, sh, 0, nullptr, partial_b.p, s, nk),
                "probe gather");
-        hipchk(launch_round_prep(factors_s.p, divq_s.p, inv.p, meta_d.p, wl_s.p, na, 3, sh_s, p.exponential_weighting,
-                                 lam_s.p, nullptr, nullptr, partial_s.p, nullptr, s),
+        hipchk(launch_round_prep(factors_s.p, divq_s.p, meta_d.p, wl_s.p, na, 3, sh_s, p.exponential_weighting,
+                                 lam_s.p, nullptr, partial_s.p, nullptr, s),
                "probe divergence");
         hipchk(launch_risk_reduce(partial_s.p, eblk, partial_b.p, gblk, na, sums.p, s), "probe reduce");
       });
@@ -1818,8 +1823,10 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
         for (uint32_t c0 = 0; c0 < slots; c0 += C) {
           const uint32_t na = std::min(C, slots - c0);
           P.evaluate(wl_g.p + c0, na, 3, lam_g.p + c0, P.partial.p, nullptr);
-          hipchk(launch_gather(P.gather_src(), P.off.p, P.invw.p, P.u64.p, P.meta_d.p, wl_g.p + c0, na, P.sh, 0,
-                               nullptr, P.partial_b.p, P.s),
+          GatherSrc gs = P.gather_src();
+          gs.slot_lam = lam_g.p + c0;
+          hipchk(launch_gather(gs, P.off.p, P.invw.p, P.u64.p, P.meta_d.p, wl_g.p + c0, na, P.sh, 0, nullptr,
+                               P.partial_b.p, P.s),
                  "grid gather");
           hipchk(launch_risk_reduce(P.partial.p, eval_blocks(P.sh, 3), P.partial_b.p, P.gather_blocks, na,
                                     sums_g.p + (size_t)c0 * 6, P.s),
@@ -2115,7 +2122,7 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     int16_t* jobs;
     float *uvar, *dw, *B0, *factors, *divq, *w0;
     int32_t *pos, *cnt, *off, *list, *inv;
-    char* rec;
+    char* s01;
   } L{};
   auto carve = [&](Carve& c) {
     L.d2 = optimize ? c.take<uint8_t>(vol) : nullptr;
@@ -2139,7 +2146,7 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     L.invw = c.take<double>(vol);
     L.dw = optimize ? c.take<float>(vol) : nullptr;
     L.B0 = c.take<float>((size_t)g.Nw * sh.p_sel * (fused_eval(sh) ? 1 : sh.m));
-    L.rec = fused_eval(sh) ? c.take<char>(round_record_bytes(sh)) : nullptr;
+    L.s01 = fused_eval(sh) ? c.take<char>(singular_pair_bytes(sh)) : nullptr;
     L.w0 = fused_eval(sh) ? c.take<float>((size_t)g.Nw * sh.p_sel * sh.m) : nullptr;
     L.factors = c.take<float>((size_t)nvar * sh.p_sel * sh.fstride);
     L.divq = optimize ? c.take<float>((size_t)nvar * sh.p_sel * 2 * sr) : nullptr;
@@ -2171,7 +2178,8 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
   gsrc.B0 = L.B0;
   gsrc.factors = L.factors;
   gsrc.dr = L.B0;
-  gsrc.rec = L.rec;
+  gsrc.S01 = L.s01;
+  gsrc.exp_w = p.exponential_weighting;
   gsrc.W0 = L.w0;
   gsrc.list = L.list;
   gsrc.nvar_stored = nvar;
@@ -2215,7 +2223,7 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     hipchk(launch_weights(L.off, L.d2, 1, sh, d2neg, d2pos, L.invw, L.dw, s), "weights");
     hipchk(launch_fill32(L.stats, 1, 0, s), "clear stats");
     hipchk(launch_svd(L.uvar, L.pos, L.sel, 1, nvar, nvar, sh, L.factors, 30, 1e-6, L.stats, s,
-                      g_svd_mapping.load() == 1 ? 1 : 0, L.inv, L.w0),
+                      g_svd_mapping.load() == 1 ? 1 : 0, L.inv, L.w0, L.s01),
            "svd");
     if (optimize) hipchk(launch_divq(L.factors, L.pos, L.sel, L.dw, 1, nvar, sh, L.divq, s), "divergence factors");
     double alpha = alpha0, mu = mu0, sigma = sigma0, lambda = lambda0;
@@ -2251,7 +2259,7 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
           m.lambda = l;
           hipchk(hipMemcpyAsync(L.meta, &m, sizeof m, hipMemcpyHostToDevice, s), "H2D meta");
           hipchk(launch_evaluation(sh, L.factors, L.divq, L.meta, L.wl, 1, nvar, 3, p.exponential_weighting, L.inv,
-                                   L.rec, L.B0, L.partial, nullptr, nullptr, s),
+                                   L.B0, L.partial, nullptr, nullptr, s),
                  "eval");
           hipchk(launch_gather(gsrc, L.off, L.invw, L.u64, L.meta, L.wl, 1, sh, 0, nullptr, L.partial_b, s), "gather");
           hipchk(launch_risk_reduce(L.partial, eblk, L.partial_b, gblk, 1, L.sums, s), "reduce");
@@ -2276,8 +2284,8 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     // the output frame at the window's lambda (Pipeline::finish)
     m.lambda = lambda;
     hipchk(hipMemcpyAsync(L.meta, &m, sizeof m, hipMemcpyHostToDevice, s), "H2D meta");
-    hipchk(launch_evaluation(sh, L.factors, nullptr, L.meta, L.wl, 1, nvar, 1, p.exponential_weighting, L.inv, L.rec,
-                             L.B0, nullptr, nullptr, nullptr, s),
+    hipchk(launch_evaluation(sh, L.factors, nullptr, L.meta, L.wl, 1, nvar, 1, p.exponential_weighting, L.inv, L.B0,
+                             nullptr, nullptr, nullptr, s),
            "eval final");
     hipchk(launch_gather(gsrc, L.off, L.invw, L.u64, L.meta, L.wl, 1, sh, 1, Y, nullptr, s), "gather final");
     hipchk(hipStreamSynchronize(s), "window sync");  // m and jobs are reused by the next window

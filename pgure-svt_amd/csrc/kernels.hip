@@ -442,6 +442,19 @@ hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs
   return hipErrorInvalidValue;
 }
 
+// Shrink factor d = g(s)/s of a singular value s of a patch whose largest
+// is s0, at lambda (src/svt.hpp:135-139): (s - w)_+ / s with w =
+// |s0 exp(-lambda s^2 / 2)| (exponential weighting) or lambda.  One
+// expression, never contracted, for every kernel that forms d: the
+// evaluation and the gather must agree on each value and on which ranks
+// are kept.
+template <class T>
+__device__ __forceinline__ T shrink_factor(T s, T s0, double lam, int exp_w) {
+#pragma clang fp contract(off)
+  const T w = exp_w ? fabs(s0 * exp((T)(-0.5 * lam) * (s * s))) : (T)lam;
+  return s > T(0) ? fmax(s - w, T(0)) / s : T(0);
+}
+
 // --------------------------------------------------- batched Jacobi SVD --
 // One-sided (Hestenes) Jacobi on the m x n Casorati matrix of each patch
 // (arma::svd_econ at src/svt.hpp:111).  Output per patch, sorted by
@@ -524,7 +537,8 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
                                              uint32_t p_sel, uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw,
                                              uint32_t vol, uint32_t lim, uint32_t fstride, int max_sweeps, T tol2,
                                              T* __restrict__ factors, uint32_t* __restrict__ stats,
-                                             const int32_t* __restrict__ place, T* __restrict__ W0) {
+                                             const int32_t* __restrict__ place, T* __restrict__ W0,
+                                             T* __restrict__ S01) {
   constexpr int M = BS * BS, R = (M + L - 1) / L, SRN = (N + 3) & ~3;
   const int n = RT ? (int)nw : N;
   const int SR = RT ? ((n + 3) & ~3) : SRN;
@@ -621,9 +635,11 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
   T* FS = FW + n * M;
   // The U variant also writes the static rows of the fused evaluation: at
   // the patch's bucket place j_k = place[b][k][p] of every frame k, its
-  // rank-0 row w_0 v_0[k] (from the values stored in its factors)
+  // rank-0 row w_0 v_0[k] (from the values stored in its factors) into W0
+  // and its two largest singular values (s_0, s_1) into S01
   const bool rows0 = W0 != nullptr && var == 0;
   T w0[R];
+  T s0v = T(0), s1v = T(0);
   int rank[N];
   T inv[N];
   T amax = a[0];
@@ -644,7 +660,10 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
       if (q < M) FW[rank[l] * M + q] = x[h][l];
       if (rank[l] == 0) w0[h] = x[h][l];
     }
-    if (lane == 0) FS[rank[l]] = sqrt(a[l]);
+    const T sv = sqrt(a[l]);
+    if (lane == 0) FS[rank[l]] = sv;
+    if (rank[l] == 0) s0v = sv;
+    if (rank[l] == 1) s1v = sv;
   }
 
   // V^T, column by column: element k of v_l = X^T w_l / s_l^2 from column
@@ -656,6 +675,13 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
       const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
       const uint32_t row = min((uint32_t)(pk & 0xffff), lim), col = min((uint32_t)(pk >> 16), lim);
       const int32_t jk = rows0 ? place[((size_t)b * nw + k) * p_sel + p] : 0;
+      if (rows0 && lane == 0) {
+        typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type T2;
+        T2 sp;
+        sp.x = s0v;
+        sp.y = s1v;
+        reinterpret_cast<T2*>(S01)[((size_t)b * nw + k) * p_sel + jk] = sp;
+      }
       T xk[R];
 #pragma unroll
       for (int h = 0; h < R; ++h) {
@@ -984,10 +1010,7 @@ __global__ __launch_bounds__(256) void k_eval(
   // of the group with g > 0
   auto shrink = [&](T s, T s0) {
     T d = T(0);
-    if (valid && lane < n) {
-      const T w = exp_w ? fabs(s0 * exp((T)(-0.5 * lam) * (s * s))) : (T)lam;
-      d = s > T(0) ? fmax(s - w, T(0)) / s : T(0);
-    }
+    if (valid && lane < n) d = shrink_factor(s, s0, lam, exp_w);
     return d;
   };
   auto kept_ranks = [&](T d) {
@@ -1233,28 +1256,28 @@ __global__ __launch_bounds__(256) void k_divq(const T* __restrict__ factors, con
 // The reconstruction of a patch is B = sum_r d_r w_r v_r^T (src/svt.hpp:
 // 135-146 with src/utils.hpp:96-106), d_r = g(s_r)/s_r, w_r = X v_r = s_r u_r;
 // factors are sorted by descending s and g(s)/s grows with s, so the kept
-// ranks (d_r > 0) are a prefix.  The thread writes the patch's round record
-// (d_0, kept-rank count) at the patch's bucket place in every frame, rec
-// [slot][k][inv[b][k][p]], and the factors d_r of further kept ranks to dr
-// [slot][p][r].  The gather forms the patch's rows from them, the window's
-// static rows (w_0 v_0[k] at every bucket place, written by k_svd) and, for
-// further ranks, w_r and v_r from the factors, as it stages them, so the
-// reconstruction itself is never written.  With PGURE (qdiv set) the thread
-// also adds the patch's divergence sum sum_r (d1_r q1_r - 2 d0_r q0_r +
-// d2_r q2_r) (src/pgure.hpp:136, see k_divq) to the FP64 partial of its
-// block.  The singular values and divergence factors come from the
-// rank-major planes of k_divq, or (qdiv null) the U variant's singular
-// values from its factors.  RT: window length n = nw at run time (n <= N).
-// rec null: the divergence sums only (no records), e.g. over a subsample of
-// the patches decomposed on their own.
+// ranks (d_r > 0) are a prefix.  The gather forms d_0 and whether rank 1 is
+// kept itself, from the window's static singular pairs (s_0, s_1) that
+// k_svd writes at every bucket place; this thread writes, for a patch that
+// keeps rank 1, the factors d_r of its further kept ranks to dr
+// [slot][p][r], r >= 1, followed by a 0 when fewer than n ranks are kept.
+// The gather forms the patch's rows from d_0 and the static rows (w_0 v_0[k]
+// at every bucket place, written by k_svd) and, for further ranks, from dr
+// and w_r, v_r of the factors, as it stages them, so the reconstruction
+// itself is never written.  With PGURE (qdiv set) the thread also adds the
+// patch's divergence sum sum_r (d1_r q1_r - 2 d0_r q0_r + d2_r q2_r)
+// (src/pgure.hpp:136, see k_divq) to the FP64 partial of its block.  The
+// singular values and divergence factors come from the rank-major planes of
+// k_divq, or (qdiv null) the U variant's singular values from its factors.
+// RT: window length n = nw at run time (n <= N).  dr null: the divergence
+// sums only, e.g. over a subsample of the patches decomposed on their own.
 template <int BS, int N, class T, bool RT = false>
 __global__ __launch_bounds__(256) void k_round_prep(const T* __restrict__ factors, const T* __restrict__ qdiv,
-                                                    const int32_t* __restrict__ inv, const WinMeta* __restrict__ meta,
+                                                    const WinMeta* __restrict__ meta,
                                                     const uint32_t* __restrict__ wl, uint32_t nvar_stored,
                                                     uint32_t p_sel, uint32_t nw, uint32_t fstride, int exp_w,
-                                                    const double* __restrict__ slot_lam, T* __restrict__ rec,
-                                                    T* __restrict__ dr, double* __restrict__ partial,
-                                                    double* __restrict__ kept, uint32_t nk) {
+                                                    const double* __restrict__ slot_lam, T* __restrict__ dr,
+                                                    double* __restrict__ partial, double* __restrict__ kept) {
   constexpr int M = BS * BS;
   __shared__ double red[4];
   const int n = RT ? (int)nw : N;
@@ -1270,50 +1293,20 @@ __global__ __launch_bounds__(256) void k_round_prep(const T* __restrict__ factor
     return qdiv ? qdiv[(((size_t)b * nvar_stored + v) * 2 * SR + r) * p_sel + pa] : FW[n * M + r];
   };
   auto qval = [&](int v, int r) -> T { return qdiv[(((size_t)b * nvar_stored + v) * 2 * SR + SR + r) * p_sel + pa]; };
-  // shrink factor of rank r (the expression of the reference's threshold,
-  // rounded as in k_eval)
-  auto shrink = [&](T s, T s0) -> T {
-    const T w = exp_w ? fabs(s0 * exp((T)(-0.5 * lam) * (s * s))) : (T)lam;
-    return s > T(0) ? fmax(s - w, T(0)) / s : T(0);
-  };
   double s4 = 0.0;
   int kr = 0;  // kept ranks of the evaluated variants (profiling)
   const int nvar_run = qdiv ? 3 : 1;
-  T d0 = T(0);
   for (int v = 0; v < nvar_run && valid; ++v) {
     const T s0 = sval(v, 0);
     int r = 0;
     for (; r < n; ++r) {
       const T sr = (r == 0) ? s0 : sval(v, r);
-      const T d = shrink(sr, s0);
+      const T d = shrink_factor(sr, s0, lam, exp_w);
       if (!(d > T(0))) break;
       if (qdiv) s4 = fma((v == 0 ? -2.0 : 1.0) * (double)d, (double)qval(v, r), s4);
-      if (v == 0 && rec) {
-        if (r == 0)
-          d0 = d;
-        else
-          dr[((size_t)slot * p_sel + p) * nw + r] = d;
-      }
+      if (v == 0 && dr && r > 0) dr[((size_t)slot * p_sel + p) * nw + r] = d;
     }
-    if (v == 0 && rec) {
-      // d_0 and the kept-rank count (a small integer, exact in T) at the
-      // patch's place in every frame, or in the nk < nw frames the gather
-      // reads (frame i (nw - 1) / (nk - 1), i < nk)
-      typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type T2;
-      T2 rv;
-      rv.x = d0;
-      rv.y = (T)r;
-      const int32_t* I = inv + (size_t)b * nw * p_sel + p;
-      T2* Rb = reinterpret_cast<T2*>(rec) + (size_t)slot * nw * p_sel;
-      if (nk >= (uint32_t)n) {
-        for (int k = 0; k < n; ++k) Rb[(size_t)k * p_sel + I[(size_t)k * p_sel]] = rv;
-      } else {
-        for (uint32_t i = 0; i < nk; ++i) {
-          const uint32_t k = nk > 1 ? i * (nw - 1) / (nk - 1) : nw / 2;
-          Rb[(size_t)k * p_sel + I[(size_t)k * p_sel]] = rv;
-        }
-      }
-    }
+    if (v == 0 && dr && r > 1 && r < n) dr[((size_t)slot * p_sel + p) * nw + r] = T(0);  // end of the kept ranks
     kr += r;
   }
   if (!qdiv && !kept) return;  // uniform over the launch
@@ -1443,9 +1436,11 @@ __global__ __launch_bounds__(256) void k_gather_direct(const T* __restrict__ B0,
 // block covers a tile of TR x TC voxels of one frame k; the patches that can
 // cover the tile sit in one contiguous bucket range per patch column.  The
 // block forms those patches' rows for frame k, B[q][k] = d_0 (w_0[q] v_0[k])
-// followed by fma(d_r w_r[q], v_r[k], .) over further kept ranks r (d_r from
-// k_round_prep's records, w_0 v_0[k] from the SVD's static rows, further
-// ranks from the factors; in rank order), and stores them in LDS,
+// followed by fma(d_r w_r[q], v_r[k], .) over further kept ranks r (d_0 and
+// whether rank 1 is kept formed here from the SVD's static singular pairs,
+// d_r of further ranks from k_round_prep, w_0 v_0[k] from the SVD's static
+// rows, further ranks from the factors; in rank order), and stores them in
+// LDS,
 // the column ranges packed one after another (TOT rows per tile), one plane
 // per patch row (lanes on neighbouring voxels read
 // neighbouring words).  Each thread then sums its voxel: first the leading
@@ -1461,11 +1456,12 @@ __global__ __launch_bounds__(256) void k_gather_direct(const T* __restrict__ B0,
 //   mode 2: whole window volume of slot 0 into out
 template <int BS, class T>
 __global__ __launch_bounds__(256) void k_gather_tile(
-    const T* __restrict__ factors, const T* __restrict__ dr, const T* __restrict__ rec, const T* __restrict__ W0,
+    const T* __restrict__ factors, const T* __restrict__ dr, const T* __restrict__ S01, const T* __restrict__ W0,
     const int32_t* __restrict__ list, const int32_t* __restrict__ off, const double* __restrict__ invw,
-    const double* __restrict__ u64, const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl, uint32_t nr,
-    uint32_t fs, uint32_t nw, uint32_t vol, uint32_t n1, uint32_t p_sel, uint32_t fstride, uint32_t nvar_stored,
-    int mode, uint32_t nk, double* __restrict__ out, double* __restrict__ partial) {
+    const double* __restrict__ u64, const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl,
+    const double* __restrict__ slot_lam, int exp_w, uint32_t nr, uint32_t fs, uint32_t nw, uint32_t vol, uint32_t n1,
+    uint32_t p_sel, uint32_t fstride, uint32_t nvar_stored, int mode, uint32_t nk, double* __restrict__ out,
+    double* __restrict__ partial) {
   // TOT: staged patch rows per tile, the bucket ranges of its NC patch
   // columns packed one after another; COLCAP: rows per patch column
   constexpr int M = BS * BS, TR = 32, TC = 8, NC = TC + BS - 1, NRW = TR + BS, TOT = 448, COLCAP = 64;
@@ -1487,6 +1483,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(
   xcd_block(bx, slot);
   const uint32_t b = wl[slot];
   const WinMeta m = meta[b];
+  const double lam = slot_lam ? slot_lam[slot] : m.lambda;
   const uint32_t ntr = (nr + TR - 1) / TR;  // square frames
   // mode 0 covers nk frames of the window: all (nk = nw), or nk spread over
   // it (k_i = i (nw - 1) / (nk - 1)); mode 1 the reference frame
@@ -1500,13 +1497,13 @@ __global__ __launch_bounds__(256) void k_gather_tile(
   const int32_t* O = off + ((size_t)b * nw + k) * ((size_t)n1 * n1 + 1);
   const int32_t* Lk = list + ((size_t)b * nw + k) * p_sel;  // frame k's patches in bucket order
   // bucket place j: the static row w_0 v_0[k] at W0k + j * M, the patch's
-  // round record (d_0, kept ranks) at Rk + 2 j; patch id: d_r of
-  // further ranks at Ds[id * nw + r]; in the U variant's factors w_r at
-  // Ws + id * fstride + r * M (W^T) and v_r[k] at Vk + id * fstride + r * sr
-  // (V^T)
+  // singular pair (s_0, s_1) at Sk + 2 j; patch id: d_r of further ranks at
+  // Ds[id * nw + r] (up to a 0, or r = nw); in the U variant's factors w_r
+  // at Ws + id * fstride + r * M (W^T) and v_r[k] at Vk + id * fstride +
+  // r * sr (V^T)
   const uint32_t sr = (nw + 3) & ~3u;
   const T* W0k = W0 + ((size_t)b * nw + k) * p_sel * M;
-  const T* Rk = rec + ((size_t)slot * nw + k) * p_sel * 2;
+  const T* Sk = S01 + ((size_t)b * nw + k) * p_sel * 2;
   const T* Ds = dr + (size_t)slot * p_sel * nw;
   const T* Vk = factors + (size_t)b * nvar_stored * p_sel * fstride + k;
   const T* Ws = factors + (size_t)b * nvar_stored * p_sel * fstride + nw * sr;
@@ -1552,17 +1549,19 @@ __global__ __launch_bounds__(256) void k_gather_tile(
     constexpr int QW = M / W, RPI = 256 / QW, NH = (COLCAP + RPI - 1) / RPI;
     const int lr0 = (int)threadIdx.x / QW, q = ((int)threadIdx.x % QW) * W;
     // Every load of a row is at its bucket place (the patch index, the
-    // thread's values of the static row, the round record), issued for
+    // thread's values of the static row, the singular pair), issued for
     // every row before the first use; further ranks are read as needed.
     // Row value q: d_0 (w_0[q] v_0[k]), then the FMA chain of the
     // reconstruction over the further kept ranks, fma(d_r w_r[q], v_r[k], .).
     int32_t ids[NC][NH];
     ST w0[NC][NH];
-    T d0[NC][NH], kn[NC][NH];
+    T s0[NC][NH], s1[NC][NH];
     // 4 x 4 patches in FP32: the 4 lanes of a row are one quad; lanes 0, 2
-    // and 3 of the quad load the row's patch index, d_0 and kept ranks, and
-    // the quad shares them by DPP broadcasts after the wait, so the loads in
-    // flight hold one register per row instead of three
+    // and 3 of the quad load the row's patch index, s_0 and s_1 (the loads
+    // in flight hold one register per row instead of three).  After the
+    // wait lane 2 forms d_0 = shrink(s_0, s_0) and lane 3 shrink(s_1, s_0)
+    // in the same instructions, and the quad shares the index and the two
+    // factors by DPP broadcasts.
     constexpr bool QUAD = (QW == 4 && sizeof(T) == 4);
     uint32_t sc[NC][NH];
     const int sub = (int)threadIdx.x & 3;
@@ -1576,12 +1575,12 @@ __global__ __launch_bounds__(256) void k_gather_tile(
         const int32_t j = use ? j0 + lr : 0;
         w0[lc][h] = *reinterpret_cast<const ST*>(W0k + (size_t)j * M + q);
         if constexpr (QUAD) {
-          const void* a = sub == 0 ? (const void*)(Lk + j) : (const void*)(Rk + 2 * (size_t)j + (sub - 2));
+          const void* a = sub == 0 ? (const void*)(Lk + j) : (const void*)(Sk + 2 * (size_t)j + (sub - 2));
           sc[lc][h] = (use && sub != 1) ? *reinterpret_cast<const uint32_t*>(a) : (sub == 0 ? 0xffffffffu : 0u);
         } else {
           ids[lc][h] = use ? Lk[j] : -1;
-          d0[lc][h] = Rk[2 * (size_t)j];
-          kn[lc][h] = Rk[2 * (size_t)j + 1];
+          s0[lc][h] = Sk[2 * (size_t)j];
+          s1[lc][h] = Sk[2 * (size_t)j + 1];
         }
       }
     }
@@ -1589,27 +1588,37 @@ __global__ __launch_bounds__(256) void k_gather_tile(
     for (int lc = 0; lc < NC; ++lc) {
 #pragma unroll
       for (int h = 0; h < NH; ++h) {
+        T d0, d1;
         if constexpr (QUAD) {  // every lane of the quad is active here
           const int x = (int)sc[lc][h];
           ids[lc][h] = __builtin_amdgcn_mov_dpp(x, 0x00, 0xF, 0xF, false);  // quad_perm [0,0,0,0]
-          d0[lc][h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0xAA, 0xF, 0xF, false));
-          kn[lc][h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0xFF, 0xF, 0xF, false));
+          const T sa = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0xAA, 0xF, 0xF, false));  // [2,2,2,2]
+          // lane 2: shrink(s_0, s_0); lane 3: shrink(s_1, s_0) (lanes 0 and 1
+          // form an unused value)
+          const int d = __builtin_bit_cast(int, shrink_factor(__builtin_bit_cast(T, x), sa, lam, exp_w));
+          d0 = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0xAA, 0xF, 0xF, false));
+          d1 = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0xFF, 0xF, 0xF, false));  // [3,3,3,3]
+        } else {
+          d0 = shrink_factor(s0[lc][h], s0[lc][h], lam, exp_w);
+          d1 = shrink_factor(s1[lc][h], s0[lc][h], lam, exp_w);
         }
         if (ids[lc][h] >= 0) {
-          const int n = (int)kn[lc][h];
           const int32_t id = ids[lc][h];
           ST row;
           T* ac = reinterpret_cast<T*>(&row);
           const T* ww = reinterpret_cast<const T*>(&w0[lc][h]);
 #pragma unroll
-          for (int w = 0; w < W; ++w) ac[w] = (n > 0) ? d0[lc][h] * ww[w] : T(0);
-          for (int rk = 1; rk < n; ++rk) {
-            const T vk = Vk[(size_t)id * fstride + (size_t)rk * sr];
-            const T d = Ds[(size_t)id * nw + rk];
-            const ST wv = *reinterpret_cast<const ST*>(Ws + (size_t)id * fstride + (size_t)rk * M + q);
-            const T* wk = reinterpret_cast<const T*>(&wv);
+          for (int w = 0; w < W; ++w) ac[w] = (d0 > T(0)) ? d0 * ww[w] : T(0);
+          if (d1 > T(0)) {
+            for (int rk = 1; rk < (int)nw; ++rk) {
+              const T d = Ds[(size_t)id * nw + rk];
+              if (!(d > T(0))) break;
+              const T vk = Vk[(size_t)id * fstride + (size_t)rk * sr];
+              const ST wv = *reinterpret_cast<const ST*>(Ws + (size_t)id * fstride + (size_t)rk * M + q);
+              const T* wk = reinterpret_cast<const T*>(&wv);
 #pragma unroll
-            for (int w = 0; w < W; ++w) ac[w] = fma(d * wk[w], vk, ac[w]);
+              for (int w = 0; w < W; ++w) ac[w] = fma(d * wk[w], vk, ac[w]);
+            }
           }
           const int lr = lr0 + RPI * h;
 #pragma unroll
@@ -1668,12 +1677,17 @@ __global__ __launch_bounds__(256) void k_gather_tile(
           for (int32_t j = bnd[cb][e]; j < bnd[cb][e + 1]; ++j) {
             // element qv of the patch's row, formed as in the staging
             const int32_t id = Lk[j];
-            const T* R = Rk + 2 * (size_t)j;
-            const int n = (int)R[1];
-            T x = (n > 0) ? R[0] * W0k[(size_t)j * M + qv] : T(0);
-            for (int rk = 1; rk < n; ++rk)
-              x = fma(Ds[(size_t)id * nw + rk] * Ws[(size_t)id * fstride + (size_t)rk * M + qv],
-                      Vk[(size_t)id * fstride + (size_t)rk * sr], x);
+            const T sa = Sk[2 * (size_t)j], sb = Sk[2 * (size_t)j + 1];
+            const T d0 = shrink_factor(sa, sa, lam, exp_w);
+            T x = (d0 > T(0)) ? d0 * W0k[(size_t)j * M + qv] : T(0);
+            if (shrink_factor(sb, sa, lam, exp_w) > T(0)) {
+              for (int rk = 1; rk < (int)nw; ++rk) {
+                const T d = Ds[(size_t)id * nw + rk];
+                if (!(d > T(0))) break;
+                x = fma(d * Ws[(size_t)id * fstride + (size_t)rk * M + qv], Vk[(size_t)id * fstride + (size_t)rk * sr],
+                        x);
+              }
+            }
             acc += (double)x;
           }
         }
@@ -1813,7 +1827,7 @@ bool svd_shape_supported(uint32_t bs, uint32_t nw, bool fp64) {
 
 hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
                       uint32_t nvar_stored, const Shape& sh, void* factors, int max_sweeps, double tol,
-                      uint32_t* stats, hipStream_t s, int mapping, const int32_t* place, void* W0) {
+                      uint32_t* stats, hipStream_t s, int mapping, const int32_t* place, void* W0, void* S01) {
   if (nvar > nvar_stored) return hipErrorInvalidValue;
   const uint32_t lim = sh.nr - sh.bs;
   const uint32_t groups = nvar * sh.p_sel;
@@ -1822,7 +1836,7 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
     const dim3 grid((groups * L + 255) / 256, nb);                                                             \
     hipLaunchKernelGGL((k_svd<B, N, L, float>), grid, dim3(256), 0, s, (const float*)uvar, pos, sel, nvar,     \
                        nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps,            \
-                       (float)(tol * tol), (float*)factors, stats, place, (float*)W0);             \
+                       (float)(tol * tol), (float*)factors, stats, place, (float*)W0, (float*)S01);             \
     return hipGetLastError();                                                                                  \
   }
 #define PG_SVD(B, N, LE, L)                                                                                      \
@@ -1833,7 +1847,7 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
       const dim3 grid((groups * LD + 255) / 256, nb);                                                          \
       hipLaunchKernelGGL((k_svd<B, N, LD, double>), grid, dim3(256), 0, s, (const double*)uvar, pos, sel, nvar, \
                          nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps, tol * tol, \
-                         (double*)factors, stats, place, (double*)W0);                            \
+                         (double*)factors, stats, place, (double*)W0, (double*)S01);                            \
       return hipGetLastError();                                                                                \
     }                                                                                                          \
     PG_SVD_F32(B, N, L)                                                                                        \
@@ -1857,7 +1871,7 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
     const dim3 grid((groups * L + 255) / 256, nb);                                                             \
     hipLaunchKernelGGL((k_svd<B, NM, L, float, true>), grid, dim3(256), 0, s, (const float*)uvar, pos, sel,    \
                        nvar, nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps,      \
-                       (float)(tol * tol), (float*)factors, stats, place, (float*)W0);             \
+                       (float)(tol * tol), (float*)factors, stats, place, (float*)W0, (float*)S01);             \
     return hipGetLastError();                                                                                  \
   }
   PG_GENERIC(PG_SVDG)
@@ -1880,34 +1894,32 @@ uint32_t eval_blocks(const Shape& sh, uint32_t nvar_run) {
 
 hipError_t launch_evaluation(const Shape& sh, const void* factors, const void* divq, const WinMeta* meta,
                              const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, int exp_w,
-                             const int32_t* inv, void* rec, void* rows, double* partial, const double* slot_lam,
-                             double* kept, hipStream_t s, uint32_t nk) {
+                             const int32_t* inv, void* rows, double* partial, const double* slot_lam, double* kept,
+                             hipStream_t s) {
   if (fused_eval(sh))
-    return launch_round_prep(factors, nvar_run == 3 ? divq : nullptr, inv, meta, wl, na, nvar_stored, sh, exp_w,
-                             slot_lam, rec, rows, nvar_run == 3 ? partial : nullptr, kept, s, gather_frames(sh, nk));
+    return launch_round_prep(factors, nvar_run == 3 ? divq : nullptr, meta, wl, na, nvar_stored, sh, exp_w,
+                             slot_lam, rows, nvar_run == 3 ? partial : nullptr, kept, s);
   return launch_eval(factors, meta, wl, na, nvar_stored, nvar_run, sh, exp_w, divq, inv, rows, partial, s, slot_lam,
                      kept);
 }
 
-size_t round_record_bytes(const Shape& sh) { return (size_t)sh.nw * sh.p_sel * 2 * (sh.fp64 ? 8 : 4); }
+size_t singular_pair_bytes(const Shape& sh) { return (size_t)sh.nw * sh.p_sel * 2 * (sh.fp64 ? 8 : 4); }
 
-hipError_t launch_round_prep(const void* factors, const void* qdiv, const int32_t* inv, const WinMeta* meta,
-                             const uint32_t* wl, uint32_t na, uint32_t nvar_stored, const Shape& sh, int exp_w,
-                             const double* slot_lam, void* rec, void* dr, double* partial, double* kept,
-                             hipStream_t s, uint32_t nk) {
+hipError_t launch_round_prep(const void* factors, const void* qdiv, const WinMeta* meta, const uint32_t* wl,
+                             uint32_t na, uint32_t nvar_stored, const Shape& sh, int exp_w, const double* slot_lam,
+                             void* dr, double* partial, double* kept, hipStream_t s) {
   const uint32_t nblk = eval_blocks(sh, 1);
-  if (nk == 0 || nk > sh.nw) nk = sh.nw;
 #define PG_RP(B, N, L, LS)                                                                                       \
   if (sh.bs == B && sh.nw == N) {                                                                              \
     if constexpr (B <= 4) {                                                                                    \
       if (sh.fp64)                                                                                             \
         hipLaunchKernelGGL((k_round_prep<B, N, double>), dim3(nblk, na), dim3(256), 0, s,                      \
-                           (const double*)factors, (const double*)qdiv, inv, meta, wl, nvar_stored, sh.p_sel,   \
-                           sh.nw, sh.fstride, exp_w, slot_lam, (double*)rec, (double*)dr, partial, kept, nk);   \
+                           (const double*)factors, (const double*)qdiv, meta, wl, nvar_stored, sh.p_sel, sh.nw, \
+                           sh.fstride, exp_w, slot_lam, (double*)dr, partial, kept);                            \
       else                                                                                                     \
         hipLaunchKernelGGL((k_round_prep<B, N, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)factors, \
-                           (const float*)qdiv, inv, meta, wl, nvar_stored, sh.p_sel, sh.nw, sh.fstride, exp_w,  \
-                           slot_lam, (float*)rec, (float*)dr, partial, kept, nk);                               \
+                           (const float*)qdiv, meta, wl, nvar_stored, sh.p_sel, sh.nw, sh.fstride, exp_w,       \
+                           slot_lam, (float*)dr, partial, kept);                                                \
       return hipGetLastError();                                                                                \
     }                                                                                                          \
   }
@@ -1919,8 +1931,8 @@ hipError_t launch_round_prep(const void* factors, const void* qdiv, const int32_
   if constexpr (B <= 4) {                                                                                      \
     if (sh.bs == B && nm == NM) {                                                                              \
       hipLaunchKernelGGL((k_round_prep<B, NM, float, true>), dim3(nblk, na), dim3(256), 0, s,                  \
-                         (const float*)factors, (const float*)qdiv, inv, meta, wl, nvar_stored, sh.p_sel,       \
-                         sh.nw, sh.fstride, exp_w, slot_lam, (float*)rec, (float*)dr, partial, kept, nk);       \
+                         (const float*)factors, (const float*)qdiv, meta, wl, nvar_stored, sh.p_sel, sh.nw,     \
+                         sh.fstride, exp_w, slot_lam, (float*)dr, partial, kept);                               \
       return hipGetLastError();                                                                                \
     }                                                                                                          \
   }
@@ -2021,14 +2033,14 @@ hipError_t launch_gather(const GatherSrc& src, const int32_t* off, const double*
   case B:                                                                                                      \
     if (sh.fp64)                                                                                               \
       hipLaunchKernelGGL((k_gather_tile<B, double>), dim3(nblk, na), dim3(256), 0, s, (const double*)src.factors, \
-                         (const double*)src.dr, (const double*)src.rec, (const double*)src.W0,                 \
-                         src.list, off, invw, u64, meta, wl,                            \
+                         (const double*)src.dr, (const double*)src.S01, (const double*)src.W0,                 \
+                         src.list, off, invw, u64, meta, wl, src.slot_lam, src.exp_w,                            \
                          sh.nr, sh.fs, sh.nw,                                                                  \
                          sh.vol, sh.n1, sh.p_sel, sh.fstride, src.nvar_stored, mode, nkk, out, partial);       \
     else                                                                                                       \
       hipLaunchKernelGGL((k_gather_tile<B, float>), dim3(nblk, na), dim3(256), 0, s, (const float*)src.factors,  \
-                         (const float*)src.dr, (const float*)src.rec, (const float*)src.W0,                    \
-                         src.list, off, invw, u64, meta, wl,                             \
+                         (const float*)src.dr, (const float*)src.S01, (const float*)src.W0,                    \
+                         src.list, off, invw, u64, meta, wl, src.slot_lam, src.exp_w,                             \
                          sh.nr, sh.fs, sh.nw,                                                                  \
                          sh.vol, sh.n1, sh.p_sel, sh.fstride, src.nvar_stored, mode, nkk, out, partial);       \
     break;

@@ -45,12 +45,13 @@ hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs
 // mapping 0: the default lanes per matrix of the shape, 1: the alternative
 // (FP32 only; see svd_lanes).  W0 set (patches up to 4 x 4): the U variant
 // also writes the static rows of the fused evaluation, w_0 v_0[k] into W0
-// [nb][nw][p_sel][m] at the bucket places place [nb][nw][p_sel] (the
+// [nb][nw][p_sel][m], and the singular pairs (s_0, s_1) into S01
+// [nb][nw][p_sel][2], at the bucket places place [nb][nw][p_sel] (the
 // buckets' inverse).
 hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
                       uint32_t nvar_stored, const Shape& sh, void* factors, int max_sweeps, double tol,
                       uint32_t* stats, hipStream_t s, int mapping = 0, const int32_t* place = nullptr,
-                      void* W0 = nullptr);
+                      void* W0 = nullptr, void* S01 = nullptr);
 uint32_t svd_lanes(uint32_t bs, uint32_t nw, int mapping);
 // position buckets of every (window, frame): cnt and off are [nb][nw][n1*n1
 // (+1)], list [nb][nw][p_sel]
@@ -76,38 +77,38 @@ hipError_t launch_eval(const void* factors, const WinMeta* meta, const uint32_t*
 hipError_t launch_divq(const void* factors, const int32_t* pos, const uint32_t* sel, const void* dw, uint32_t nb,
                        uint32_t nvar, const Shape& sh, void* divq, hipStream_t s);
 // Patches up to 4 x 4 evaluate lambda in two launches: launch_round_prep
-// (per patch: its round record, d_0 and the kept-rank count, at its bucket
-// place j = inv[b][k][p] of every frame k, rec [na][nw][p_sel][2]
-// (round_record_bytes per slot), the factors d_r of further kept
-// ranks into dr [na][p_sel][nw]; with PGURE (qdiv = divq) the divergence
-// sums into partial [na][eval_blocks]), then launch_gather, which forms the
-// patch rows from them, the window's static rows (written by launch_svd: W0
-// [nb][nw][p_sel][m] in bucket order) and the factors
-// as it gathers.  Larger patches: launch_eval
-// writes the rows to B0, launch_gather reads them.
+// (per patch that keeps rank 1: the factors d_r of its further kept ranks
+// into dr [na][p_sel][nw], ended by a 0; with PGURE (qdiv = divq) the
+// divergence sums into partial [na][eval_blocks]), then launch_gather,
+// which forms d_0 and the patch rows from the window's static singular
+// pairs and rows (written by launch_svd: S01 [nb][nw][p_sel][2], W0
+// [nb][nw][p_sel][m] in bucket order; singular_pair_bytes per window), dr
+// and the factors as it gathers.  Larger patches: launch_eval writes the
+// rows to B0, launch_gather reads them.
 bool fused_eval(const Shape& sh);
-size_t round_record_bytes(const Shape& sh);
-hipError_t launch_round_prep(const void* factors, const void* qdiv, const int32_t* inv, const WinMeta* meta,
-                             const uint32_t* wl, uint32_t na, uint32_t nvar_stored, const Shape& sh, int exp_w,
-                             const double* slot_lam, void* rec, void* dr, double* partial, double* kept,
-                             hipStream_t s, uint32_t nk = 0);
+size_t singular_pair_bytes(const Shape& sh);
+hipError_t launch_round_prep(const void* factors, const void* qdiv, const WinMeta* meta, const uint32_t* wl,
+                             uint32_t na, uint32_t nvar_stored, const Shape& sh, int exp_w, const double* slot_lam,
+                             void* dr, double* partial, double* kept, hipStream_t s);
 // One lambda evaluation of the listed slots, either form: `rows` is dr
-// (fused) or B0; rec is used by the fused form only, and holds records of
-// the frames a gather with the same nk reads (every frame when nk is 0).
+// (fused) or B0.
 hipError_t launch_evaluation(const Shape& sh, const void* factors, const void* divq, const WinMeta* meta,
                              const uint32_t* wl, uint32_t na, uint32_t nvar_stored, uint32_t nvar_run, int exp_w,
-                             const int32_t* inv, void* rec, void* rows, double* partial, const double* slot_lam,
-                             double* kept, hipStream_t s, uint32_t nk = 0);
-// what the gather reads: fused (bs <= 4) the factors, the round records,
-// dr and the static rows with the bucket lists; otherwise the rows B0
+                             const int32_t* inv, void* rows, double* partial, const double* slot_lam, double* kept,
+                             hipStream_t s);
+// what the gather reads: fused (bs <= 4) the factors, dr, the static
+// singular pairs and rows with the bucket lists, and the lambdas (slot_lam
+// per slot, or the windows' meta) with the weighting; otherwise the rows B0
 struct GatherSrc {
   const void* B0 = nullptr;
   const void* factors = nullptr;
   const void* dr = nullptr;
-  const void* rec = nullptr;
+  const void* S01 = nullptr;
   const void* W0 = nullptr;
   const int32_t* list = nullptr;
   uint32_t nvar_stored = 1;
+  const double* slot_lam = nullptr;
+  int exp_w = 0;
 };
 // values per stored factor block of one (patch, variant): V^T (nw rows of
 // sr = nw rounded up to 4 values), W^T (nw rows of m values), the nw
