@@ -76,6 +76,10 @@ SURVEY_ALGO = {
 #                                         (8); plus per kept rank its singular value and divergence
 #                                         factor (8) and, for the U variant's further ranks, d_r
 #                                         written (4 / 3 per counted rank)
+#   static_rows         bucket place      per (patch, frame): v_0[k] and the place read (8), the
+#                                         patch's w_0 row and (s_0, s_1) read once per patch
+#                                         ((4 m + 8) / n), the row w_0 v_0[k] and the pair
+#                                         written (4 m + 8)
 #   gather_risk         voxel             every staged row read once (fp32): per (patch, frame)
 #                                         its static row w_0 v_0[k] (4 m), its round record (8)
 #                                         and patch index (4), P n / NVOX of them per voxel; U
@@ -85,6 +89,7 @@ ALGO = {
     "svd_jacobi": ("valu", 14 * M * NW**2 + 8 * NW**3),
     "divergence_factors": ("hbm", 4.0 * NW * (NW + M) + 4.0 * NW + 4.0 * NW / 3 + 4.0 * NVOX / (3 * P_SEL)),
     "eval_reconstruct": ("hbm", EVAL_FIXED),
+    "static_rows": ("hbm", 8.0 + (4.0 * M + 8.0) / NW + 4.0 * M + 8.0),
     "gather_risk": ("hbm", (4.0 * M + 12.0) * P_SEL * NW / NVOX + 16),
 }
 
@@ -313,6 +318,7 @@ def roofline(prof):
 STAGE_KERNELS = {
     "svd_jacobi": ("pgsvt::k_svd<",),
     "divergence_factors": ("pgsvt::k_divq<",),
+    "static_rows": ("pgsvt::k_static_rows<",),
     "eval_reconstruct": ("pgsvt::k_round_prep<", "pgsvt::k_eval<"),
     "gather_risk": ("pgsvt::k_gather_",),
 }

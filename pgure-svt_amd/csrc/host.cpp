@@ -916,14 +916,16 @@ struct Pipeline {
     // units: matrices decomposed
     prof->run("svd_jacobi", (double)nb * nvar_run * sh.p_sel, [&] {
       // rotation threshold |c| > tol sqrt(ab): 1e-6 in FP32, 1e-12 in FP64;
-      // the U variant also lays out the fused evaluation's static rows and
-      // singular pairs
-      const bool fused = fused_eval(sh);
       hipchk(launch_svd(uvar.p, pos.p, sel_d.p, nb, nvar_run, nvar, sh, factors.p, 30, sh.fp64 ? 1e-12 : 1e-6,
-                        stats.p, s, svd_mapping, fused ? inv.p : nullptr, fused ? w0b.p : nullptr,
-                        fused ? s01b.p : nullptr),
+                        stats.p, s, svd_mapping),
              "svd");
     });
+    if (fused_eval(sh)) {
+      // units: bucket places (patch x frame) of the static rows and pairs
+      prof->run("static_rows", (double)nb * sh.nw * sh.p_sel, [&] {
+        hipchk(launch_static_rows(factors.p, inv.p, nb, nvar, sh, w0b.p, s01b.p, s), "static rows");
+      });
+    }
     if (optimize && nvar_run == 3) {
       // units: matrices
       prof->run("divergence_factors", (double)nb * nvar_run * sh.p_sel, [&] {
@@ -2223,8 +2225,9 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     hipchk(launch_weights(L.off, L.d2, 1, sh, d2neg, d2pos, L.invw, L.dw, s), "weights");
     hipchk(launch_fill32(L.stats, 1, 0, s), "clear stats");
     hipchk(launch_svd(L.uvar, L.pos, L.sel, 1, nvar, nvar, sh, L.factors, 30, 1e-6, L.stats, s,
-                      g_svd_mapping.load() == 1 ? 1 : 0, L.inv, L.w0, L.s01),
+                      g_svd_mapping.load() == 1 ? 1 : 0),
            "svd");
+    if (fused_eval(sh)) hipchk(launch_static_rows(L.factors, L.inv, 1, nvar, sh, L.w0, L.s01, s), "static rows");
     if (optimize) hipchk(launch_divq(L.factors, L.pos, L.sel, L.dw, 1, nvar, sh, L.divq, s), "divergence factors");
     double alpha = alpha0, mu = mu0, sigma = sigma0, lambda = lambda0;
     if (optimize) {

@@ -536,9 +536,7 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
                                              const uint32_t* __restrict__ sel, uint32_t nvar, uint32_t nvar_stored,
                                              uint32_t p_sel, uint32_t p_all, uint32_t nr, uint32_t fs, uint32_t nw,
                                              uint32_t vol, uint32_t lim, uint32_t fstride, int max_sweeps, T tol2,
-                                             T* __restrict__ factors, uint32_t* __restrict__ stats,
-                                             const int32_t* __restrict__ place, T* __restrict__ W0,
-                                             T* __restrict__ S01) {
+                                             T* __restrict__ factors, uint32_t* __restrict__ stats) {
   constexpr int M = BS * BS, R = (M + L - 1) / L, SRN = (N + 3) & ~3;
   const int n = RT ? (int)nw : N;
   const int SR = RT ? ((n + 3) & ~3) : SRN;
@@ -633,14 +631,13 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
   T* F = factors + (((size_t)b * nvar_stored + var) * p_sel + p) * fstride;
   T* FW = F + n * SR;
   T* FS = FW + n * M;
-  // The U variant also writes the static rows of the fused evaluation: at
-  // the patch's bucket place j_k = place[b][k][p] of every frame k, its
-  // rank-0 row w_0 v_0[k] (from the values stored in its factors) into W0
-  // and its two largest singular values (s_0, s_1) into S01
-  const bool rows0 = W0 != nullptr && var == 0;
-  T w0[R];
-  T s0v = T(0), s1v = T(0);
-  int rank[N];
+  // ranks (< N <= 32) packed in bytes, four per register: the epilogue
+  // keeps X, 1/s^2 and the ranks live, and unpacked ranks would cost the
+  // kernel a wave per SIMD
+  uint32_t rk8[(N + 3) / 4];
+#pragma unroll
+  for (int i = 0; i < (N + 3) / 4; ++i) rk8[i] = 0u;
+  auto rank_of = [&](int l) -> int { return (int)((rk8[l / 4] >> (8 * (l % 4))) & 0xffu); };
   T inv[N];
   T amax = a[0];
 #pragma unroll
@@ -649,21 +646,18 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
 #pragma unroll
   for (int l = 0; l < N; ++l) {
     // (RT: the zero columns past n rank after every real one)
-    rank[l] = 0;
+    uint32_t rl = 0;
 #pragma unroll
-    for (int m = 0; m < N; ++m) rank[l] += (a[m] > a[l] || (a[m] == a[l] && m < l)) ? 1 : 0;
+    for (int m = 0; m < N; ++m) rl += (a[m] > a[l] || (a[m] == a[l] && m < l)) ? 1u : 0u;
+    rk8[l / 4] |= rl << (8 * (l % 4));
     inv[l] = a[l] > eps * eps * amax ? T(1) / a[l] : T(0);
     if (RT && l >= n) continue;
 #pragma unroll
     for (int h = 0; h < R; ++h) {
       const int q = lane + L * h;
-      if (q < M) FW[rank[l] * M + q] = x[h][l];
-      if (rank[l] == 0) w0[h] = x[h][l];
+      if (q < M) FW[rl * M + q] = x[h][l];
     }
-    const T sv = sqrt(a[l]);
-    if (lane == 0) FS[rank[l]] = sv;
-    if (rank[l] == 0) s0v = sv;
-    if (rank[l] == 1) s1v = sv;
+    if (lane == 0) FS[rl] = sqrt(a[l]);
   }
 
   // V^T, column by column: element k of v_l = X^T w_l / s_l^2 from column
@@ -674,14 +668,6 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
     if (k < n) {
       const int32_t pk = pos[((size_t)b * nw + k) * p_all + it];
       const uint32_t row = min((uint32_t)(pk & 0xffff), lim), col = min((uint32_t)(pk >> 16), lim);
-      const int32_t jk = rows0 ? place[((size_t)b * nw + k) * p_sel + p] : 0;
-      if (rows0 && lane == 0) {
-        typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type T2;
-        T2 sp;
-        sp.x = s0v;
-        sp.y = s1v;
-        reinterpret_cast<T2*>(S01)[((size_t)b * nw + k) * p_sel + jk] = sp;
-      }
       T xk[R];
 #pragma unroll
       for (int h = 0; h < R; ++h) {
@@ -695,21 +681,58 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
 #pragma unroll
         for (int h = 1; h < R; ++h) pv = fma(xk[h], x[h][l], pv);
         const T vk = group_sum<L>(pv) * inv[l];
-        if (lane == k % L) F[rank[l] * SR + k] = vk;
-        if (rows0 && rank[l] == 0) {  // every lane holds vk
-          T* o = W0 + (((size_t)b * nw + k) * p_sel + jk) * M;
-#pragma unroll
-          for (int h = 0; h < R; ++h) {
-            const int q = lane + L * h;
-            if (q < M) o[q] = w0[h] * vk;
-          }
-        }
+        const int rl = rank_of(l);
+        if (lane == k % L) F[rl * SR + k] = vk;
       }
     } else {
 #pragma unroll
       for (int l = 0; l < N; ++l)
-        if ((!RT || l < n) && lane == k % L) F[rank[l] * SR + k] = T(0);
+        if ((!RT || l < n) && lane == k % L) F[rank_of(l) * SR + k] = T(0);
     }
+  }
+}
+
+// Static data of the fused evaluation (patches up to 4 x 4), from the U
+// variant's factors once the SVD is done: at the patch's bucket place
+// j_k = place[b][k][p] of every frame k, its rank-0 row w_0 v_0[k] into W0
+// [nb][nw][p_sel][m] and its two largest singular values (s_0, s_1) into
+// S01 [nb][nw][p_sel][2].  4 lanes per (patch, frame), frames fastest, so
+// a patch's 15 rows read its w_0 from one cache line and each row is one
+// line-sized store.  (Inside k_svd these stores would cost it a wave per
+// SIMD: its epilogue already holds X, 1/s^2 and the ranks.)
+template <int BS, class T>
+__global__ __launch_bounds__(256) void k_static_rows(const T* __restrict__ factors, const int32_t* __restrict__ place,
+                                                     uint32_t nvar_stored, uint32_t p_sel, uint32_t nw,
+                                                     uint32_t fstride, T* __restrict__ W0, T* __restrict__ S01) {
+  constexpr int M = BS * BS, QL = (M + 3) / 4;  // row elements per lane
+  const uint32_t b = blockIdx.y;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t sub = t % 4, row = t / 4;
+  const uint32_t p = row / nw, k = row % nw;
+  if (p >= p_sel) return;
+  const uint32_t SR = (nw + 3) & ~3u;
+  const T* F = factors + ((size_t)b * nvar_stored * p_sel + p) * fstride;  // the U variant
+  const T* FW = F + (size_t)nw * SR;
+  const T* FS = FW + (size_t)nw * M;
+  const T v0k = F[k];  // V^T row 0, element k
+  const int32_t jk = place[((size_t)b * nw + k) * p_sel + p];
+  T* o = W0 + (((size_t)b * nw + k) * p_sel + jk) * M;
+  if constexpr (M == 16 && sizeof(T) == 4) {  // lane sub: elements 4 sub .. 4 sub + 3
+    const float4 w = *reinterpret_cast<const float4*>(FW + 4 * sub);
+    *reinterpret_cast<float4*>(o + 4 * sub) = make_float4(w.x * v0k, w.y * v0k, w.z * v0k, w.w * v0k);
+  } else {
+#pragma unroll
+    for (int i = 0; i < QL; ++i) {
+      const uint32_t q = sub + 4 * i;
+      if (q < (uint32_t)M) o[q] = FW[q] * v0k;
+    }
+  }
+  if (sub == 0) {
+    typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type T2;
+    T2 sp;
+    sp.x = FS[0];
+    sp.y = nw > 1 ? FS[1] : T(0);
+    reinterpret_cast<T2*>(S01)[((size_t)b * nw + k) * p_sel + jk] = sp;
   }
 }
 
@@ -1827,7 +1850,7 @@ bool svd_shape_supported(uint32_t bs, uint32_t nw, bool fp64) {
 
 hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
                       uint32_t nvar_stored, const Shape& sh, void* factors, int max_sweeps, double tol,
-                      uint32_t* stats, hipStream_t s, int mapping, const int32_t* place, void* W0, void* S01) {
+                      uint32_t* stats, hipStream_t s, int mapping) {
   if (nvar > nvar_stored) return hipErrorInvalidValue;
   const uint32_t lim = sh.nr - sh.bs;
   const uint32_t groups = nvar * sh.p_sel;
@@ -1836,7 +1859,7 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
     const dim3 grid((groups * L + 255) / 256, nb);                                                             \
     hipLaunchKernelGGL((k_svd<B, N, L, float>), grid, dim3(256), 0, s, (const float*)uvar, pos, sel, nvar,     \
                        nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps,            \
-                       (float)(tol * tol), (float*)factors, stats, place, (float*)W0, (float*)S01);             \
+                       (float)(tol * tol), (float*)factors, stats);                                                \
     return hipGetLastError();                                                                                  \
   }
 #define PG_SVD(B, N, LE, L)                                                                                      \
@@ -1847,7 +1870,7 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
       const dim3 grid((groups * LD + 255) / 256, nb);                                                          \
       hipLaunchKernelGGL((k_svd<B, N, LD, double>), grid, dim3(256), 0, s, (const double*)uvar, pos, sel, nvar, \
                          nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps, tol * tol, \
-                         (double*)factors, stats, place, (double*)W0, (double*)S01);                            \
+                         (double*)factors, stats);                                                               \
       return hipGetLastError();                                                                                \
     }                                                                                                          \
     PG_SVD_F32(B, N, L)                                                                                        \
@@ -1871,12 +1894,32 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
     const dim3 grid((groups * L + 255) / 256, nb);                                                             \
     hipLaunchKernelGGL((k_svd<B, NM, L, float, true>), grid, dim3(256), 0, s, (const float*)uvar, pos, sel,    \
                        nvar, nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps,      \
-                       (float)(tol * tol), (float*)factors, stats, place, (float*)W0, (float*)S01);             \
+                       (float)(tol * tol), (float*)factors, stats);                                                \
     return hipGetLastError();                                                                                  \
   }
   PG_GENERIC(PG_SVDG)
 #undef PG_SVDG
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_static_rows(const void* factors, const int32_t* place, uint32_t nb, uint32_t nvar_stored,
+                              const Shape& sh, void* W0, void* S01, hipStream_t s) {
+  if (!fused_eval(sh) || nvar_stored == 0) return hipErrorInvalidValue;
+  const dim3 grid((uint32_t)(((size_t)sh.p_sel * sh.nw * 4 + 255) / 256), nb);
+#define PG_SR(B)                                                                                                 \
+  case B:                                                                                                      \
+    if (sh.fp64)                                                                                               \
+      hipLaunchKernelGGL((k_static_rows<B, double>), grid, dim3(256), 0, s, (const double*)factors, place,      \
+                         nvar_stored, sh.p_sel, sh.nw, sh.fstride, (double*)W0, (double*)S01);                  \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_static_rows<B, float>), grid, dim3(256), 0, s, (const float*)factors, place,        \
+                         nvar_stored, sh.p_sel, sh.nw, sh.fstride, (float*)W0, (float*)S01);                    \
+    return hipGetLastError();
+  switch (sh.bs) {
+    PG_SR(2) PG_SR(3) PG_SR(4)
+    default: return hipErrorInvalidValue;
+  }
+#undef PG_SR
 }
 
 bool fused_eval(const Shape& sh) { return sh.bs <= 4; }

@@ -43,15 +43,16 @@ hipError_t launch_arps(const double* w64, const int16_t* jobs, uint32_t max_jobs
 // SVD of the first nvar of the nvar_stored variants of every window (uvar
 // [nb][nvar_stored][vol] -> factors [nb][nvar_stored][p_sel][fstride]).
 // mapping 0: the default lanes per matrix of the shape, 1: the alternative
-// (FP32 only; see svd_lanes).  W0 set (patches up to 4 x 4): the U variant
-// also writes the static rows of the fused evaluation, w_0 v_0[k] into W0
-// [nb][nw][p_sel][m], and the singular pairs (s_0, s_1) into S01
-// [nb][nw][p_sel][2], at the bucket places place [nb][nw][p_sel] (the
-// buckets' inverse).
+// (FP32 only; see svd_lanes).
 hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel, uint32_t nb, uint32_t nvar,
                       uint32_t nvar_stored, const Shape& sh, void* factors, int max_sweeps, double tol,
-                      uint32_t* stats, hipStream_t s, int mapping = 0, const int32_t* place = nullptr,
-                      void* W0 = nullptr, void* S01 = nullptr);
+                      uint32_t* stats, hipStream_t s, int mapping = 0);
+// the static data of the fused evaluation (patches up to 4 x 4) from the U
+// variant's factors: w_0 v_0[k] into W0 [nb][nw][p_sel][m] and the singular
+// pairs (s_0, s_1) into S01 [nb][nw][p_sel][2], at the bucket places place
+// [nb][nw][p_sel] (the buckets' inverse)
+hipError_t launch_static_rows(const void* factors, const int32_t* place, uint32_t nb, uint32_t nvar_stored,
+                              const Shape& sh, void* W0, void* S01, hipStream_t s);
 uint32_t svd_lanes(uint32_t bs, uint32_t nw, int mapping);
 // position buckets of every (window, frame): cnt and off are [nb][nw][n1*n1
 // (+1)], list [nb][nw][p_sel]
