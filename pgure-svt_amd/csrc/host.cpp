@@ -674,6 +674,7 @@ struct Pipeline {
   // the fused evaluation's static data per window (patches up to 4 x 4):
   // rows w_0 v_0[k] and singular pairs (s_0, s_1) per (frame, bucket place)
   DBuf<char> w0b, s01b;
+  uint32_t static_frames = 0;  // frame subset of the static data (0: every frame; see launch_static_rows)
   DBuf<int16_t> jobs_d;
   uint32_t max_jobs = 0, gather_blocks = 0;
   // learned-lambda probe: the patch subsample (sh_s: p_sel = its size) with
@@ -922,8 +923,10 @@ struct Pipeline {
     });
     if (fused_eval(sh)) {
       // units: bucket places (patch x frame) of the static rows and pairs
-      prof->run("static_rows", (double)nb * sh.nw * sh.p_sel, [&] {
-        hipchk(launch_static_rows(factors.p, inv.p, nb, nvar, sh, w0b.p, s01b.p, s), "static rows");
+      const uint32_t nf = (static_frames && static_frames < sh.nw) ? static_frames + 1 : sh.nw;
+      prof->run("static_rows", (double)nb * nf * sh.p_sel, [&] {
+        hipchk(launch_static_rows(factors.p, inv.p, meta_d.p, nb, nvar, sh, w0b.p, s01b.p, s, static_frames),
+               "static rows");
       });
     }
     if (optimize && nvar_run == 3) {
@@ -1859,6 +1862,9 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
       }
       continue;
     }
+    // the learned path gathers only the probe's frames and the output
+    // frames: the static data of those frames suffice
+    P.static_frames = learned ? (probe ? gather_frames(P.sh, kProbeFrames) : 1u) : 0u;
     P.decompose(ms, learned ? 1 : P.nvar);
     if (probe) P.decompose_probe(nb);
     std::vector<WindowPrep> prep(nb);
@@ -1984,6 +1990,7 @@ int run_core(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf, co
         mf.push_back(ms[b]);
         pf.push_back(prep[b]);
       }
+      P.static_frames = 0;  // the search gathers every frame
       P.decompose(mf, 3);
       hipchk(hipStreamSynchronize(P.s), "fallback decompose sync");
       // SBPLX from the probe's estimate of the minimum, steps of the size of
@@ -2227,7 +2234,7 @@ int run_noalloc(const void* X, int dtype, uint32_t nr, uint32_t nc, uint32_t nf,
     hipchk(launch_svd(L.uvar, L.pos, L.sel, 1, nvar, nvar, sh, L.factors, 30, 1e-6, L.stats, s,
                       g_svd_mapping.load() == 1 ? 1 : 0),
            "svd");
-    if (fused_eval(sh)) hipchk(launch_static_rows(L.factors, L.inv, 1, nvar, sh, L.w0, L.s01, s), "static rows");
+    if (fused_eval(sh)) hipchk(launch_static_rows(L.factors, L.inv, L.meta, 1, nvar, sh, L.w0, L.s01, s), "static rows");
     if (optimize) hipchk(launch_divq(L.factors, L.pos, L.sel, L.dw, 1, nvar, sh, L.divq, s), "divergence factors");
     double alpha = alpha0, mu = mu0, sigma = sigma0, lambda = lambda0;
     if (optimize) {

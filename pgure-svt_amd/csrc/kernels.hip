@@ -699,17 +699,22 @@ __global__ __launch_bounds__(256) void k_svd(const T* __restrict__ u32, const in
 // S01 [nb][nw][p_sel][2].  4 lanes per (patch, frame), frames fastest, so
 // a patch's 15 rows read its w_0 from one cache line and each row is one
 // line-sized store.  (Inside k_svd these stores would cost it a wave per
-// SIMD: its epilogue already holds X, 1/s^2 and the ranks.)
+// SIMD: its epilogue already holds X, 1/s^2 and the ranks.)  nk = 0: every
+// frame; nk > 0: only the frames a gather reads with that frame subset
+// (i (nw - 1) / (nk - 1), i < nk) and the window's output frame (mode 1).
 template <int BS, class T>
 __global__ __launch_bounds__(256) void k_static_rows(const T* __restrict__ factors, const int32_t* __restrict__ place,
-                                                     uint32_t nvar_stored, uint32_t p_sel, uint32_t nw,
-                                                     uint32_t fstride, T* __restrict__ W0, T* __restrict__ S01) {
+                                                     const WinMeta* __restrict__ meta, uint32_t nvar_stored,
+                                                     uint32_t p_sel, uint32_t nw, uint32_t nk, uint32_t fstride,
+                                                     T* __restrict__ W0, T* __restrict__ S01) {
   constexpr int M = BS * BS, QL = (M + 3) / 4;  // row elements per lane
   const uint32_t b = blockIdx.y;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t sub = t % 4, row = t / 4;
-  const uint32_t p = row / nw, k = row % nw;
+  const uint32_t nf = nk ? nk + 1 : nw;  // frames written per patch
+  const uint32_t p = row / nf, i = row % nf;
   if (p >= p_sel) return;
+  const uint32_t k = !nk ? i : (i == nk ? meta[b].ref : (nk > 1 ? i * (nw - 1) / (nk - 1) : nw / 2));
   const uint32_t SR = (nw + 3) & ~3u;
   const T* F = factors + ((size_t)b * nvar_stored * p_sel + p) * fstride;  // the U variant
   const T* FW = F + (size_t)nw * SR;
@@ -1902,18 +1907,20 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_static_rows(const void* factors, const int32_t* place, uint32_t nb, uint32_t nvar_stored,
-                              const Shape& sh, void* W0, void* S01, hipStream_t s) {
+hipError_t launch_static_rows(const void* factors, const int32_t* place, const WinMeta* meta, uint32_t nb,
+                              uint32_t nvar_stored, const Shape& sh, void* W0, void* S01, hipStream_t s, uint32_t nk) {
   if (!fused_eval(sh) || nvar_stored == 0) return hipErrorInvalidValue;
-  const dim3 grid((uint32_t)(((size_t)sh.p_sel * sh.nw * 4 + 255) / 256), nb);
+  if (nk >= sh.nw) nk = 0;
+  const uint32_t nf = nk ? nk + 1 : sh.nw;
+  const dim3 grid((uint32_t)(((size_t)sh.p_sel * nf * 4 + 255) / 256), nb);
 #define PG_SR(B)                                                                                                 \
   case B:                                                                                                      \
     if (sh.fp64)                                                                                               \
-      hipLaunchKernelGGL((k_static_rows<B, double>), grid, dim3(256), 0, s, (const double*)factors, place,      \
-                         nvar_stored, sh.p_sel, sh.nw, sh.fstride, (double*)W0, (double*)S01);                  \
+      hipLaunchKernelGGL((k_static_rows<B, double>), grid, dim3(256), 0, s, (const double*)factors, place, meta, \
+                         nvar_stored, sh.p_sel, sh.nw, nk, sh.fstride, (double*)W0, (double*)S01);              \
     else                                                                                                       \
-      hipLaunchKernelGGL((k_static_rows<B, float>), grid, dim3(256), 0, s, (const float*)factors, place,        \
-                         nvar_stored, sh.p_sel, sh.nw, sh.fstride, (float*)W0, (float*)S01);                    \
+      hipLaunchKernelGGL((k_static_rows<B, float>), grid, dim3(256), 0, s, (const float*)factors, place, meta,   \
+                         nvar_stored, sh.p_sel, sh.nw, nk, sh.fstride, (float*)W0, (float*)S01);                \
     return hipGetLastError();
   switch (sh.bs) {
     PG_SR(2) PG_SR(3) PG_SR(4)

@@ -50,9 +50,12 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
 // the static data of the fused evaluation (patches up to 4 x 4) from the U
 // variant's factors: w_0 v_0[k] into W0 [nb][nw][p_sel][m] and the singular
 // pairs (s_0, s_1) into S01 [nb][nw][p_sel][2], at the bucket places place
-// [nb][nw][p_sel] (the buckets' inverse)
-hipError_t launch_static_rows(const void* factors, const int32_t* place, uint32_t nb, uint32_t nvar_stored,
-                              const Shape& sh, void* W0, void* S01, hipStream_t s);
+// [nb][nw][p_sel] (the buckets' inverse); for every frame (nk = 0), or for
+// the frames that gathers with frame subset nk and the final frames
+// (meta[b].ref) read
+hipError_t launch_static_rows(const void* factors, const int32_t* place, const WinMeta* meta, uint32_t nb,
+                              uint32_t nvar_stored, const Shape& sh, void* W0, void* S01, hipStream_t s,
+                              uint32_t nk = 0);
 uint32_t svd_lanes(uint32_t bs, uint32_t nw, int mapping);
 // position buckets of every (window, frame): cnt and off are [nb][nw][n1*n1
 // (+1)], list [nb][nw][p_sel]
