@@ -1482,8 +1482,15 @@ __global__ __launch_bounds__(256) void k_gather_direct(const T* __restrict__ B0,
 //   mode 0: risk sums per block: sum Uhat^2, sum Uhat, sum Uhat U, sum U, sum U^2
 //   mode 1: final frame: Y(:,:,out) = uMax * Uhat(:,:,ref)
 //   mode 2: whole window volume of slot 0 into out
+// 5 waves per SIMD for the FP32 4 x 4 instance (at most 96 VGPRs): its
+// latency-bound staging needs the waves more than the registers
 template <int BS, class T>
-__global__ __launch_bounds__(256) void k_gather_tile(
+constexpr int gather_min_waves() {
+  return (BS == 4 && sizeof(T) == 4) ? 5 : 1;
+}
+
+template <int BS, class T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_min_waves<BS, T>()))) void k_gather_tile(
     const T* __restrict__ factors, const T* __restrict__ dr, const T* __restrict__ S01, const T* __restrict__ W0,
     const int32_t* __restrict__ list, const int32_t* __restrict__ off, const double* __restrict__ invw,
     const double* __restrict__ u64, const WinMeta* __restrict__ meta, const uint32_t* __restrict__ wl,
@@ -1587,9 +1594,11 @@ __global__ __launch_bounds__(256) void k_gather_tile(
     // 4 x 4 patches in FP32: the 4 lanes of a row are one quad; lanes 0, 2
     // and 3 of the quad load the row's patch index, s_0 and s_1 (the loads
     // in flight hold one register per row instead of three).  After the
-    // wait lane 2 forms d_0 = shrink(s_0, s_0) and lane 3 shrink(s_1, s_0)
-    // in the same instructions, and the quad shares the index and the two
-    // factors by DPP broadcasts.
+    // wait the rows go in pairs (patch columns lc, lc + 1): lanes 2 and 3
+    // form d_0 = shrink(s_0, s_0) and shrink(s_1, s_0) of the first, lanes 0
+    // and 1 those of the second (its s_0, s_1 moved down by DPP), in the
+    // same instructions, and the quad shares the index and the factors by
+    // DPP broadcasts.
     constexpr bool QUAD = (QW == 4 && sizeof(T) == 4);
     uint32_t sc[NC][NH];
     const int sub = (int)threadIdx.x & 3;
@@ -1612,6 +1621,7 @@ __global__ __launch_bounds__(256) void k_gather_tile(
         }
       }
     }
+    T dn0[NH], dn1[NH];  // the second row's factors of a pair (QUAD)
 #pragma unroll
     for (int lc = 0; lc < NC; ++lc) {
 #pragma unroll
@@ -1620,12 +1630,29 @@ __global__ __launch_bounds__(256) void k_gather_tile(
         if constexpr (QUAD) {  // every lane of the quad is active here
           const int x = (int)sc[lc][h];
           ids[lc][h] = __builtin_amdgcn_mov_dpp(x, 0x00, 0xF, 0xF, false);  // quad_perm [0,0,0,0]
-          const T sa = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(x, 0xAA, 0xF, 0xF, false));  // [2,2,2,2]
-          // lane 2: shrink(s_0, s_0); lane 3: shrink(s_1, s_0) (lanes 0 and 1
-          // form an unused value)
-          const int d = __builtin_bit_cast(int, shrink_factor(__builtin_bit_cast(T, x), sa, lam, exp_w));
-          d0 = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0xAA, 0xF, 0xF, false));
-          d1 = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0xFF, 0xF, 0xF, false));  // [3,3,3,3]
+          if (lc % 2 == 1) {  // formed with the row before
+            d0 = dn0[h];
+            d1 = dn1[h];
+          } else {
+            // lanes 2, 3: this row's s_0, s_1; lanes 0, 1: the next row's
+            // (quad_perm [2,3,2,3]), or unused values for the last row
+            int v = x, sa = __builtin_amdgcn_mov_dpp(x, 0xAA, 0xF, 0xF, false);  // [2,2,2,2]
+            if (lc + 1 < NC) {
+              const int y = (int)sc[lc + 1][h];
+              const int yv = __builtin_amdgcn_mov_dpp(y, 0xEE, 0xF, 0xF, false);
+              const int ya = __builtin_amdgcn_mov_dpp(y, 0xAA, 0xF, 0xF, false);
+              v = sub < 2 ? yv : x;
+              sa = sub < 2 ? ya : sa;
+            }
+            const int d = __builtin_bit_cast(
+                int, shrink_factor(__builtin_bit_cast(T, v), __builtin_bit_cast(T, sa), lam, exp_w));
+            d0 = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0xAA, 0xF, 0xF, false));  // [2,2,2,2]
+            d1 = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0xFF, 0xF, 0xF, false));  // [3,3,3,3]
+            if (lc + 1 < NC) {
+              dn0[h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0x00, 0xF, 0xF, false));
+              dn1[h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0x55, 0xF, 0xF, false));  // [1,1,1,1]
+            }
+          }
         } else {
           d0 = shrink_factor(s0[lc][h], s0[lc][h], lam, exp_w);
           d1 = shrink_factor(s1[lc][h], s0[lc][h], lam, exp_w);
