@@ -1475,9 +1475,8 @@ __global__ __launch_bounds__(256) void k_gather_direct(const T* __restrict__ B0,
 // patch of each of its bs x bs buckets (one LDS read and a select, no
 // branch), then, only in waves where a bucket holds several patches, the
 // remaining ones.  The order is fixed, so the sums are reproducible.  A tile
-// with more than TOT patch rows, or a column range longer than COLCAP
-// (trajectories piling up), forms each contribution where it is summed, in
-// the same order.  Blocks of one tile run frame after frame, so that the
+// with more than TOT patch rows (trajectories piling up) forms each
+// contribution where it is summed, in the same order.  Blocks of one tile run frame after frame, so that the
 // factor rows the tile's patches use in every frame meet in one L2.
 //   mode 0: risk sums per block: sum Uhat^2, sum Uhat, sum Uhat U, sum U, sum U^2
 //   mode 1: final frame: Y(:,:,out) = uMax * Uhat(:,:,ref)
@@ -1498,8 +1497,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_min_
     uint32_t p_sel, uint32_t fstride, uint32_t nvar_stored, int mode, uint32_t nk, double* __restrict__ out,
     double* __restrict__ partial) {
   // TOT: staged patch rows per tile, the bucket ranges of its NC patch
-  // columns packed one after another; COLCAP: rows per patch column
-  constexpr int M = BS * BS, TR = 32, TC = 8, NC = TC + BS - 1, NRW = TR + BS, TOT = 448, COLCAP = 64;
+  // columns packed one after another
+  constexpr int M = BS * BS, TR = 32, TC = 8, NC = TC + BS - 1, NRW = TR + BS, TOT = 448;
   constexpr int RS = TOT + 1;                                // plane stride (odd: spreads the planes over banks)
   constexpr int W = (sizeof(T) == 4 && M % 4 == 0) ? 4 : 1;  // row values per thread while staging
   typedef typename std::conditional<W == 4, float4, T>::type ST;
@@ -1565,80 +1564,81 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_min_
   __syncthreads();
   // column ranges, packed: column lc's rows start at LDS row pre[lc]
   int32_t pre[NC + 1];
-  bool fits = true;
   pre[0] = 0;
 #pragma unroll
-  for (int lc = 0; lc < NC; ++lc) {
-    const int32_t cnt = sT[lc * NRW + NRW - 1] - sT[lc * NRW];
-    fits = fits && cnt <= COLCAP;
-    pre[lc + 1] = pre[lc] + cnt;
-  }
-  fits = fits && pre[NC] <= TOT;
+  for (int lc = 0; lc < NC; ++lc) pre[lc + 1] = pre[lc] + (sT[lc * NRW + NRW - 1] - sT[lc * NRW]);
+  const bool fits = pre[NC] <= TOT;
 #pragma unroll
   for (int lc = 0; lc < NC; ++lc)
     if ((int)threadIdx.x == lc) sP[lc] = pre[lc];
   if (fits) {
-    // patch column lc: thread -> (row lr, values q .. q + W - 1) of the
-    // column's bucket range, RPI rows per pass.  The patch indices of every
-    // row are loaded first, then every row is formed.
-    constexpr int QW = M / W, RPI = 256 / QW, NH = (COLCAP + RPI - 1) / RPI;
+    // the tile's rows in LDS order (row R of column lc is bucket place
+    // sT[lc][0] + R - pre[lc]): thread -> (rows lr0 + RPI s, values q ..
+    // q + W - 1), NS slots.  Every row's loads are issued first, then every
+    // row is formed.
+    constexpr int QW = M / W, RPI = 256 / QW, NS = (TOT + RPI - 1) / RPI;
     const int lr0 = (int)threadIdx.x / QW, q = ((int)threadIdx.x % QW) * W;
+    const int tot = pre[NC];
     // Every load of a row is at its bucket place (the patch index, the
     // thread's values of the static row, the singular pair), issued for
     // every row before the first use; further ranks are read as needed.
     // Row value q: d_0 (w_0[q] v_0[k]), then the FMA chain of the
     // reconstruction over the further kept ranks, fma(d_r w_r[q], v_r[k], .).
-    int32_t ids[NC][NH];
-    ST w0[NC][NH];
-    T s0[NC][NH], s1[NC][NH];
+    int32_t ids[NS];
+    ST w0[NS];
+    T s0[NS], s1[NS];
     // 4 x 4 patches in FP32: the 4 lanes of a row are one quad; lanes 0, 2
     // and 3 of the quad load the row's patch index, s_0 and s_1 (the loads
     // in flight hold one register per row instead of three).  After the
-    // wait the rows go in pairs (patch columns lc, lc + 1): lanes 2 and 3
-    // form d_0 = shrink(s_0, s_0) and shrink(s_1, s_0) of the first, lanes 0
-    // and 1 those of the second (its s_0, s_1 moved down by DPP), in the
-    // same instructions, and the quad shares the index and the factors by
-    // DPP broadcasts.
+    // wait the slots go in pairs (s, s + 1): lanes 2 and 3 form d_0 =
+    // shrink(s_0, s_0) and shrink(s_1, s_0) of the first row, lanes 0 and 1
+    // those of the second (its s_0, s_1 moved down by DPP), in the same
+    // instructions, and the quad shares the index and the factors by DPP
+    // broadcasts.
     constexpr bool QUAD = (QW == 4 && sizeof(T) == 4);
-    uint32_t sc[NC][NH];
+    uint32_t sc[NS];
     const int sub = (int)threadIdx.x & 3;
 #pragma unroll
-    for (int lc = 0; lc < NC; ++lc) {
-      const int32_t j0 = sT[lc * NRW], cnt = sT[lc * NRW + NRW - 1] - j0;
+    for (int sl = 0; sl < NS; ++sl) {
+      const int R = lr0 + RPI * sl;
+      const bool use = lr0 < RPI && R < tot;
+      // the row's column: lc = number of columns after the first that start
+      // at or before R, base = pre[lc]
+      int lc = 0, base = 0;
 #pragma unroll
-      for (int h = 0; h < NH; ++h) {
-        const int lr = lr0 + RPI * h;
-        const bool use = lr0 < RPI && lr < cnt;
-        const int32_t j = use ? j0 + lr : 0;
-        w0[lc][h] = *reinterpret_cast<const ST*>(W0k + (size_t)j * M + q);
-        if constexpr (QUAD) {
-          const void* a = sub == 0 ? (const void*)(Lk + j) : (const void*)(Sk + 2 * (size_t)j + (sub - 2));
-          sc[lc][h] = (use && sub != 1) ? *reinterpret_cast<const uint32_t*>(a) : (sub == 0 ? 0xffffffffu : 0u);
-        } else {
-          ids[lc][h] = use ? Lk[j] : -1;
-          s0[lc][h] = Sk[2 * (size_t)j];
-          s1[lc][h] = Sk[2 * (size_t)j + 1];
-        }
+      for (int cc = 1; cc < NC; ++cc) {
+        const bool after = pre[cc] <= R;
+        lc += after ? 1 : 0;
+        base = after ? pre[cc] : base;
+      }
+      const int32_t j = use ? sT[lc * NRW] + (R - base) : 0;
+      w0[sl] = *reinterpret_cast<const ST*>(W0k + (size_t)j * M + q);
+      if constexpr (QUAD) {
+        const void* a = sub == 0 ? (const void*)(Lk + j) : (const void*)(Sk + 2 * (size_t)j + (sub - 2));
+        sc[sl] = (use && sub != 1) ? *reinterpret_cast<const uint32_t*>(a) : (sub == 0 ? 0xffffffffu : 0u);
+      } else {
+        ids[sl] = use ? Lk[j] : -1;
+        s0[sl] = Sk[2 * (size_t)j];
+        s1[sl] = Sk[2 * (size_t)j + 1];
       }
     }
-    T dn0[NH], dn1[NH];  // the second row's factors of a pair (QUAD)
+    T dn0 = T(0), dn1 = T(0);  // the second row's factors of a pair (QUAD)
 #pragma unroll
-    for (int lc = 0; lc < NC; ++lc) {
-#pragma unroll
-      for (int h = 0; h < NH; ++h) {
+    for (int sl = 0; sl < NS; ++sl) {
+      {
         T d0, d1;
         if constexpr (QUAD) {  // every lane of the quad is active here
-          const int x = (int)sc[lc][h];
-          ids[lc][h] = __builtin_amdgcn_mov_dpp(x, 0x00, 0xF, 0xF, false);  // quad_perm [0,0,0,0]
-          if (lc % 2 == 1) {  // formed with the row before
-            d0 = dn0[h];
-            d1 = dn1[h];
+          const int x = (int)sc[sl];
+          ids[sl] = __builtin_amdgcn_mov_dpp(x, 0x00, 0xF, 0xF, false);  // quad_perm [0,0,0,0]
+          if (sl % 2 == 1) {  // formed with the row before
+            d0 = dn0;
+            d1 = dn1;
           } else {
             // lanes 2, 3: this row's s_0, s_1; lanes 0, 1: the next row's
             // (quad_perm [2,3,2,3]), or unused values for the last row
             int v = x, sa = __builtin_amdgcn_mov_dpp(x, 0xAA, 0xF, 0xF, false);  // [2,2,2,2]
-            if (lc + 1 < NC) {
-              const int y = (int)sc[lc + 1][h];
+            if (sl + 1 < NS) {
+              const int y = (int)sc[sl + 1];
               const int yv = __builtin_amdgcn_mov_dpp(y, 0xEE, 0xF, 0xF, false);
               const int ya = __builtin_amdgcn_mov_dpp(y, 0xAA, 0xF, 0xF, false);
               v = sub < 2 ? yv : x;
@@ -1648,20 +1648,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_min_
                 int, shrink_factor(__builtin_bit_cast(T, v), __builtin_bit_cast(T, sa), lam, exp_w));
             d0 = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0xAA, 0xF, 0xF, false));  // [2,2,2,2]
             d1 = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0xFF, 0xF, 0xF, false));  // [3,3,3,3]
-            if (lc + 1 < NC) {
-              dn0[h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0x00, 0xF, 0xF, false));
-              dn1[h] = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0x55, 0xF, 0xF, false));  // [1,1,1,1]
+            if (sl + 1 < NS) {
+              dn0 = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0x00, 0xF, 0xF, false));
+              dn1 = __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(d, 0x55, 0xF, 0xF, false));  // [1,1,1,1]
             }
           }
         } else {
-          d0 = shrink_factor(s0[lc][h], s0[lc][h], lam, exp_w);
-          d1 = shrink_factor(s1[lc][h], s0[lc][h], lam, exp_w);
+          d0 = shrink_factor(s0[sl], s0[sl], lam, exp_w);
+          d1 = shrink_factor(s1[sl], s0[sl], lam, exp_w);
         }
-        if (ids[lc][h] >= 0) {
-          const int32_t id = ids[lc][h];
+        if (ids[sl] >= 0) {
+          const int32_t id = ids[sl];
           ST row;
           T* ac = reinterpret_cast<T*>(&row);
-          const T* ww = reinterpret_cast<const T*>(&w0[lc][h]);
+          const T* ww = reinterpret_cast<const T*>(&w0[sl]);
 #pragma unroll
           for (int w = 0; w < W; ++w) ac[w] = (d0 > T(0)) ? d0 * ww[w] : T(0);
           if (d1 > T(0)) {
@@ -1675,9 +1675,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_min_
               for (int w = 0; w < W; ++w) ac[w] = fma(d * wk[w], vk, ac[w]);
             }
           }
-          const int lr = lr0 + RPI * h;
+          const int R = lr0 + RPI * sl;
 #pragma unroll
-          for (int w = 0; w < W; ++w) sB[(q + w) * RS + pre[lc] + lr] = ac[w];
+          for (int w = 0; w < W; ++w) sB[(q + w) * RS + R] = ac[w];
         }
       }
     }
