@@ -208,3 +208,42 @@ def test_shapes_fixed_lambda_and_risk(gpu, seq64x40, svd_mapping, bs, step, T, e
     ro = O.window_risk(seq64x40, t, noise, [lam / 3, lam, 3 * lam], **kw)
     log("  risk rel =", np.abs(rg - ro) / np.abs(ro))
     assert np.all(np.abs(rg - ro) <= RISK_TOL * np.abs(ro))
+
+
+def _bowl_sequence(n=64, nf=20, seed=5):
+    """Every frame but the middle one is a paraboloid bowl: the motion
+    search slides each block downhill, so the trajectories pile up (up to
+    225 patches on one position of a frame) and tiles of the gather hold
+    more patch rows than it stages in LDS (975 against 448); those tiles
+    form each contribution where it is summed."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:n, 0:n]
+    bowl = (yy - 30.5) ** 2 + (xx - 30.5) ** 2
+    X = np.empty((n, n, nf))
+    for k in range(nf):
+        X[:, :, k] = 1000.0 + (0.0 if k == nf // 2 else bowl) + rng.random((n, n))
+    return np.asfortranarray(X)
+
+
+def test_piled_up_trajectories(gpu):
+    X = _bowl_sequence()
+    kw = dict(patch_size=4, patch_overlap=1, trajectory_length=15, motion_window=7, motion_filter=-1)
+    t = 10
+    tg = G.window_trajectories(X, t, **kw)
+    to = O.window_trajectories(X, t, **kw)
+    assert np.array_equal(tg, to)
+    n1 = X.shape[0] - 3
+    pile = max(int(np.bincount(tg[k][:, 0] + n1 * tg[k][:, 1]).max()) for k in range(tg.shape[0]))
+    log("most patches on one position:", pile)
+    assert pile > 100
+    for expw, lam in ((True, 0.5), (False, 0.05)):
+        noise = [0.02, 0.03, 0.04]
+        rg = G.window_risk(X, t, noise, [lam / 3, lam, 3 * lam], exponential_weighting=expw, **kw)
+        ro = O.window_risk(X, t, noise, [lam / 3, lam, 3 * lam], exponential_weighting=expw, **kw)
+        log("  expw", expw, "risk rel =", np.abs(rg - ro) / np.abs(ro))
+        assert np.all(np.abs(rg - ro) <= RISK_TOL * np.abs(ro))
+        Yg, _ = G.denoise(X, optimize_pgure=False, lambda1=lam, exponential_weighting=expw, **kw)
+        Yo, _ = O.denoise(X, optimize_pgure=0, lambda1=lam, exponential_weighting=expw, **kw)
+        e = G.rel_l2(Yg, Yo)
+        log("  frames rel L2 =", e)
+        assert e < FRAME_TOL
