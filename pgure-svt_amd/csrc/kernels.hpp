@@ -85,7 +85,7 @@ hipError_t launch_divq(const void* factors, const int32_t* pos, const uint32_t* 
 // into dr [na][p_sel][nw], ended by a 0; with PGURE (qdiv = divq) the
 // divergence sums into partial [na][eval_blocks]), then launch_gather,
 // which forms d_0 and the patch rows from the window's static singular
-// pairs and rows (written by launch_svd: S01 [nb][nw][p_sel][2], W0
+// pairs and rows (written by launch_static_rows: S01 [nb][nw][p_sel][2], W0
 // [nb][nw][p_sel][m] in bucket order; singular_pair_bytes per window), dr
 // and the factors as it gathers.  Larger patches: launch_eval writes the
 // rows to B0, launch_gather reads them.
