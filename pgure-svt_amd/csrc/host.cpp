@@ -653,6 +653,10 @@ struct Pipeline {
   DBuf<double> Xd, Zd, Yd;
   double* Yout = nullptr;           // final frames (Yd.p or a caller device buffer)
   HBuf<char> host_copy;             // host copy of a device-resident input (pinned)
+  // the host copy is made on a stream of its own, beside the GPU work; the
+  // host readers wait for `copied` (host_ready)
+  hipStream_t cs = nullptr;
+  hipEvent_t copied = nullptr;
   std::unique_ptr<Profiler> prof;
   DBuf<uint8_t> d2;
   DBuf<uint32_t> sel_d;
@@ -687,10 +691,18 @@ struct Pipeline {
   DBuf<uint32_t> stats;
 
   ~Pipeline() {
-    // drain the stream before the member buffers go back to the cache
+    // drain the streams before the member buffers go back to the caches
     if (s) (void)hipStreamSynchronize(s);
+    if (cs) (void)hipStreamSynchronize(cs);
     prof.reset();
+    if (copied) (void)hipEventDestroy(copied);
+    if (cs) (void)hipStreamDestroy(cs);
     if (s) (void)hipStreamDestroy(s);
+  }
+
+  // the host copy of a device-resident input is complete (thread-safe)
+  void host_ready() const {
+    if (copied) hipchk(hipEventSynchronize(copied), "host copy of X");
   }
 
   // X_ is a host pointer, or a device pointer when on_device is set; the
@@ -733,7 +745,10 @@ struct Pipeline {
     if (on_device) {
       if (optimize || host_noise) {  // the noise estimator runs on the host
         host_copy.alloc(N * dtype_size(dtype));
-        hipchk(hipMemcpyAsync(host_copy.p, X, N * dtype_size(dtype), hipMemcpyDeviceToHost, s), "D2H X");
+        hipchk(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
+        hipchk(hipEventCreateWithFlags(&copied, hipEventDisableTiming), "hipEventCreate");
+        hipchk(hipMemcpyAsync(host_copy.p, X, N * dtype_size(dtype), hipMemcpyDeviceToHost, cs), "D2H X");
+        hipchk(hipEventRecord(copied, cs), "event record");
         X = host_copy.p;
       }
     } else {
@@ -1109,6 +1124,7 @@ struct Pipeline {
   // accu(u) of the normalised window t in Armadillo's order (even / odd
   // partial sums), the lambda start point of src/pguresvt.hpp:139
   double host_window_accu(uint32_t t) const {
+    host_ready();
     const WinMeta m = meta_of(t);
     const size_t base = (size_t)m.s0 * sh.fs;
     double a1 = 0, a2 = 0;
@@ -1123,6 +1139,7 @@ struct Pipeline {
 
   // frame f normalised by scale (true division, as u /= uMax)
   void host_frame(uint32_t f, double scale, double* out) const {
+    host_ready();
     const size_t base = (size_t)f * sh.fs;
     for (size_t k = 0; k < sh.fs; ++k) out[k] = elem(X, dtype, base + k) / scale;
   }
