@@ -5,7 +5,9 @@
 // the complete quadtree whose size has an F-table entry), two-pass sums of
 // the values and of their 5-point residuals; the host replays the
 // reference's recursion with the decisions.  Leaf statistics: one workgroup
-// per distinct leaf.  The robust variance (median absolute deviation of the
+// per distinct leaf; leaves too large for its registers (every frame's root
+// block among them) have their values and Laplacians formed once, spread
+// over the GPU (k_leaf_prep), and the workgroup streams them.  The robust variance (median absolute deviation of the
 // 9-point Laplacian) is bit-identical to the host estimator: every Laplacian
 // value is formed per element in the host's order, and the medians are
 // exact order statistics, selected by a radix walk over the values' bit
