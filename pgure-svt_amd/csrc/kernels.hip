@@ -1827,14 +1827,14 @@ __global__ __launch_bounds__(256) void k_risk_reduce(const double* __restrict__ 
 #define PG_SHAPES(X) \
   X(4, 3, 16, 4) X(4, 5, 16, 4) X(4, 7, 16, 4) X(4, 9, 16, 4) X(4, 11, 16, 4) X(4, 13, 16, 4) X(4, 15, 16, 4) \
   X(2, 3, 16, 4) X(3, 3, 16, 4) X(3, 5, 16, 4) X(3, 7, 16, 4) X(3, 9, 16, 4) \
-  X(6, 7, 64, 16) X(6, 15, 64, 16) X(6, 31, 64, 32) X(8, 7, 64, 16) X(8, 15, 64, 32) X(8, 31, 64, 32)
+  X(6, 7, 64, 16) X(6, 15, 64, 16) X(6, 31, 64, 32) X(8, 7, 64, 16) X(8, 15, 64, 16) X(8, 31, 64, 32)
 
 // second lanes-per-matrix mapping of the SVD for each shape, timed against
 // the first by the autotuner of the batched path
 #define PG_SVD_ALT(X) \
   X(4, 3, 8) X(4, 5, 8) X(4, 7, 8) X(4, 9, 8) X(4, 11, 8) X(4, 13, 8) X(4, 15, 8) \
   X(2, 3, 2) X(3, 3, 8) X(3, 5, 8) X(3, 7, 8) X(3, 9, 8) \
-  X(6, 7, 32) X(6, 15, 32) X(6, 31, 64) X(8, 7, 32) X(8, 15, 64) X(8, 31, 64)
+  X(6, 7, 32) X(6, 15, 32) X(6, 31, 64) X(8, 7, 32) X(8, 15, 32) X(8, 31, 64)
 
 // Every other shape, FP32 only: the window length is known at run time
 // (the kernels' RT form) up to NM frames.  (patch size, NM, lanes per
@@ -1897,9 +1897,9 @@ hipError_t launch_svd(const void* uvar, const int32_t* pos, const uint32_t* sel,
 #define PG_SVD(B, N, LE, L)                                                                                      \
   if (sh.bs == B && sh.nw == N && (sh.fp64 || mapping == 0)) {                                                 \
     if (sh.fp64) {                                                                                             \
-      /* FP64: at least 16 lanes per matrix (fewer rows per lane fit the registers), 64 for the 64 x 31     \
-         matrices (at 32 they would not fit) */                                                                \
-      constexpr int LD = (L < 16) ? 16 : ((B == 8 && N == 31) ? 64 : L);                                      \
+      /* FP64: at least 16 lanes per matrix (fewer rows per lane fit the registers); the 64-row matrices     \
+         of patch 8 take 32 (15 frames) and 64 (31 frames) */                                                  \
+      constexpr int LD = (B == 8 && N == 31) ? 64 : (B == 8 && N == 15) ? 32 : (L < 16 ? 16 : L);            \
       const dim3 grid((groups * LD + 255) / 256, nb);                                                          \
       hipLaunchKernelGGL((k_svd<B, N, LD, double>), grid, dim3(256), 0, s, (const double*)uvar, pos, sel, nvar, \
                          nvar_stored, sh.p_sel, sh.p_all, sh.nr, sh.fs, sh.nw, sh.vol, lim, sh.fstride, max_sweeps, tol * tol, \
