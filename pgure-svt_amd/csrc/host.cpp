@@ -68,7 +68,7 @@ uint32_t g_probe_refine = 2;
 thread_local int g_last_fallbacks = 0;
 // probe sample: every kProbeStride-th selected patch in all three variants,
 // the data terms over kProbeFrames frames spread over the window
-constexpr uint32_t kProbeStride = 16, kProbeFrames = 5;
+constexpr uint32_t kProbeStride = 32, kProbeFrames = 5;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
