@@ -261,7 +261,9 @@ int pgsvt_release_cache(void);
 
 /* Per-stage timing of later pgsvt_denoise* calls (no reference
  * counterpart; the reference prints wall times only, src/pguresvt.hpp:168).
- * pgsvt_profile_enable(1) clears and starts collection, (0) stops it.
+ * pgsvt_profile_enable(1) clears and starts collection, (0) stops it;
+ * (2) also counts the ranks each lambda evaluation keeps ("eval_kept_ranks",
+ * at the price of one more reduction and copy per search round).
  * GPU stages are timed with HIP events on the library's stream.
  * pgsvt_profile_read writes a JSON object
  *   {"<stage>": {"launches": n, "ms": total, "units": work units}, ...}

@@ -125,12 +125,16 @@ struct ProfEntry {
 };
 std::mutex g_prof_mu;
 bool g_prof_on = false;
+// also count the ranks each evaluation keeps (one more reduction and copy
+// per search round)
+bool g_prof_count = false;
 std::vector<ProfEntry> g_prof;
 
 struct ProfPending {
   int idx;
   hipEvent_t a, b;
   double units;
+  bool own_a, own_b;  // events returned to the pool with this entry
 };
 
 class Profiler {
@@ -152,18 +156,29 @@ class Profiler {
     hipchk(hipEventRecord(a, s_), "event record");
     launch();
     hipchk(hipEventRecord(b, s_), "event record");
-    int idx;
-    {
-      std::lock_guard<std::mutex> g(g_prof_mu);
-      idx = -1;
-      for (size_t k = 0; k < g_prof.size(); ++k)
-        if (g_prof[k].name == name) idx = (int)k;
-      if (idx < 0) {
-        g_prof.push_back(ProfEntry{name});
-        idx = (int)g_prof.size() - 1;
-      }
+    pend_.push_back({entry(name), a, b, units, true, true});
+  }
+  // two stages launched back to back, timed with three events: `name`
+  // covers both, `n1` the first and `n2` the second (an event between
+  // every pair of kernels delays the next dispatch; the shared middle event
+  // saves three of six)
+  template <class F1, class F2>
+  void run2(const char* name, double units, const char* n1, double u1, F1 launch1, const char* n2, double u2,
+            F2 launch2) {
+    if (!g_prof_on || suspended) {
+      launch1();
+      launch2();
+      return;
     }
-    pend_.push_back({idx, a, b, units});
+    const hipEvent_t a = event(), m = event(), b = event();
+    hipchk(hipEventRecord(a, s_), "event record");
+    launch1();
+    hipchk(hipEventRecord(m, s_), "event record");
+    launch2();
+    hipchk(hipEventRecord(b, s_), "event record");
+    pend_.push_back({entry(name), a, b, units, true, true});
+    pend_.push_back({entry(n1), a, m, u1, false, true});
+    pend_.push_back({entry(n2), m, b, u2, false, false});
   }
   // read the events of everything already synchronised
   void flush() {
@@ -177,13 +192,20 @@ class Profiler {
         g_prof[p.idx].ms += ms;
         g_prof[p.idx].units += p.units;
       }
-      pool_.push_back(p.a);
-      pool_.push_back(p.b);
+      if (p.own_a) pool_.push_back(p.a);
+      if (p.own_b) pool_.push_back(p.b);
     }
     pend_.clear();
   }
 
  private:
+  int entry(const char* name) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (size_t k = 0; k < g_prof.size(); ++k)
+      if (g_prof[k].name == name) return (int)k;
+    g_prof.push_back(ProfEntry{name});
+    return (int)g_prof.size() - 1;
+  }
   hipStream_t s_;
   std::vector<ProfPending> pend_;
   std::vector<hipEvent_t> pool_;  // events of flushed launches, reused
@@ -1074,23 +1096,22 @@ struct Pipeline {
     hipchk(hipMemcpyAsync(meta_d.p, meta_h.p, ms.size() * sizeof(WinMeta), hipMemcpyHostToDevice, s), "H2D meta");
     hipchk(hipMemcpyAsync(wl_d.p, wl_h.p, na * sizeof(uint32_t), hipMemcpyHostToDevice, s), "H2D list");
     // units: thresholded patch reconstructions (patch x variant)
-    // while profiling, the evaluation also counts the ranks it keeps (the
-    // factor rows it reads)
-    const bool count = g_prof_on && !prof->suspended;
+    // profiling with rank counts: the evaluation also counts the ranks it
+    // keeps (the factor rows it reads)
+    const bool count = g_prof_count && !prof->suspended;
     // units of the whole evaluation: voxels of the evaluated windows
-    prof->run("search_eval", (double)na * sh.vol, [&] {
-      prof->run("eval_reconstruct", (double)na * 3 * sh.p_sel,
-                [&] { evaluate(wl_d.p, na, 3, nullptr, partial.p, count ? partial_k.p : nullptr); });
-      // units: voxels
-      prof->run("gather_risk", (double)na * sh.vol, [&] {
-        hipchk(launch_gather(gather_src(), off.p, invw.p, u64.p, meta_d.p, wl_d.p, na, sh, 0, nullptr, partial_b.p,
-                             s),
-               "gather");
-        hipchk(launch_risk_reduce(partial.p, eval_blocks(sh, 3), partial_b.p, gather_blocks, na, sums.p, s,
-                                  count ? partial_k.p : nullptr, count ? kept_d.p : nullptr),
-               "reduce");
-      });
-    });
+    // (gather_risk units: voxels)
+    prof->run2(
+        "search_eval", (double)na * sh.vol, "eval_reconstruct", (double)na * 3 * sh.p_sel,
+        [&] { evaluate(wl_d.p, na, 3, nullptr, partial.p, count ? partial_k.p : nullptr); }, "gather_risk",
+        (double)na * sh.vol, [&] {
+          hipchk(launch_gather(gather_src(), off.p, invw.p, u64.p, meta_d.p, wl_d.p, na, sh, 0, nullptr, partial_b.p,
+                               s),
+                 "gather");
+          hipchk(launch_risk_reduce(partial.p, eval_blocks(sh, 3), partial_b.p, gather_blocks, na, sums.p, s,
+                                    count ? partial_k.p : nullptr, count ? kept_d.p : nullptr),
+                 "reduce");
+        });
     out.resize((size_t)na * 6);
     hipchk(hipMemcpyAsync(sums_h.p, sums.p, (size_t)na * 6 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H sums");
     if (count)
@@ -2537,6 +2558,7 @@ int pgsvt_denoise_batch(pgsvt_job* jobs, uint32_t n, uint32_t n_lambda, uint32_t
 void pgsvt_profile_enable(int on) {
   std::lock_guard<std::mutex> g(g_prof_mu);
   g_prof_on = on != 0;
+  g_prof_count = on >= 2;
   g_prof.clear();
 }
 
