@@ -5,6 +5,8 @@ within 1e-5 relative L2; PGURE risk within 1e-4 relative; singular values
 within 1e-5 of the largest one of their patch.
 """
 
+import ctypes as C
+import json
 import sys
 
 import numpy as np
@@ -12,6 +14,7 @@ import pytest
 
 import oracle_lib as O
 import pgsvt_lib as G
+from pguresvt import _pguresvt as P
 from pguresvt import phantom
 
 pytestmark = pytest.mark.gpu
@@ -247,3 +250,42 @@ def test_piled_up_trajectories(gpu):
         e = G.rel_l2(Yg, Yo)
         log("  frames rel L2 =", e)
         assert e < FRAME_TOL
+
+
+def _profile_json(L):
+    n = L.pgsvt_profile_read(None, 0)
+    buf = C.create_string_buffer(n)
+    assert L.pgsvt_profile_read(buf, n) == 0
+    return json.loads(buf.value.decode())
+
+
+def test_profiling_leaves_results_unchanged(gpu, seq64):
+    # the profiler (events around the stages; at level 2 also the count of
+    # the ranks each evaluation keeps) must not change a single output value
+    L = P.load()
+    L.pgsvt_profile_enable.argtypes = [C.c_int]
+    L.pgsvt_profile_read.argtypes = [C.c_char_p, C.c_uint64]
+    Y0, e0 = G.denoise(seq64)
+    try:
+        for level in (1, 2):
+            log("profile level", level)
+            L.pgsvt_profile_enable(level)
+            Y, e = G.denoise(seq64)
+            prof = _profile_json(L)
+            L.pgsvt_profile_enable(0)
+            assert np.array_equal(Y, Y0) and np.array_equal(e, e0)
+            for stage in ("search_eval", "eval_reconstruct", "gather_risk", "svd_jacobi"):
+                assert prof[stage]["launches"] > 0 and prof[stage]["ms"] > 0, stage
+            # a round's two stages are timed within the round
+            assert prof["eval_reconstruct"]["ms"] + prof["gather_risk"]["ms"] <= prof["search_eval"]["ms"] * (1 + 1e-6)
+            assert prof["search_eval"]["launches"] == prof["gather_risk"]["launches"]
+            if level == 1:
+                assert "eval_kept_ranks" not in prof
+            else:
+                # every patch-variant of every evaluation keeps between 0 and
+                # nw ranks
+                kept, pv = prof["eval_kept_ranks"]["units"], prof["eval_reconstruct"]["units"]
+                log("  kept ranks %.0f over %.0f patch-variants" % (kept, pv))
+                assert 0 < kept <= 15 * pv
+    finally:
+        L.pgsvt_profile_enable(0)
