@@ -66,9 +66,11 @@ std::mutex g_fallback_mu;
 double g_probe_delta = 0.1, g_probe_gap = 0.01;
 uint32_t g_probe_refine = 2;
 thread_local int g_last_fallbacks = 0;
-// probe sample: every kProbeStride-th selected patch in all three variants,
-// the data terms over kProbeFrames frames spread over the window
-constexpr uint32_t kProbeStride = 32, kProbeFrames = 5;
+// probe sample: every s-th selected patch in all three variants, s at most
+// kProbeMaxStride and small enough to keep kProbeMinSample patches (the
+// divergence sum is scaled up from the sample), the data terms over
+// kProbeFrames frames spread over the window
+constexpr uint32_t kProbeMaxStride = 64, kProbeMinSample = 1000, kProbeFrames = 5;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -706,6 +708,7 @@ struct Pipeline {
   // learned-lambda probe: the patch subsample (sh_s: p_sel = its size) with
   // its three-variant factors and divergence factors, per-slot lambdas
   bool probe_on = false;
+  uint32_t probe_stride = 1;
   Shape sh_s;
   DBuf<uint32_t> sel_s, wl_s;
   DBuf<char> factors_s, divq_s;
@@ -1000,7 +1003,8 @@ struct Pipeline {
   // the learned-lambda probe's patch subsample (before alloc_batch)
   void setup_probe() {
     std::vector<uint32_t> sub;
-    for (uint32_t q = 0; q < sh.p_sel; q += kProbeStride) sub.push_back(sel[q]);
+    probe_stride = std::max(1u, std::min(kProbeMaxStride, sh.p_sel / kProbeMinSample));
+    for (uint32_t q = 0; q < sh.p_sel; q += probe_stride) sub.push_back(sel[q]);
     sh_s = sh;
     sh_s.p_sel = (uint32_t)sub.size();
     sel_s.alloc(sub.size());
@@ -1015,7 +1019,7 @@ struct Pipeline {
   void decompose_probe(uint32_t nb) {
     prof->run("probe_decompose", (double)nb * 2 * sh_s.p_sel, [&] {
       const size_t fw = sh.fstride;  // 32-bit words per factor block (FP32)
-      hipchk(launch_copy_rows(factors.p, factors_s.p, sh_s.p_sel, (uint32_t)fw, kProbeStride,
+      hipchk(launch_copy_rows(factors.p, factors_s.p, sh_s.p_sel, (uint32_t)fw, probe_stride,
                               (size_t)nvar * sh.p_sel * fw, (size_t)3 * sh_s.p_sel * fw, nb, s),
              "probe U factors");
       hipchk(launch_svd(uvar.p + sizeof(float) * sh.vol, pos.p, sel_s.p, nb, 2, 3, sh_s,
