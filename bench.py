@@ -481,6 +481,9 @@ def main():
                     help="timed calls with host buffers (H2D and D2H included), median reported")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-lambda comparison")
+    ap.add_argument("--learned-only", type=int, metavar="SEED", default=None,
+                    help="time only the learned-lambda path (with its probe and fallback) on the held-out "
+                         "sequence of that phantom seed, and print its stage times (a profiling target)")
     ap.add_argument("--batch-mix", type=int, default=8, metavar="S",
                     help="sequences of the batched uneven workload timed for the batch_mix field (0: skip)")
     ap.add_argument("--copy-probe", type=int, metavar="MIB", default=None,
@@ -500,6 +503,20 @@ def main():
         print("bench.py runs on one GPU only: the path shards by output frame across GPUs with no exchange, "
               "but this build measures a single MI355X (got --gpus %d)" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.learned_only is not None:
+        from pguresvt import learned as Lr
+        from pguresvt import phantom
+
+        model = Lr.LambdaModel.load(Lr.default_model_path())
+        Lr.set_fallback()
+        _, noisy = phantom.noisy_sequence(N, FRAMES, blobs=8, drift=2, alpha=1.0, mu=0.0, sigma=8.0,
+                                          seed=a.learned_only)
+        dt, prof, _, _ = run_gpu(noisy, a.steps, a.warmup, model)
+        print(json.dumps({"metric": "learned-lambda frames/s", "value": FRAMES * a.steps / dt, "unit": "frames/s",
+                          "seed": a.learned_only, "steps": a.steps, "warmup": a.warmup,
+                          "fallback_windows": Lr.last_fallbacks(),
+                          "kernels_ms_per_step": {k: v["ms"] / a.steps for k, v in prof.items()}}))
+        return
     clean, noisy = workload()
     dt, prof, Y, est = run_gpu(noisy, a.steps, a.warmup)
     step_times = run_gpu.step_times
